@@ -1,0 +1,285 @@
+"""MI355X-native AutoGrid4 grid-map engine.
+
+Host mirror of the autogrid4 interface over the C-ABI of
+``libautogrid_amd.so``:
+
+* :func:`run_gpf` -- the whole drop-in (``autogrid4 -p x.gpf -l x.glg``):
+  GPF/PDBQT in, ``.map``/``.fld``/``.xyz`` out (main.cpp:214-3582).
+* :class:`Job` -- the front end: GPF keyword semantics (main.cpp:778-1819),
+  tables and bond vectors (main.cpp:1607-1639, 2040-2736) as numpy arrays.
+* :class:`Context` -- one GPU; :meth:`Context.maps_plain` runs the per-point
+  map computation (main.cpp:2805-3431) on the device.
+
+Everything numeric runs in the native library; this module only moves
+arrays.  It raises when the library is missing -- there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from ._abi import AgError, check, lib
+
+__all__ = ["AgError", "Context", "Job", "PreparedState", "run_gpf", "format_map_values",
+           "LIB_PATH", "CLI_PATH"]
+
+LIB_PATH = _abi.LIB_PATH
+CLI_PATH = _abi.CLI_PATH
+
+
+@dataclass
+class PreparedState:
+    """The seam state of include/autogrid_amd.h as numpy arrays."""
+    coord: np.ndarray            # [n,3] float64, grid frame
+    charge: np.ndarray           # [n] float64
+    vol: np.ndarray
+    solpar: np.ndarray
+    type: np.ndarray             # [n] int32
+    hbond: np.ndarray            # [n] int32
+    rexp: np.ndarray             # [n] int32
+    rvector: np.ndarray          # [n,3] float64
+    rvector2: np.ndarray
+    disorder: np.ndarray         # [n] uint8
+    n1: tuple
+    ne: tuple
+    spacing: float
+    maps: np.ndarray             # structured: is_covalent,is_hbonder,hbond,solpar_probe,vol_probe
+    elec_index: int
+    dsolv_index: int
+    n_maps: int
+    separate_desolvation: int
+    disorder_h: int
+    hydrogen_type: int
+    dddiel: int
+    inv_diel_cal: float
+    coeff_estat: float
+    solpar_q: float
+    e_vdw_hb: np.ndarray         # [T, M, 1024] float32
+    sol_fn: np.ndarray           # [8192] float64
+    r_eps: np.ndarray            # [8192] float64
+    cov_pos: tuple = (0.0, 0.0, 0.0)
+    cov_half_width: float = 1.0
+    cov_barrier: float = 1000.0
+    floating_grid: int = 0
+    constriction_grid: int = 0
+    constriction_cutoff: float = 7.5
+    # ag_setup fields (what the table / bond-vector builders read)
+    pairs: np.ndarray | None = None
+    r_smooth: float = 0.5
+    coeff_desolv: float = 0.1322
+    t_types: dict = field(default_factory=dict)
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n_atoms(self) -> int:
+        return int(self.coord.shape[0])
+
+    @property
+    def points(self) -> int:
+        return int(self.n1[0]) * int(self.n1[1]) * int(self.n1[2])
+
+    @property
+    def n_atom_maps(self) -> int:
+        return int(len(self.maps))
+
+    def structs(self):
+        """ctypes (Receptor, Grid, MapSet, Tables, Setup) pointing into this state."""
+        arrs = dict(
+            coord=np.ascontiguousarray(self.coord, np.float64), charge=np.ascontiguousarray(self.charge, np.float64),
+            vol=np.ascontiguousarray(self.vol, np.float64), solpar=np.ascontiguousarray(self.solpar, np.float64),
+            type=np.ascontiguousarray(self.type, np.int32), hbond=np.ascontiguousarray(self.hbond, np.int32),
+            rexp=np.ascontiguousarray(self.rexp, np.int32), rvector=np.ascontiguousarray(self.rvector, np.float64),
+            rvector2=np.ascontiguousarray(self.rvector2, np.float64),
+            disorder=np.ascontiguousarray(self.disorder, np.uint8),
+            e_vdw_hb=np.ascontiguousarray(self.e_vdw_hb, np.float32),
+            sol_fn=np.ascontiguousarray(self.sol_fn, np.float64), r_eps=np.ascontiguousarray(self.r_eps, np.float64))
+        P = _abi._ptr
+        rec = _abi.Receptor(self.n_atoms, P(arrs["coord"], C.c_double), P(arrs["charge"], C.c_double),
+                            P(arrs["vol"], C.c_double), P(arrs["solpar"], C.c_double),
+                            P(arrs["type"], C.c_int32), P(arrs["hbond"], C.c_int32),
+                            P(arrs["rexp"], C.c_int32), P(arrs["rvector"], C.c_double),
+                            P(arrs["rvector2"], C.c_double), P(arrs["disorder"], C.c_uint8))
+        grid = _abi.Grid((C.c_int32 * 3)(*self.n1), (C.c_int32 * 3)(*self.ne), float(self.spacing))
+        md = (_abi.MapDesc * max(1, self.n_atom_maps))()
+        for i, m in enumerate(self.maps):
+            md[i] = _abi.MapDesc(int(m["is_covalent"]), int(m["is_hbonder"]), int(m["hbond"]), 0,
+                                 float(m["solpar_probe"]), float(m["vol_probe"]))
+        ms = _abi.MapSet(self.n_atom_maps, C.cast(md, C.POINTER(_abi.MapDesc)), self.elec_index, self.dsolv_index,
+                         self.n_maps, self.separate_desolvation, self.disorder_h, self.hydrogen_type,
+                         self.dddiel, self.floating_grid, self.constriction_grid, 0, self.inv_diel_cal,
+                         self.coeff_estat, self.solpar_q, (C.c_double * 3)(*self.cov_pos),
+                         self.cov_half_width, self.cov_barrier, self.constriction_cutoff)
+        tab = _abi.Tables(int(self.e_vdw_hb.shape[0]), 0, P(arrs["e_vdw_hb"], C.c_float),
+                          P(arrs["sol_fn"], C.c_double), P(arrs["r_eps"], C.c_double))
+        setup = _abi.Setup()
+        setup.n_rec_types = int(self.e_vdw_hb.shape[0])
+        setup.n_atom_maps = self.n_atom_maps
+        keep = [arrs, md]
+        if self.pairs is not None:
+            pa = (_abi.Pair * max(1, self.pairs.size))()
+            for i, p in enumerate(self.pairs.reshape(-1)):
+                pa[i] = _abi.Pair(*[p[k].item() for k in ("cA", "cB", "Rij", "eps", "xA", "xB", "hbonder")], 0)
+            setup.pairs = C.cast(pa, C.POINTER(_abi.Pair))
+            keep.append(pa)
+        setup.r_smooth = self.r_smooth
+        setup.coeff_desolv = self.coeff_desolv
+        setup.dddiel = self.dddiel
+        setup.disorder_h = self.disorder_h
+        for k in ("t_hydrogen", "t_nonhb_hydrogen", "t_carbon", "t_arom_carbon", "t_oxygen", "t_sulphur",
+                  "t_nitrogen", "t_nonhb_nitrogen"):
+            setattr(setup, k, int(self.t_types.get(k, -1)))
+        self._keep[:] = keep   # the structs point into these arrays
+        return rec, grid, ms, tab, setup
+
+
+MAP_DTYPE = np.dtype([("is_covalent", np.int32), ("is_hbonder", np.int32), ("hbond", np.int32),
+                      ("solpar_probe", np.float64), ("vol_probe", np.float64)])
+PAIR_DTYPE = np.dtype([("cA", np.float64), ("cB", np.float64), ("Rij", np.float64), ("eps", np.float64),
+                       ("xA", np.int32), ("xB", np.int32), ("hbonder", np.int32)])
+
+
+def _arr(ptr, n, dtype, shape=None):
+    if n == 0 or not ptr:
+        return np.zeros(shape if shape else (0,), dtype)
+    a = np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+    return a.reshape(shape) if shape else a
+
+
+class Job:
+    """A GPF read by the engine's front end (ag_job_open)."""
+
+    def __init__(self, gpf_path: str, glg_path: str | None = None, program: str = "autogrid4",
+                 files: bool = False):
+        h = C.c_void_p()
+        flags = 0 if files else _abi.AG_JOB_NO_FILES
+        check(lib().ag_job_open(os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
+                                program.encode(), flags, C.byref(h)))
+        self._h = h
+        self.prepared = False
+
+    def close(self):
+        if self._h:
+            lib().ag_job_close(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def prepare(self) -> "Job":
+        check(lib().ag_job_prepare(self._h))
+        self.prepared = True
+        return self
+
+    @property
+    def points(self) -> int:
+        return int(lib().ag_job_points(self._h))
+
+    def state(self) -> PreparedState:
+        """Copy of the parsed (and, after prepare(), prepared) state."""
+        rec, grid, ms, tab, setup = (_abi.Receptor(), _abi.Grid(), _abi.MapSet(), _abi.Tables(), _abi.Setup())
+        check(lib().ag_job_views(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab), C.byref(setup)))
+        n = rec.n_atoms
+        T, M = tab.n_rec_types, ms.n_atom_maps
+        maps = np.zeros(M, MAP_DTYPE)
+        if M and ms.maps:
+            for i in range(M):
+                d = ms.maps[i]
+                maps[i] = (d.is_covalent, d.is_hbonder, d.hbond, d.solpar_probe, d.vol_probe)
+        pairs = None
+        if setup.pairs and T * M:
+            pairs = np.zeros((M, T), PAIR_DTYPE)
+            for i in range(M * T):
+                p = setup.pairs[i]
+                pairs.reshape(-1)[i] = (p.cA, p.cB, p.Rij, p.eps, p.xA, p.xB, p.hbonder)
+        return PreparedState(
+            coord=_arr(rec.coord, 3 * n, np.float64, (n, 3)), charge=_arr(rec.charge, n, np.float64),
+            vol=_arr(rec.vol, n, np.float64), solpar=_arr(rec.solpar, n, np.float64),
+            type=_arr(rec.type, n, np.int32), hbond=_arr(rec.hbond, n, np.int32),
+            rexp=_arr(rec.rexp, n, np.int32), rvector=_arr(rec.rvector, 3 * n, np.float64, (n, 3)),
+            rvector2=_arr(rec.rvector2, 3 * n, np.float64, (n, 3)), disorder=_arr(rec.disorder, n, np.uint8),
+            n1=tuple(grid.n1), ne=tuple(grid.ne), spacing=grid.spacing, maps=maps,
+            elec_index=ms.elec_index, dsolv_index=ms.dsolv_index, n_maps=ms.n_maps,
+            separate_desolvation=ms.separate_desolvation, disorder_h=ms.disorder_h,
+            hydrogen_type=ms.hydrogen_type, dddiel=ms.dddiel, inv_diel_cal=ms.inv_diel_cal,
+            coeff_estat=ms.coeff_estat, solpar_q=ms.solpar_q,
+            e_vdw_hb=_arr(tab.e_vdw_hb, T * M * _abi.AG_NEINT, np.float32, (T, M, _abi.AG_NEINT)),
+            sol_fn=_arr(tab.sol_fn, _abi.AG_NDIEL, np.float64), r_eps=_arr(tab.r_eps, _abi.AG_NDIEL, np.float64),
+            cov_pos=tuple(ms.cov_pos), cov_half_width=ms.cov_half_width, cov_barrier=ms.cov_barrier,
+            floating_grid=ms.floating_grid, constriction_grid=ms.constriction_grid,
+            constriction_cutoff=ms.constriction_cutoff, pairs=pairs, r_smooth=setup.r_smooth,
+            coeff_desolv=setup.coeff_desolv,
+            t_types={k: getattr(setup, k) for k in ("t_hydrogen", "t_nonhb_hydrogen", "t_carbon", "t_arom_carbon",
+                                                    "t_oxygen", "t_sulphur", "t_nitrogen", "t_nonhb_nitrogen")})
+
+
+class Context:
+    """One GPU and its stream (ag_context_create)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().ag_context_create(C.byref(h), int(device)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ag_context_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def maps_plain(self, st: PreparedState, out: np.ndarray | None = None) -> np.ndarray:
+        """[n_maps, n1z, n1y, n1x] float32 maps for the state, computed on the GPU."""
+        rec, grid, ms, tab, _ = st.structs()
+        nz, ny, nx = st.n1[2], st.n1[1], st.n1[0]
+        if out is None:
+            out = np.empty((st.n_maps, nz, ny, nx), np.float32)
+        assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == st.n_maps * st.points
+        check(lib().ag_maps_plain(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
+                                  out.ctypes.data, None, None, 0))
+        return out
+
+    def maps_plain_device(self, st: PreparedState, out_ptr: int) -> None:
+        """Same, writing into device memory at out_ptr ([n_maps][P] float32)."""
+        rec, grid, ms, tab, _ = st.structs()
+        check(lib().ag_maps_plain(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
+                                  C.c_void_p(out_ptr), None, None, 1))
+
+    def run_gpf(self, gpf_path: str, glg_path: str | None = None, program: str = "autogrid4") -> None:
+        check(lib().ag_run_gpf(self._h, os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
+                               program.encode()))
+
+
+def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program: str = "autogrid4") -> None:
+    """autogrid4 -p gpf -l glg, on one GPU."""
+    with Context(device) as ctx:
+        ctx.run_gpf(gpf_path, glg_path, program)
+
+
+def format_map_values(values: np.ndarray) -> bytes:
+    """The reference's map text for float32 values (one value per line)."""
+    v = np.ascontiguousarray(values, np.float32).reshape(-1)
+    cap = 16 * v.size + 16
+    buf = C.create_string_buffer(cap)
+    n = lib().ag_format_map_values(v.ctypes.data, v.size, buf, cap)
+    if n < 0:
+        raise AgError(1, "format buffer too small")
+    return buf.raw[:n]

@@ -1,0 +1,127 @@
+"""ctypes binding of the C-ABI declared in include/autogrid_amd.h.
+
+The engine is the in-tree shared library ``autogrid_amd/libautogrid_amd.so``
+(C++ host front end + gfx950 HIP kernels).  Loading it fails loudly when it
+has not been built; there is no Python or CPU fallback for the map
+computation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libautogrid_amd.so")
+CLI_PATH = os.path.join(_HERE, "bin", "autogrid4")
+
+AG_OK = 0
+STATUS_NAMES = {0: "AG_OK", 1: "AG_ERR_BAD_INPUT", 2: "AG_ERR_NO_DONOR_H", 3: "AG_ERR_OOM",
+                4: "AG_ERR_DEVICE", 5: "AG_ERR_IO", 6: "AG_ERR_UNSUPPORTED"}
+AG_JOB_NO_FILES = 1
+AG_NEINT = 1024
+AG_NDIEL = 8192
+
+
+class AgError(RuntimeError):
+    """A non-OK ag_status, with the library's message."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Receptor(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("coord", C.POINTER(C.c_double)),
+                ("charge", C.POINTER(C.c_double)), ("vol", C.POINTER(C.c_double)),
+                ("solpar", C.POINTER(C.c_double)), ("type", C.POINTER(C.c_int32)),
+                ("hbond", C.POINTER(C.c_int32)), ("rexp", C.POINTER(C.c_int32)),
+                ("rvector", C.POINTER(C.c_double)), ("rvector2", C.POINTER(C.c_double)),
+                ("disorder", C.POINTER(C.c_uint8))]
+
+
+class Grid(C.Structure):
+    _fields_ = [("n1", C.c_int32 * 3), ("ne", C.c_int32 * 3), ("spacing", C.c_double)]
+
+
+class MapDesc(C.Structure):
+    _fields_ = [("is_covalent", C.c_int32), ("is_hbonder", C.c_int32), ("hbond", C.c_int32),
+                ("reserved", C.c_int32), ("solpar_probe", C.c_double), ("vol_probe", C.c_double)]
+
+
+class MapSet(C.Structure):
+    _fields_ = [("n_atom_maps", C.c_int32), ("maps", C.POINTER(MapDesc)),
+                ("elec_index", C.c_int32), ("dsolv_index", C.c_int32), ("n_maps", C.c_int32),
+                ("separate_desolvation", C.c_int32), ("disorder_h", C.c_int32),
+                ("hydrogen_type", C.c_int32), ("dddiel", C.c_int32), ("floating_grid", C.c_int32),
+                ("constriction_grid", C.c_int32), ("reserved", C.c_int32),
+                ("inv_diel_cal", C.c_double), ("coeff_estat", C.c_double), ("solpar_q", C.c_double),
+                ("cov_pos", C.c_double * 3), ("cov_half_width", C.c_double),
+                ("cov_barrier", C.c_double), ("constriction_cutoff", C.c_double)]
+
+
+class Tables(C.Structure):
+    _fields_ = [("n_rec_types", C.c_int32), ("reserved", C.c_int32),
+                ("e_vdw_hb", C.POINTER(C.c_float)), ("sol_fn", C.POINTER(C.c_double)),
+                ("r_eps", C.POINTER(C.c_double))]
+
+
+class Pair(C.Structure):
+    _fields_ = [("cA", C.c_double), ("cB", C.c_double), ("Rij", C.c_double), ("eps", C.c_double),
+                ("xA", C.c_int32), ("xB", C.c_int32), ("hbonder", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Setup(C.Structure):
+    _fields_ = [("n_rec_types", C.c_int32), ("n_atom_maps", C.c_int32),
+                ("pairs", C.POINTER(Pair)), ("r_smooth", C.c_double), ("coeff_desolv", C.c_double),
+                ("dddiel", C.c_int32), ("disorder_h", C.c_int32),
+                ("t_hydrogen", C.c_int32), ("t_nonhb_hydrogen", C.c_int32), ("t_carbon", C.c_int32),
+                ("t_arom_carbon", C.c_int32), ("t_oxygen", C.c_int32), ("t_sulphur", C.c_int32),
+                ("t_nitrogen", C.c_int32), ("t_nonhb_nitrogen", C.c_int32)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(the engine has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.ag_last_error.restype = C.c_char_p
+        L.ag_abi_version.restype = C.c_int
+        L.ag_context_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.ag_context_destroy.argtypes = [vp]
+        L.ag_job_open.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp)]
+        L.ag_job_prepare.argtypes = [vp]
+        L.ag_job_views.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
+                                   C.POINTER(Tables), C.POINTER(Setup)]
+        L.ag_job_points.argtypes = [vp]
+        L.ag_job_points.restype = C.c_int64
+        L.ag_job_close.argtypes = [vp]
+        L.ag_job_close.restype = None
+        L.ag_run_gpf.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p]
+        L.ag_maps_plain.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
+                                    C.POINTER(Tables), vp, vp, vp, C.c_int]
+        L.ag_format_map_values.argtypes = [vp, C.c_int64, C.c_char_p, C.c_int64]
+        L.ag_format_map_values.restype = C.c_int64
+        L.ag_job_write_maps.argtypes = [vp, vp, vp, vp, C.c_int]
+        for name in ("ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",
+                     "ag_job_views", "ag_run_gpf", "ag_maps_plain", "ag_job_write_maps"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != AG_OK:
+        raise AgError(status, lib().ag_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,1045 @@
+// Device side of the MI355X AutoGrid4 engine (gfx950, wave64).
+//
+// One call of ag_maps_plain runs, on the context's stream:
+//   1. cell list   : atoms within 8 A of the box binned into cubic cells by a
+//                    counting sort; each cell's atoms are then ordered by atom
+//                    index so every later traversal is deterministic.
+//   2. near kernel : one workgroup per 8x8x4 brick of grid points.  It stages
+//                    the atoms within 8.03 A of the brick (cells in fixed
+//                    order, ballot compaction) and evaluates, per point, the
+//                    8 A part of the reference's per-point loop
+//                    (main.cpp:2866-3431): covalent maps, closest donor H,
+//                    directional H-bond factors, vdW/H-bond table terms,
+//                    desolvation, the d map, the H-bond min/max fix-up, and
+//                    the electrostatic terms of every atom the far kernel
+//                    leaves out.  Bin indices floor(128 r) are taken from an
+//                    FP32 distance unless 128 r lies within 5e-4 of an
+//                    integer, where the reference's FP64 distance is
+//                    recomputed, so the table bins match the reference's.
+//   3. far kernel  : the electrostatic sum over all atoms (main.cpp:2940-2982)
+//                    for every pair whose FP32 bin is >= 1025 (r > 8 A); the
+//                    dielectric table sits in LDS with zeros below bin 1025,
+//                    atoms stream through scalar registers, each lane owns a
+//                    run of 8 points along x, and the partial sums of every
+//                    64-atom chunk are added with Kahan compensation onto
+//                    the near kernel's value.
+// No atomics touch map values; every sum has a fixed order.
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ag_internal.h"
+
+#define HIP_CHECK(x)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess)                                                           \
+            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                        " at " #x);                                     \
+    } while (0)
+
+namespace {
+
+constexpr int kBX = 8, kBY = 8, kBZ = 4;          // brick of the near kernel
+constexpr int kNearThreads = kBX * kBY * kBZ;     // 256
+constexpr int kMapSlots = 16;                     // maps per near-kernel pass
+constexpr int kCandCap = 2048;                    // candidates per brick
+constexpr int kHCap = 512;                        // donor-H candidates per brick
+constexpr int kChunk = 256;                       // candidate records staged per pass
+constexpr float kNearRadius = 8.03f;              // staging radius (A)
+constexpr int kFarK = 8;                          // points per lane, far kernel
+constexpr int kFarThreads = 256;
+constexpr int kFarAtomChunk = 64;                 // Kahan block
+constexpr int kFarBinMin = 1025;                  // far kernel takes bins >= this
+constexpr float kFarBias = 1.0e-3f;               // added to (128 r)^2, keeps rsq finite
+constexpr float kGuardBins = 5.0e-4f;             // FP64 recompute band around bin edges
+
+// Map roles of the near kernel.
+enum : int { ROLE_NONE = 0, ROLE_AFF = 1, ROLE_DES = 2, ROLE_COV = 4 };
+// H-bond cases (main.cpp:3286-3390).
+enum : int { HBC_PLAIN = 0, HBC_ACC_PROBE = 1, HBC_MINMAX_RACC = 2, HBC_DON_PROBE = 3 };
+
+struct __align__(16) NearAtom {   // 96 bytes
+    double x, y, z;               // grid frame, as the reference holds them
+    float qe;                     // charge * coeff_estat
+    float vol;
+    float solq;                   // solpar + solpar_q * |charge|
+    float dvol;                   // solpar_q * vol (d map)
+    float rv[3];
+    int meta;                     // type | hbond << 8 | rexp << 16 | disorder << 24
+    float rv2[3];
+    int index;                    // atom index in the receptor
+    float xE, yE, zE;             // the far kernel's coordinates (128 * coord, float)
+    float pad;
+};
+static_assert(sizeof(NearAtom) == 96, "NearAtom layout");
+
+struct NearParams {
+    int n1[3], ne[3];
+    double spacing;
+    float s128;                   // (float)(spacing * 128)
+    int nb[3], nbricks;
+    // cell list
+    double cell_origin[3];
+    double cell_size;
+    int nc[3];
+    const int *cell_start;        // [ncells + 1]
+    const NearAtom *atoms;        // sorted by (cell, index)
+    const double *coord64;        // [n][3] by atom index
+    const double *rvec64;         // [n][3]
+    const double *rvec2_64;       // [n][3]
+    // tables
+    const float *tab;             // [T][1024][kMapSlots] for this pass's maps
+    const float *sol;             // [1025]
+    const float *reps;            // [8192] electrostatic factor (r_eps or 332/diel)
+    // maps of this pass
+    int nslots;
+    int role[kMapSlots];
+    int hbc[kMapSlots];           // 4 bits per atom hbond class 0..6
+    float sp[kMapSlots];          // solpar_probe
+    float volp[kMapSlots];        // vol_probe
+    int out_slot[kMapSlots];      // output slot of each map
+    int e_slot, d_slot;           // -1 when not produced in this pass
+    int has_affinity;             // any AFF/DES slot, or d map
+    int disorder_h, hydrogen_type;
+    double cov_pos[3], cov_hw, cov_barrier;
+    float *maps;
+    long long P;
+    int *status;
+};
+
+struct FarParams {
+    int n1[3], ne[3];
+    float s128;
+    int runs_per_row, nruns;
+    const float4 *atoms;          // (xE, yE, zE, w = q * coeff_estat * 128)
+    int n_atoms;
+    const float *tab;             // [tab_len]
+    int tab_len;
+    float tmax;                   // tab_len - 1
+    float *emap;                  // e map: in = near part, out = total
+};
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    // Consecutive workgroup ids go round-robin over the 8 XCDs; give each XCD
+    // a contiguous range of tiles so neighbouring tiles share its L2.  XCD x
+    // receives q + (x < r) workgroups and owns tiles [x*q + min(x, r), ...).
+    const int q = nblocks / 8, r = nblocks % 8;
+    const int x = bid % 8, k = bid / 8;
+    return x * q + min(x, r) + k;
+}
+
+// The far kernel's per-pair bin input, reproduced bit for bit by the near
+// kernel when it decides which atoms the far kernel left out.
+__device__ __forceinline__ float far_rb(float ax, float ay, float az, float px, float py, float pz) {
+#pragma clang fp contract(off)
+    const float dy = ay - py, dz = az - pz;
+    const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
+    const float dx = ax - px;
+    const float rb2 = __builtin_fmaf(dx, dx, dyz2);
+    return rb2 * __builtin_amdgcn_rsqf(rb2);
+}
+
+// ---------------------------------------------------------------------------
+// cell list
+// ---------------------------------------------------------------------------
+
+struct CellParams {
+    double origin[3];
+    double cs;
+    int nc[3];
+    int n_atoms;
+};
+
+__device__ __forceinline__ int cell_of(const CellParams &c, const double *p) {
+    int ix = (int)floor((p[0] - c.origin[0]) / c.cs);
+    int iy = (int)floor((p[1] - c.origin[1]) / c.cs);
+    int iz = (int)floor((p[2] - c.origin[2]) / c.cs);
+    if (ix < 0 || iy < 0 || iz < 0 || ix >= c.nc[0] || iy >= c.nc[1] || iz >= c.nc[2]) return -1;
+    return ix + c.nc[0] * (iy + c.nc[1] * iz);
+}
+
+__global__ void k_cell_count(CellParams c, const double *coord, int *atom_cell, int *count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n_atoms) return;
+    int cell = cell_of(c, coord + 3 * (size_t)i);
+    atom_cell[i] = cell;
+    if (cell >= 0) atomicAdd(&count[cell], 1);
+}
+
+// Exclusive scan of count[0..n) into start[0..n], one workgroup of 1024.
+__global__ void __launch_bounds__(1024) k_scan(const int *count, int *start, int n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(n, b + per);
+    int s = 0;
+    for (int i = b; i < e; i++) s += count[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int i = b; i < e; i++) {
+        start[i] = run;
+        run += count[i];
+    }
+    if (t == 1023) start[n] = part[1023];
+}
+
+__global__ void k_cell_scatter(int n, const int *atom_cell, const int *start, int *cursor, int *slot_atom) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int cell = atom_cell[i];
+    if (cell < 0) return;
+    int pos = start[cell] + atomicAdd(&cursor[cell], 1);
+    slot_atom[pos] = i;
+}
+
+// Order each cell's atoms by atom index (insertion sort; cells hold few atoms).
+__global__ void k_cell_sort(int ncells, const int *start, int *slot_atom) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int b = start[c], e = start[c + 1];
+    for (int i = b + 1; i < e; i++) {
+        int v = slot_atom[i];
+        int j = i - 1;
+        while (j >= b && slot_atom[j] > v) {
+            slot_atom[j + 1] = slot_atom[j];
+            j--;
+        }
+        slot_atom[j + 1] = v;
+    }
+}
+
+struct AtomSrc {
+    const double *coord, *charge, *vol, *solpar, *rvec, *rvec2;
+    const int *type, *hbond, *rexp;
+    const unsigned char *disorder;
+    double coeff_estat, solpar_q;
+};
+
+// nslots = start[ncells], the number of atoms that landed in a cell.
+__global__ void k_gather(const int *nslots, const int *slot_atom, AtomSrc s, NearAtom *out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *nslots) return;
+    int i = slot_atom[k];
+    NearAtom a;
+    a.x = s.coord[3 * i];
+    a.y = s.coord[3 * i + 1];
+    a.z = s.coord[3 * i + 2];
+    double q = s.charge[i];
+    a.qe = (float)(q * s.coeff_estat);
+    a.vol = (float)s.vol[i];
+    a.solq = (float)(s.solpar[i] + s.solpar_q * fabs(q));
+    a.dvol = (float)(s.solpar_q * s.vol[i]);
+    for (int d = 0; d < 3; d++) {
+        a.rv[d] = (float)s.rvec[3 * i + d];
+        a.rv2[d] = (float)s.rvec2[3 * i + d];
+    }
+    a.meta = (s.type[i] & 0xff) | ((s.hbond[i] & 0xff) << 8) | ((s.rexp[i] & 0xff) << 16) |
+             ((s.disorder[i] ? 1 : 0) << 24);
+    a.index = i;
+    a.xE = (float)(a.x * 128.0);
+    a.yE = (float)(a.y * 128.0);
+    a.zE = (float)(a.z * 128.0);
+    a.pad = 0.f;
+    out[k] = a;
+}
+
+__global__ void k_far_atoms(int n, const double *coord, const double *charge, double coeff_estat,
+                            float4 *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = make_float4((float)(coord[3 * i] * 128.0), (float)(coord[3 * i + 1] * 128.0),
+                         (float)(coord[3 * i + 2] * 128.0), (float)(charge[i] * coeff_estat * 128.0));
+}
+
+// ---------------------------------------------------------------------------
+// near kernel
+// ---------------------------------------------------------------------------
+
+// Block-wide ordered compaction helper: returns this thread's output position
+// among flagged threads and the block total (256 threads = 4 waves).
+__device__ __forceinline__ int block_compact(bool flag, int *wave_tot, int &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(flag);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = __popcll(m);
+    __syncthreads();
+    int base = 0;
+    total = 0;
+    for (int w = 0; w < kNearThreads / 64; w++) {
+        if (w < wave) base += wave_tot[w];
+        total += wave_tot[w];
+    }
+    __syncthreads();
+    return base + before;
+}
+
+__device__ __forceinline__ double ref_dist(double ax, double ay, double az, double cx, double cy, double cz,
+                                           double &dx, double &dy, double &dz) {
+#pragma clang fp contract(off)
+    // hypotenuse(): sqrt(dx*dx + dy*dy + dz*dz), no fused operations
+    dx = ax - cx;
+    dy = ay - cy;
+    dz = az - cz;
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
+    __shared__ int cand[kCandCap];
+    __shared__ int hcand[kHCap];
+    __shared__ NearAtom rec[kChunk];
+    __shared__ int wave_tot[kNearThreads / 64];
+    __shared__ int cell_off[kNearThreads + 1];
+    __shared__ int cstart_s[kNearThreads];
+    __shared__ int ncand_s, nh_s;
+
+    const int tid = threadIdx.x;
+    const int brick = xcd_remap(blockIdx.x, gridDim.x);
+    const int bx = brick % p.nb[0];
+    const int by = (brick / p.nb[0]) % p.nb[1];
+    const int bz = brick / (p.nb[0] * p.nb[1]);
+    const int ix = bx * kBX + (tid % kBX);
+    const int iy = by * kBY + ((tid / kBX) % kBY);
+    const int iz = bz * kBZ + tid / (kBX * kBY);
+    const bool active = ix < p.n1[0] && iy < p.n1[1] && iz < p.n1[2];
+
+    // brick box in the grid frame
+    const int ix0 = bx * kBX, iy0 = by * kBY, iz0 = bz * kBZ;
+    const int ix1 = min(ix0 + kBX, p.n1[0]) - 1, iy1 = min(iy0 + kBY, p.n1[1]) - 1,
+              iz1 = min(iz0 + kBZ, p.n1[2]) - 1;
+    const double bmin[3] = {(ix0 - p.ne[0]) * p.spacing, (iy0 - p.ne[1]) * p.spacing, (iz0 - p.ne[2]) * p.spacing};
+    const double bmax[3] = {(ix1 - p.ne[0]) * p.spacing, (iy1 - p.ne[1]) * p.spacing, (iz1 - p.ne[2]) * p.spacing};
+
+    // point, reference arithmetic: c = (double)(i - ne) * spacing
+    const double cx = (double)(ix - p.ne[0]) * p.spacing;
+    const double cy = (double)(iy - p.ne[1]) * p.spacing;
+    const double cz = (double)(iz - p.ne[2]) * p.spacing;
+    // the far kernel's point coordinates
+    const float pxE = (float)(ix - p.ne[0]) * p.s128;
+    const float pyE = (float)(iy - p.ne[1]) * p.s128;
+    const float pzE = (float)(iz - p.ne[2]) * p.s128;
+
+    // ---- stage candidate atoms: cells overlapping the brick box + radius ----
+    int c0[3], c1[3];
+    for (int d = 0; d < 3; d++) {
+        c0[d] = (int)floor((bmin[d] - kNearRadius - p.cell_origin[d]) / p.cell_size);
+        c1[d] = (int)floor((bmax[d] + kNearRadius - p.cell_origin[d]) / p.cell_size);
+        c0[d] = max(c0[d], 0);
+        c1[d] = min(c1[d], p.nc[d] - 1);
+    }
+    const int ncx = c1[0] - c0[0] + 1, ncy = c1[1] - c0[1] + 1, ncz = c1[2] - c0[2] + 1;
+    const int ncell = (ncx > 0 && ncy > 0 && ncz > 0) ? ncx * ncy * ncz : 0;
+    if (tid == 0) { ncand_s = 0; nh_s = 0; }
+    __syncthreads();
+    const float r2cut = kNearRadius * kNearRadius;
+    for (int cbase = 0; cbase < ncell; cbase += kNearThreads) {
+        // atom counts of this batch of cells, exclusive scan in cell_off
+        const int ci = cbase + tid;
+        int cstart = 0, ccount = 0;
+        if (ci < ncell) {
+            const int gx = c0[0] + ci % ncx, gy = c0[1] + (ci / ncx) % ncy, gz = c0[2] + ci / (ncx * ncy);
+            const int g = gx + p.nc[0] * (gy + p.nc[1] * gz);
+            cstart = p.cell_start[g];
+            ccount = p.cell_start[g + 1] - cstart;
+        }
+        cell_off[tid] = ccount;
+        __syncthreads();
+        for (int off = 1; off < kNearThreads; off <<= 1) {
+            int v = tid >= off ? cell_off[tid - off] : 0;
+            __syncthreads();
+            cell_off[tid] += v;
+            __syncthreads();
+        }
+        const int total = cell_off[kNearThreads - 1];
+        const int my_end = cell_off[tid];
+        __syncthreads();
+        cell_off[tid + 1] = my_end;
+        if (tid == 0) cell_off[0] = 0;
+        cstart_s[tid] = cstart;
+        __syncthreads();
+        for (int j0 = 0; j0 < total; j0 += kNearThreads) {
+            const int j = j0 + tid;
+            bool ok = false, is_h = false;
+            int slot = -1;
+            if (j < total) {
+                // cell of item j: last c with cell_off[c] <= j
+                int lo = 0, hi = kNearThreads;  // cell_off[hi] > j
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (cell_off[mid] <= j) lo = mid; else hi = mid;
+                }
+                slot = cstart_s[lo] + (j - cell_off[lo]);
+                const NearAtom &a = p.atoms[slot];
+                // distance from the atom to the brick box
+                float dd = 0.f;
+                const double av[3] = {a.x, a.y, a.z};
+                for (int d = 0; d < 3; d++) {
+                    double t = av[d] < bmin[d] ? bmin[d] - av[d] : (av[d] > bmax[d] ? av[d] - bmax[d] : 0.0);
+                    dd += (float)(t * t);
+                }
+                ok = dd <= r2cut;
+                const int hb = (a.meta >> 8) & 0xff;
+                is_h = ok && (hb == AG_HB_DS || hb == AG_HB_D1);
+            }
+            int tot;
+            const int pos = block_compact(ok, wave_tot, tot);
+            const int base = ncand_s;
+            if (ok && base + pos < kCandCap) cand[base + pos] = slot;
+            int htot;
+            const int hpos = block_compact(is_h, wave_tot, htot);
+            const int hbase = nh_s;
+            if (is_h && hbase + hpos < kHCap) hcand[hbase + hpos] = slot;
+            __syncthreads();
+            if (tid == 0) { ncand_s = base + tot; nh_s = hbase + htot; }
+            __syncthreads();
+        }
+    }
+    const int ncand = ncand_s, nh = nh_s;
+    if (ncand > kCandCap || nh > kHCap) {
+        if (tid == 0) atomicOr(p.status, 1);
+        return;
+    }
+
+    // ---- covalent maps (main.cpp:2866-2879) ----
+    float acc[kMapSlots], hmin[kMapSlots], hmax[kMapSlots];
+#pragma unroll
+    for (int m = 0; m < kMapSlots; m++) {
+        acc[m] = 0.f;
+        hmin[m] = 999999.f;
+        hmax[m] = -999999.f;
+    }
+#pragma unroll
+    for (int m = 0; m < kMapSlots; m++) {
+        if (m < p.nslots && (p.role[m] & ROLE_COV)) {
+            double dx = p.cov_pos[0] - cx, dy = p.cov_pos[1] - cy, dz = p.cov_pos[2] - cz;
+            double rcov = sqrt(dx * dx + dy * dy + dz * dz) / p.cov_hw;
+            if (rcov < 1.0e-6) rcov = 1.0e-6;
+            acc[m] = (float)(p.cov_barrier * (1. - exp(-0.69314718055994530942 * rcov * rcov)));
+        }
+    }
+
+    // ---- closest donor H (main.cpp:2985-3015), FP64, lowest index on ties ----
+    double best_r = 999999.0;
+    int best_idx = 0x7fffffff, best_slot = -1;
+    for (int h = 0; h < nh; h++) {
+        const int s = hcand[h];
+        const NearAtom &a = p.atoms[s];
+        double dx, dy, dz;
+        const double r = ref_dist(a.x, a.y, a.z, cx, cy, cz, dx, dy, dz);
+        if (r < best_r || (r == best_r && a.index < best_idx)) {
+            best_r = r;
+            best_idx = a.index;
+            best_slot = s;
+        }
+    }
+    float hrv[3] = {0.f, 0.f, 0.f};
+    if (best_slot >= 0) {
+        const NearAtom &a = p.atoms[best_slot];
+        hrv[0] = a.rv[0]; hrv[1] = a.rv[1]; hrv[2] = a.rv[2];
+    }
+
+    float e_near = 0.f, d_acc = 0.f;
+    const float *tab = p.tab;
+    const float *reps = p.reps;
+
+    for (int cbase = 0; cbase < ncand; cbase += kChunk) {
+        const int nc = min(kChunk, ncand - cbase);
+        __syncthreads();
+        for (int j = tid; j < nc; j += kNearThreads) rec[j] = p.atoms[cand[cbase + j]];
+        __syncthreads();
+        for (int j = 0; j < nc; j++) {
+            const NearAtom &a = rec[j];
+            // geometry: FP64 difference as the reference forms it, FP32 after
+            const double dx64 = a.x - cx, dy64 = a.y - cy, dz64 = a.z - cz;
+            const float dxf = (float)dx64, dyf = (float)dy64, dzf = (float)dz64;
+            const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
+            if (r2 > 64.33f) continue;  // beyond 8.02 A: neither cutoff pass nor near e term
+            float r = sqrtf(r2);
+            float g = r * 128.f;
+            int bin;
+            bool cut;
+            const float frac = g - rintf(g);
+            if (fabsf(frac) < kGuardBins) {
+                double rdx, rdy, rdz;
+                const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                bin = (int)(r64 * 128.0);
+                cut = r64 > 8.0;
+                r = (float)r64;
+            } else {
+                bin = (int)g;
+                cut = r > 8.0f;
+            }
+            // electrostatics of pairs the far kernel leaves out
+            if (p.e_slot >= 0) {
+                bool near_e;
+                if (g < 1022.f) near_e = true;
+                else if (g > 1027.f) near_e = false;
+                else near_e = far_rb(a.xE, a.yE, a.zE, pxE, pyE, pzE) < (float)kFarBinMin;
+                if (near_e) {
+                    const float rr = r < 1.0e-6f ? 1.0e-6f : r;
+                    const float inv_rmax = 1.f / (rr > 0.5f ? rr : 0.5f);
+                    e_near += a.qe * inv_rmax * reps[min(bin, AG_NDIEL - 1)];
+                }
+            }
+            if (cut || !p.has_affinity) continue;
+            const int type = a.meta & 0xff, hb = (a.meta >> 8) & 0xff, rexp = (a.meta >> 16) & 0xff;
+            const bool dis = (a.meta >> 24) & 1;
+            if (p.disorder_h && dis && type == p.hydrogen_type) continue;
+            const float rr = r < 1.0e-6f ? 1.0e-6f : r;
+            const float inv_r = 1.f / rr;
+            const float dn[3] = {dxf * inv_r, dyf * inv_r, dzf * inv_r};
+            const int indx_n = min(bin, AG_NEINT - 1);
+            const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
+            float racc = 1.f, rdon = 1.f, Hramp = 1.f;
+            if (hb == AG_HB_D1) {
+                float cos_t = -(dn[0] * a.rv[0]) - dn[1] * a.rv[1] - dn[2] * a.rv[2];
+                bool pos;
+                if (fabsf(cos_t) < 1.0e-5f) {
+                    double rdx, rdy, rdz;
+                    double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                    if (r64 < 1.0e-6) r64 = 1.0e-6;
+                    const double ir = 1. / r64;
+                    const double *rv64 = p.rvec64 + 3 * (size_t)a.index;
+                    double c = 0.;
+                    c -= rdx * ir * rv64[0];
+                    c -= rdy * ir * rv64[1];
+                    c -= rdz * ir * rv64[2];
+                    pos = c > 0.;
+                    cos_t = (float)c;
+                } else {
+                    pos = cos_t > 0.f;
+                }
+                if (!pos) {
+                    racc = 0.f;
+                } else {
+                    const float c2 = cos_t * cos_t;
+                    racc = rexp == 2 ? c2 : (rexp == 4 ? c2 * c2 : cos_t);
+                    if (a.index != best_idx) {
+                        float ct = hrv[0] * a.rv[0] + hrv[1] * a.rv[1] + hrv[2] * a.rv[2];
+                        ct = fminf(ct, 1.f);
+                        ct = fmaxf(ct, -1.f);
+                        Hramp = 0.5f - 0.5f * cosf(acosf(ct) * (120.f / 90.f));
+                    }
+                }
+            } else if (hb == AG_HB_A1) {
+                const float cos_t = -(dn[0] * a.rv[0]) - dn[1] * a.rv[1] - dn[2] * a.rv[2];
+                rdon = cos_t <= 0.f ? 0.f : cos_t * cos_t;
+            } else if (hb == AG_HB_A2 && !dis) {
+                float cos_t = -(dn[0] * a.rv[0]) - dn[1] * a.rv[1] - dn[2] * a.rv[2];
+                bool front;
+                if (fabsf(cos_t) < 1.0e-5f) {
+                    double rdx, rdy, rdz;
+                    double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                    if (r64 < 1.0e-6) r64 = 1.0e-6;
+                    const double ir = 1. / r64;
+                    const double *rv64 = p.rvec64 + 3 * (size_t)a.index;
+                    double c = 0.;
+                    c -= rdx * ir * rv64[0];
+                    c -= rdy * ir * rv64[1];
+                    c -= rdz * ir * rv64[2];
+                    front = c >= 0.;
+                    cos_t = (float)c;
+                } else {
+                    front = cos_t >= 0.f;
+                }
+                float t0 = dn[0] * a.rv2[0] + dn[1] * a.rv2[1] + dn[2] * a.rv2[2];
+                t0 = fminf(fmaxf(t0, -1.f), 1.f);
+                t0 = 1.57079632679489661923f - acosf(t0);
+                const float cr0 = dn[1] * a.rv2[2] - dn[2] * a.rv2[1];
+                const float cr1 = dn[2] * a.rv2[0] - dn[0] * a.rv2[2];
+                const float cr2 = dn[0] * a.rv2[1] - dn[1] * a.rv2[0];
+                float rd2 = cr0 * cr0 + cr1 * cr1 + cr2 * cr2;
+                if (rd2 < 1.0e-6f) rd2 = 1.0e-6f;
+                const float inv_rd = 1.f / sqrtf(rd2);
+                float ti = (cr0 * inv_rd) * a.rv[0] + (cr1 * inv_rd) * a.rv[1] + (cr2 * inv_rd) * a.rv[2];
+                rdon = 0.f;
+                if (front) {
+                    ti = fminf(fmaxf(ti, -1.f), 1.f);
+                    ti = acosf(ti) - 1.57079632679489661923f;
+                    ti = fabsf(ti);
+                    rdon = (0.9f + 0.1f * sinf(ti + ti)) * cosf(t0);
+                } else if (cos_t >= -0.34202f) {
+                    const float u = 0.116978f - cos_t * cos_t;
+                    rdon = 562.25f * (u * u * u) * cosf(t0);
+                }
+            } else if (hb == AG_HB_A2 && dis) {
+                float cos_t = -(dn[0] * a.rv[0]) - dn[1] * a.rv[1] - dn[2] * a.rv[2];
+                cos_t = fminf(fmaxf(cos_t, -1.f), 1.f);
+                const float th = acosf(cos_t);
+                racc = 0.f;
+                rdon = 0.f;
+                if (th <= 1.24791f + 1.57079632679489661923f) {
+                    const float cc = cosf(th - 1.24791f);
+                    rdon = (cc * cc) * (cc * cc);
+                    racc = rdon;
+                }
+            }
+            // vdW / H-bond table rows for this pair: [type][bin][slot]
+            const float4 *row = reinterpret_cast<const float4 *>(tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots);
+            float E[kMapSlots];
+#pragma unroll
+            for (int q = 0; q < kMapSlots / 4; q++) {
+                const float4 v = row[q];
+                E[4 * q] = v.x; E[4 * q + 1] = v.y; E[4 * q + 2] = v.z; E[4 * q + 3] = v.w;
+            }
+            float Eh[kMapSlots];
+            bool need_h = false;
+            int indx_h = 0;
+            if (dis && p.hydrogen_type >= 0) {
+                // disordered donor/acceptor partner: table of the hydrogen type
+                // at r - 1.10 (main.cpp:3313-3349), FP64 like the reference
+                double rdx, rdy, rdz;
+                const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                indx_h = (int)((r64 - 1.10) * 128.0);
+                indx_h = min(max(0, indx_h), AG_NEINT - 1);
+                need_h = true;
+            }
+            if (need_h) {
+                const float4 *hrow = reinterpret_cast<const float4 *>(
+                    tab + ((size_t)p.hydrogen_type * AG_NEINT + indx_h) * kMapSlots);
+#pragma unroll
+                for (int q = 0; q < kMapSlots / 4; q++) {
+                    const float4 v = hrow[q];
+                    Eh[4 * q] = v.x; Eh[4 * q + 1] = v.y; Eh[4 * q + 2] = v.z; Eh[4 * q + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < kMapSlots; m++) Eh[m] = 0.f;
+            }
+            const float S = p.sol[indx_r];
+#pragma unroll
+            for (int m = 0; m < kMapSlots; m++) {
+                const int role = m < p.nslots ? p.role[m] : ROLE_NONE;
+                if (role & ROLE_AFF) {
+                    const int hc = (p.hbc[m] >> (4 * hb)) & 0xf;
+                    const float e = E[m];
+                    if (hc == HBC_PLAIN) {
+                        acc[m] += e;
+                    } else {
+                        float rsph = e / 100.f;
+                        rsph = fmaxf(rsph, 0.f);
+                        rsph = fminf(rsph, 1.f);
+                        if (hc == HBC_ACC_PROBE) {
+                            if (!dis) acc[m] += e * Hramp * (racc + (1.f - racc) * rsph);
+                            else acc[m] += Eh[m] * Hramp * (racc + (1.f - racc) * rsph);
+                        } else if (hc == HBC_MINMAX_RACC) {
+                            const float v = e * (racc + (1.f - racc) * rsph);
+                            hmin[m] = fminf(hmin[m], v);
+                            hmax[m] = fmaxf(hmax[m], v);
+                        } else {  // HBC_DON_PROBE
+                            if (dis) {
+                                acc[m] += Eh[m] * Hramp * (rdon + (1.f - rdon) * rsph);
+                            } else {
+                                const float v = e * (rdon + (1.f - rdon) * rsph);
+                                hmin[m] = fminf(hmin[m], v);
+                                hmax[m] = fmaxf(hmax[m], v);
+                            }
+                        }
+                    }
+                }
+                if (role & ROLE_DES) acc[m] += p.sp[m] * a.vol * S + a.solq * p.volp[m] * S;
+            }
+            if (p.d_slot >= 0) d_acc += a.dvol * S;
+        }
+    }
+
+    if (!active) return;
+    const long long pt = (long long)ix + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
+#pragma unroll
+    for (int m = 0; m < kMapSlots; m++) {
+        if (m >= p.nslots || p.role[m] == ROLE_NONE) continue;
+        float v = acc[m];
+        if (hmin[m] < 999999.f) v += hmin[m] + fmaxf(hmax[m], 0.f);
+        p.maps[(long long)p.out_slot[m] * p.P + pt] = v;
+    }
+    if (p.e_slot >= 0) p.maps[(long long)p.e_slot * p.P + pt] = e_near;
+    if (p.d_slot >= 0) p.maps[(long long)p.d_slot * p.P + pt] = d_acc;
+}
+
+// ---------------------------------------------------------------------------
+// far kernel
+// ---------------------------------------------------------------------------
+
+template <bool CLAMP>
+__global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
+    extern __shared__ float T[];
+    for (int i = threadIdx.x; i < p.tab_len; i += kFarThreads) T[i] = p.tab[i];
+    __syncthreads();
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int run = blk * kFarThreads + threadIdx.x;
+    if (run >= p.nruns) return;
+    const int row = run / p.runs_per_row;
+    const int ix0 = (run % p.runs_per_row) * kFarK;
+    const int iy = row % p.n1[1];
+    const int iz = row / p.n1[1];
+    const float py = (float)(iy - p.ne[1]) * p.s128;
+    const float pz = (float)(iz - p.ne[2]) * p.s128;
+    float px[kFarK], s[kFarK], c[kFarK];
+    const long long base = (long long)ix0 + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
+#pragma unroll
+    for (int k = 0; k < kFarK; k++) {
+        px[k] = (float)(ix0 + k - p.ne[0]) * p.s128;
+        s[k] = (ix0 + k < p.n1[0]) ? p.emap[base + k] : 0.f;
+        c[k] = 0.f;
+    }
+    const float4 *__restrict__ atoms = p.atoms;
+    for (int a0 = 0; a0 < p.n_atoms; a0 += kFarAtomChunk) {
+        const int a1 = min(a0 + kFarAtomChunk, p.n_atoms);
+        float part[kFarK];
+#pragma unroll
+        for (int k = 0; k < kFarK; k++) part[k] = 0.f;
+#pragma unroll 4
+        for (int a = a0; a < a1; a++) {
+#pragma clang fp contract(off)
+            const float4 at = atoms[a];
+            const float dy = at.y - py, dz = at.z - pz;
+            const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
+#pragma unroll
+            for (int k = 0; k < kFarK; k++) {
+                const float dx = at.x - px[k];
+                const float rb2 = __builtin_fmaf(dx, dx, dyz2);
+                const float rs = __builtin_amdgcn_rsqf(rb2);
+                const float rb = rb2 * rs;
+                const int bin = CLAMP ? (int)fminf(rb, p.tmax) : (int)rb;
+                part[k] = __builtin_fmaf(at.w, rs * T[bin], part[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kFarK; k++) {
+#pragma clang fp contract(off)
+            const float y = part[k] - c[k];
+            const float t = s[k] + y;
+            c[k] = (t - s[k]) - y;
+            s[k] = t;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kFarK; k++)
+        if (ix0 + k < p.n1[0]) p.emap[base + k] = s[k];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context and host orchestration
+// ---------------------------------------------------------------------------
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (bytes > cap) {
+            if (ptr) HIP_CHECK(hipFree(ptr));
+            ptr = nullptr;
+            cap = 0;
+            if (hipMalloc(&ptr, bytes) != hipSuccess) {
+                ptr = nullptr;
+                ag::fail(AG_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+            }
+            cap = bytes;
+        }
+        return ptr;
+    }
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+struct ag_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf coord, charge, vol, solpar, rvec, rvec2, type, hbond, rexp, disorder;
+    DevBuf atom_cell, cell_count, cell_start, cell_cursor, slot_atom, near_atoms, far_atoms;
+    DevBuf tab, sol, reps, fartab, maps, fmap, cmap, status;
+    hipEvent_t ev[8] = {};
+    float last_ms[4] = {0, 0, 0, 0};   // near, far, cell, total of the last call
+};
+
+namespace agdev {
+
+ag_status context_create(ag_ctx **out, int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        ag::fail(AG_ERR_DEVICE, "no HIP device is available (this engine has no CPU fallback)");
+    if (device < 0 || device >= n) ag::fail(AG_ERR_DEVICE, "HIP device index out of range");
+    HIP_CHECK(hipSetDevice(device));
+    ag_ctx *c = new ag_ctx();
+    c->device = device;
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
+    *out = c;
+    return AG_OK;
+}
+
+void context_destroy(ag_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+template <typename T>
+static T *upload(ag_ctx *c, DevBuf &b, const T *src, size_t n) {
+    T *d = (T *)b.get(sizeof(T) * n);
+    if (n) HIP_CHECK(hipMemcpyAsync(d, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+    return d;
+}
+
+void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap, int outputs_on_device) {
+    HIP_CHECK(hipSetDevice(c->device));
+    const int N = rec->n_atoms;
+    const int M = ms->n_atom_maps;
+    const int T = tab->n_rec_types;
+    if (N < 0 || N > AG_MAX_ATOMS) ag::fail(AG_ERR_BAD_INPUT, "atom count out of range");
+    if (M < 0 || M + 2 > AG_MAX_MAPS) ag::fail(AG_ERR_BAD_INPUT, "map count out of range");
+    if (T < 0 || T > AG_MAX_REC_TYPES) ag::fail(AG_ERR_BAD_INPUT, "receptor type count out of range");
+    for (int d = 0; d < 3; d++) {
+        if (grid->n1[d] < 1 || grid->n1[d] > AG_MAX_GRID_PTS || grid->ne[d] < 0)
+            ag::fail(AG_ERR_BAD_INPUT, "grid dimensions out of range");
+    }
+    if (!(grid->spacing > 0)) ag::fail(AG_ERR_BAD_INPUT, "grid spacing must be positive");
+    if (fmap || cmap || ms->floating_grid || ms->constriction_grid)
+        ag::fail(AG_ERR_UNSUPPORTED, "floating and constriction grids are not implemented on the device yet");
+    const int n_maps = M + (ms->elec_index >= 0) + (ms->dsolv_index >= 0);
+    if (ms->n_maps != n_maps) ag::fail(AG_ERR_BAD_INPUT, "n_maps does not match the map set");
+    for (int i = 0; i < N; i++) {
+        if (rec->type[i] < 0 || rec->type[i] >= T) ag::fail(AG_ERR_BAD_INPUT, "atom type index out of range");
+        if (rec->hbond[i] < 0 || rec->hbond[i] > 6) ag::fail(AG_ERR_BAD_INPUT, "atom hbond class out of range");
+    }
+    bool need_h = false;
+    bool any_aff = false;
+    for (int m = 0; m < M; m++) any_aff = any_aff || !ms->maps[m].is_covalent;
+    if (M > 0) {
+        for (int i = 0; i < N && !need_h; i++) need_h = rec->hbond[i] == AG_HB_DS || rec->hbond[i] == AG_HB_D1;
+        if (!need_h) ag::fail(AG_ERR_NO_DONOR_H, "no closestH atom was found");
+    }
+    if (ms->disorder_h && (ms->hydrogen_type < -1 || ms->hydrogen_type >= T))
+        ag::fail(AG_ERR_BAD_INPUT, "hydrogen type index out of range");
+
+    const long long P = (long long)grid->n1[0] * grid->n1[1] * grid->n1[2];
+    hipStream_t st = c->stream;
+    HIP_CHECK(hipEventRecord(c->ev[0], st));
+
+    // ---- inputs ----
+    double *d_coord = upload(c, c->coord, rec->coord, 3 * (size_t)N);
+    double *d_charge = upload(c, c->charge, rec->charge, N);
+    double *d_vol = upload(c, c->vol, rec->vol, N);
+    double *d_solpar = upload(c, c->solpar, rec->solpar, N);
+    double *d_rvec = upload(c, c->rvec, rec->rvector, 3 * (size_t)N);
+    double *d_rvec2 = upload(c, c->rvec2, rec->rvector2, 3 * (size_t)N);
+    int *d_type = upload(c, c->type, rec->type, N);
+    int *d_hbond = upload(c, c->hbond, rec->hbond, N);
+    int *d_rexp = upload(c, c->rexp, rec->rexp, N);
+    unsigned char *d_dis = upload(c, c->disorder, rec->disorder, N);
+
+    // ---- cell list over the box padded by the staging radius ----
+    const double cs = 4.0;
+    CellParams cp;
+    cp.cs = cs;
+    cp.n_atoms = N;
+    for (int d = 0; d < 3; d++) {
+        const double lo = -grid->ne[d] * grid->spacing - 8.1;
+        const double hi = (grid->n1[d] - 1 - grid->ne[d]) * grid->spacing + 8.1;
+        cp.origin[d] = lo;
+        cp.nc[d] = (int)ceil((hi - lo) / cs) + 1;
+    }
+    const int ncells = cp.nc[0] * cp.nc[1] * cp.nc[2];
+    int *d_atom_cell = (int *)c->atom_cell.get(sizeof(int) * (size_t)N);
+    int *d_count = (int *)c->cell_count.get(sizeof(int) * (size_t)ncells);
+    int *d_start = (int *)c->cell_start.get(sizeof(int) * ((size_t)ncells + 1));
+    int *d_cursor = (int *)c->cell_cursor.get(sizeof(int) * (size_t)ncells);
+    int *d_slot_atom = (int *)c->slot_atom.get(sizeof(int) * (size_t)N);
+    HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(int) * (size_t)ncells, st));
+    HIP_CHECK(hipMemsetAsync(d_cursor, 0, sizeof(int) * (size_t)ncells, st));
+    const int tpb = 256;
+    if (N > 0) k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(cp, d_coord, d_atom_cell, d_count);
+    k_scan<<<1, 1024, 0, st>>>(d_count, d_start, ncells);
+    if (N > 0) k_cell_scatter<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(N, d_atom_cell, d_start, d_cursor, d_slot_atom);
+    k_cell_sort<<<(ncells + tpb - 1) / tpb, tpb, 0, st>>>(ncells, d_start, d_slot_atom);
+    NearAtom *d_near = (NearAtom *)c->near_atoms.get(sizeof(NearAtom) * (size_t)N);
+    AtomSrc src{d_coord, d_charge, d_vol, d_solpar, d_rvec, d_rvec2, d_type, d_hbond, d_rexp, d_dis,
+                ms->coeff_estat, ms->solpar_q};
+    // records of every binned slot, in (cell, atom index) order
+    if (N > 0) k_gather<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(d_start + ncells, d_slot_atom, src, d_near);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(c->ev[1], st));
+
+    // ---- tables ----
+    // electrostatic factor table shared by both kernels
+    std::vector<float> reps(AG_NDIEL);
+    for (int k = 0; k < AG_NDIEL; k++)
+        reps[k] = ms->dddiel ? (float)tab->r_eps[k] : (float)ms->inv_diel_cal;
+    float *d_reps = upload(c, c->reps, reps.data(), AG_NDIEL);
+    std::vector<float> sol(AG_NEINT + 1);
+    for (int k = 0; k <= AG_NEINT; k++) sol[k] = (float)tab->sol_fn[k];
+    float *d_sol = upload(c, c->sol, sol.data(), AG_NEINT + 1);
+
+    // outputs
+    float *d_maps = outputs_on_device ? maps_out : (float *)c->maps.get(sizeof(float) * (size_t)n_maps * P);
+    int *d_status = (int *)c->status.get(sizeof(int));
+    HIP_CHECK(hipMemsetAsync(d_status, 0, sizeof(int), st));
+
+    // ---- near kernel, one pass per group of 16 map slots ----
+    const int groups = M > 0 ? (M + kMapSlots - 1) / kMapSlots : 1;
+    std::vector<float> gtab((size_t)(T > 0 ? T : 1) * AG_NEINT * kMapSlots);
+    NearParams np;
+    memset(&np, 0, sizeof np);
+    for (int d = 0; d < 3; d++) {
+        np.n1[d] = grid->n1[d];
+        np.ne[d] = grid->ne[d];
+        np.cell_origin[d] = cp.origin[d];
+        np.nc[d] = cp.nc[d];
+    }
+    np.spacing = grid->spacing;
+    np.s128 = (float)(grid->spacing * 128.0);
+    np.nb[0] = (grid->n1[0] + kBX - 1) / kBX;
+    np.nb[1] = (grid->n1[1] + kBY - 1) / kBY;
+    np.nb[2] = (grid->n1[2] + kBZ - 1) / kBZ;
+    np.nbricks = np.nb[0] * np.nb[1] * np.nb[2];
+    np.cell_size = cs;
+    np.cell_start = d_start;
+    np.atoms = d_near;
+    np.coord64 = d_coord;
+    np.rvec64 = d_rvec;
+    np.rvec2_64 = d_rvec2;
+    np.sol = d_sol;
+    np.reps = d_reps;
+    np.disorder_h = ms->disorder_h;
+    np.hydrogen_type = ms->hydrogen_type;
+    for (int d = 0; d < 3; d++) np.cov_pos[d] = ms->cov_pos[d];
+    np.cov_hw = ms->cov_half_width;
+    np.cov_barrier = ms->cov_barrier;
+    np.maps = d_maps;
+    np.P = P;
+    np.status = d_status;
+    for (int g = 0; g < groups; g++) {
+        const int m0 = g * kMapSlots;
+        const int ns = M > 0 ? std::min(kMapSlots, M - m0) : 0;
+        np.nslots = ns;
+        bool has_aff = false;
+        for (int j = 0; j < kMapSlots; j++) {
+            np.role[j] = ROLE_NONE;
+            np.hbc[j] = 0;
+            np.sp[j] = 0.f;
+            np.volp[j] = 0.f;
+            np.out_slot[j] = 0;
+        }
+        for (int j = 0; j < ns; j++) {
+            const int m = m0 + j;
+            const ag_map_desc &md = ms->maps[m];
+            np.out_slot[j] = m;
+            if (md.is_covalent) {
+                np.role[j] = ROLE_COV;
+                continue;
+            }
+            // separate desolvation: the map after each non-covalent map holds
+            // its desolvation (main.cpp:3385)
+            bool second = false;
+            if (ms->separate_desolvation) {
+                // walk from the first map to know whether m is a second of a pair
+                int k = 0;
+                while (k < m) {
+                    if (ms->maps[k].is_covalent) k += 1;
+                    else k += 2;
+                }
+                second = (k != m);
+            }
+            if (ms->separate_desolvation) np.role[j] = second ? ROLE_DES : ROLE_AFF;
+            else np.role[j] = ROLE_AFF | ROLE_DES;
+            has_aff = true;
+            np.sp[j] = (float)md.solpar_probe;
+            np.volp[j] = (float)md.vol_probe;
+            int hc = 0;
+            if (md.is_hbonder) {
+                const int mh = md.hbond;
+                for (int ah = 0; ah <= 6; ah++) {
+                    int cse = HBC_PLAIN;
+                    const bool atom_don = (ah == 1 || ah == 2 || ah == 6);
+                    if ((mh == 3 || mh == 5 || mh == 6) && atom_don) cse = HBC_ACC_PROBE;
+                    else if ((mh == 4 || mh == 6) && atom_don) cse = HBC_MINMAX_RACC;
+                    else if ((mh == 1 || mh == 2 || mh == 6) && (ah > 2 || ah == 6)) cse = HBC_DON_PROBE;
+                    hc |= cse << (4 * ah);
+                }
+            }
+            np.hbc[j] = hc;
+        }
+        // table for this group's maps: [T][1024][16]
+        std::fill(gtab.begin(), gtab.end(), 0.f);
+        for (int t = 0; t < T; t++)
+            for (int j = 0; j < ns; j++)
+                for (int k = 0; k < AG_NEINT; k++)
+                    gtab[((size_t)t * AG_NEINT + k) * kMapSlots + j] =
+                        tab->e_vdw_hb[((size_t)t * M + (m0 + j)) * AG_NEINT + k];
+        float *d_tab = (float *)c->tab.get(sizeof(float) * gtab.size());
+        HIP_CHECK(hipMemcpyAsync(d_tab, gtab.data(), sizeof(float) * gtab.size(), hipMemcpyHostToDevice, st));
+        np.tab = d_tab;
+        np.e_slot = (g == 0) ? ms->elec_index : -1;
+        np.d_slot = (g == 0) ? ms->dsolv_index : -1;
+        np.has_affinity = has_aff || np.d_slot >= 0;
+        if (np.nslots > 0 || np.e_slot >= 0 || np.d_slot >= 0)
+            k_near<<<np.nbricks, kNearThreads, 0, st>>>(np);
+        HIP_CHECK(hipGetLastError());
+        // the group table buffer is rewritten by the next group: wait here
+        if (g + 1 < groups) HIP_CHECK(hipStreamSynchronize(st));
+    }
+    HIP_CHECK(hipEventRecord(c->ev[2], st));
+
+    // ---- far kernel (e map) ----
+    if (ms->elec_index >= 0 && N > 0) {
+        float4 *d_far = (float4 *)c->far_atoms.get(sizeof(float4) * (size_t)N);
+        k_far_atoms<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(N, d_coord, d_charge, ms->coeff_estat, d_far);
+        std::vector<float> ft(AG_NDIEL);
+        for (int k = 0; k < AG_NDIEL; k++) ft[k] = k < kFarBinMin ? 0.f : reps[k];
+        float *d_ft = upload(c, c->fartab, ft.data(), AG_NDIEL);
+        FarParams fp;
+        memset(&fp, 0, sizeof fp);
+        for (int d = 0; d < 3; d++) {
+            fp.n1[d] = grid->n1[d];
+            fp.ne[d] = grid->ne[d];
+        }
+        fp.s128 = (float)(grid->spacing * 128.0);
+        fp.runs_per_row = (grid->n1[0] + kFarK - 1) / kFarK;
+        fp.nruns = fp.runs_per_row * grid->n1[1] * grid->n1[2];
+        fp.atoms = d_far;
+        fp.n_atoms = N;
+        fp.tab = d_ft;
+        fp.tab_len = AG_NDIEL;
+        fp.tmax = (float)(AG_NDIEL - 1);
+        fp.emap = d_maps + (size_t)ms->elec_index * P;
+        const int blocks = (fp.nruns + kFarThreads - 1) / kFarThreads;
+        k_far<true><<<blocks, kFarThreads, sizeof(float) * AG_NDIEL, st>>>(fp);
+        HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipEventRecord(c->ev[3], st));
+
+    int status = 0;
+    HIP_CHECK(hipMemcpyAsync(&status, d_status, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (!outputs_on_device)
+        HIP_CHECK(hipMemcpyAsync(maps_out, d_maps, sizeof(float) * (size_t)n_maps * P, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms[2], c->ev[0], c->ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms[0], c->ev[1], c->ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms[1], c->ev[2], c->ev[3]));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
+    if (status & 1)
+        ag::fail(AG_ERR_UNSUPPORTED,
+                 "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
+}
+
+}  // namespace agdev

@@ -1,0 +1,168 @@
+// Internal host-side types of the MI355X AutoGrid4 engine.
+#pragma once
+
+#include <cstdio>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include <stdexcept>
+
+#include "autogrid_amd.h"
+
+namespace ag {
+
+// Error carrying an ag_status; thrown inside the library, turned into a status
+// plus ag_last_error() text at the C boundary.
+struct Error : std::runtime_error {
+    ag_status status;
+    Error(ag_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+[[noreturn]] void fail(ag_status s, const std::string &msg);
+void set_last_error(const std::string &msg);
+
+// Constants of the 8 A build.
+constexpr double kApproxZero = 1.0e-6;   // APPROX_ZERO, autocomm.h:72
+constexpr double kEintClamp = 100000.0;  // EINTCLAMP, autocomm.h:89
+constexpr double kBig = 1.0e12;          // BIG, autocomm.h:73
+constexpr double kSoftNbc = 8.0;         // SOFTNBC, constants.h:150
+constexpr double kPrecision = 0.0001;    // PRECISION, autogrid.h:68
+constexpr int kMaxNbonds = 4;            // AG_MAX_NBONDS, autogrid.h:59
+
+// One atom_par entry of the parameter library (ParameterEntry, parameters.h:39-61).
+struct ParamEntry {
+    char type[8] = {0};
+    double Rij = 0, epsij_unweighted = 0, vol = 0, solpar = 0;
+    double Rij_hb = 0, epsij_hb_unweighted = 0;
+    int hbond = 0;
+    int rec_index = -1, map_index = -1, bond_index = 0;
+};
+
+// Parameter dictionary keyed on the first two characters of the type name,
+// case-sensitive (atom_parameter_manager.cpp:37-55).
+struct ParamLibrary {
+    std::vector<int> slot;             // 65536 entries, index into entries or -1
+    std::vector<ParamEntry> entries;
+    double coeff_vdW = 0, coeff_hbond = 0, coeff_estat = 0, coeff_desolv = 0,
+           coeff_tors = 0;
+    ParamLibrary();
+    static unsigned key(const char *t);
+    ParamEntry *find(const char *t);
+    void enter(const ParamEntry &e);
+    // Apply one parameter-library line (read_parameter_library.cc:84-199).
+    void apply_line(const char *line, ParamEntry &scratch);
+    void load_builtin_ad41_bound();
+    void load_file(const std::string &path);
+};
+
+struct MapInfo {
+    std::string type;         // ligand type, e.g. "OA"
+    std::string filename;
+    FILE *fp = nullptr;
+    bool is_covalent = false, is_hbonder = false;
+    int hbond = 0;
+    double solpar_probe = 0, vol_probe = 0;
+    double Rij = 0, epsij = 0, Rij_hb = 0, epsij_hb = 0;
+    ag_pair pair[AG_MAX_REC_TYPES];
+};
+
+// Everything main() holds once the GPF loop ends, plus the prepared state.
+struct Job {
+    // bookkeeping / files
+    std::string program = "autogrid4";
+    std::string gpf_path, receptor_filename, fld_filename, xyz_filename;
+    std::string fmap_filename, cmap_filename;
+    FILE *log = nullptr;
+    bool own_log = false;
+    bool make_files = true;
+    FILE *fld_fp = nullptr, *xyz_fp = nullptr, *fmap_fp = nullptr, *cmap_fp = nullptr;
+    bool xyz_open = false;       // gridfld seen and .xyz not yet written
+    int outlev = 1;
+
+    ParamLibrary lib;
+
+    // receptor
+    std::vector<double> coord;         // [n][3], centred when gridcenter seen
+    std::vector<double> charge, vol, solpar;
+    std::vector<int32_t> type, hbond;
+    int n_atoms = 0;
+    std::vector<std::string> rec_types;   // receptor_types[] in rec_index order
+    bool has_receptor_types_in_gpf = false;
+    int receptor_types_gpf_ct = 0;
+    double cmean[3] = {0, 0, 0};
+
+    // grid
+    int nelements[3] = {0, 0, 0};
+    int n1[3] = {0, 0, 0};
+    int ne[3] = {0, 0, 0};
+    bool npts_set = false;
+    double spacing = 0.375;
+    double center[3] = {0, 0, 0};
+
+    // maps
+    std::vector<MapInfo> maps;         // num_atom_maps + 2 slots as in the reference
+    int num_atom_maps = 0, num_maps = 0;
+    int elecPE = -1, dsolvPE = -1;
+    bool ligand_types_seen = false;
+    bool separate_desolvation = false;
+    bool disorder_h = false;
+    bool dddiel = false;
+    double inv_diel_cal = 0;
+    double r_smooth = 0.5;
+    double solpar_q = 0.01097;
+    double cov_pos[3] = {0, 0, 0};
+    double cov_half_width = 1.0, cov_barrier = 1000.0;
+    bool floating_grid = false, constriction_grid = false;
+    double constriction_cutoff = 7.5;
+    int num_distance_maps = 0;
+
+    // prepared state (host)
+    bool prepared = false;
+    std::vector<ag_pair> pairs;        // [num_atom_maps][n_rec_types]
+    std::vector<float> e_vdw_hb;       // [n_rec_types][num_atom_maps][1024]
+    std::vector<double> sol_fn, r_eps; // [8192]
+    std::vector<int32_t> rexp;
+    std::vector<double> rvector, rvector2;
+    std::vector<uint8_t> disorder;
+    std::vector<ag_map_desc> map_desc;
+    int t_hydrogen = -1, t_nonhb_hydrogen = -1, t_carbon = -1, t_arom_carbon = -1;
+    int t_oxygen = -1, t_sulphur = -1, t_nitrogen = -1, t_nonhb_nitrogen = -1;
+
+    ~Job();
+    int64_t points() const { return (int64_t)n1[0] * n1[1] * n1[2]; }
+    int rec_index(const char *t) { ParamEntry *p = lib.find(t); return p ? p->rec_index : -1; }
+    int map_index(const char *t) { ParamEntry *p = lib.find(t); return p ? p->map_index : -1; }
+    void logf(const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+};
+
+// front end
+void job_parse_gpf(Job &job, const std::string &gpf_path);
+void job_write_fld(Job &job);
+void job_write_map_headers(Job &job);
+// prepare
+void job_prepare(Job &job);
+void job_views(const Job &job, ag_receptor *rec, ag_grid *grid, ag_mapset *ms,
+               ag_tables *tab, ag_setup *setup);
+void build_vdw_tables(const ag_setup &s, const ag_mapset &ms, float *out);
+void build_sol_fn(double coeff_desolv, double *out);
+void build_r_eps(double *out);
+void build_bond_vectors(const ag_setup &s, int n, const double *coord,
+                        const int32_t *type, const int32_t *hbond,
+                        int32_t *rexp, double *rvector, double *rvector2,
+                        uint8_t *disorder, FILE *log);
+// writer
+int format_map_value(float v, char *out);     // returns bytes incl. '\n'
+void write_all_maps(Job &job, const float *maps, const float *fmap,
+                    const int32_t *cmap, int n_threads);
+
+}  // namespace ag
+
+// Device side (ag_device.hip).
+struct ag_ctx;
+namespace agdev {
+ag_status context_create(ag_ctx **out, int device);
+void context_destroy(ag_ctx *ctx);
+void maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
+                const ag_mapset *ms, const ag_tables *tab, float *maps,
+                float *fmap, int32_t *cmap, int outputs_on_device);
+}  // namespace agdev

@@ -1,0 +1,274 @@
+// C-ABI of the MI355X AutoGrid4 engine (include/autogrid_amd.h).
+//
+// Every entry point catches the library's exceptions and turns them into an
+// ag_status plus a thread-local message; none of them calls exit().
+
+#include <cstring>
+#include <thread>
+
+#include "ag_internal.h"
+
+struct ag_job {
+    ag::Job job;
+};
+
+namespace ag {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+void job_views(const Job &job, ag_receptor *rec, ag_grid *grid, ag_mapset *ms, ag_tables *tab,
+               ag_setup *setup) {
+    if (rec) {
+        rec->n_atoms = job.n_atoms;
+        rec->coord = job.coord.data();
+        rec->charge = job.charge.data();
+        rec->vol = job.vol.data();
+        rec->solpar = job.solpar.data();
+        rec->type = job.type.data();
+        rec->hbond = job.hbond.data();
+        rec->rexp = job.prepared ? job.rexp.data() : nullptr;
+        rec->rvector = job.prepared ? job.rvector.data() : nullptr;
+        rec->rvector2 = job.prepared ? job.rvector2.data() : nullptr;
+        rec->disorder = job.prepared ? job.disorder.data() : nullptr;
+    }
+    if (grid) {
+        for (int d = 0; d < 3; d++) {
+            grid->n1[d] = job.n1[d];
+            grid->ne[d] = job.ne[d];
+        }
+        grid->spacing = job.spacing;
+    }
+    if (ms) {
+        memset(ms, 0, sizeof *ms);
+        ms->n_atom_maps = job.num_atom_maps;
+        ms->maps = job.prepared ? job.map_desc.data() : nullptr;
+        ms->elec_index = job.elecPE;
+        ms->dsolv_index = job.dsolvPE;
+        ms->n_maps = job.num_maps;
+        ms->separate_desolvation = job.separate_desolvation;
+        ms->disorder_h = job.disorder_h;
+        ms->hydrogen_type = job.t_hydrogen;
+        ms->dddiel = job.dddiel;
+        ms->floating_grid = job.floating_grid;
+        ms->constriction_grid = job.constriction_grid;
+        ms->inv_diel_cal = job.inv_diel_cal;
+        ms->coeff_estat = job.lib.coeff_estat;
+        ms->solpar_q = job.solpar_q;
+        for (int d = 0; d < 3; d++) ms->cov_pos[d] = job.cov_pos[d];
+        ms->cov_half_width = job.cov_half_width;
+        ms->cov_barrier = job.cov_barrier;
+        ms->constriction_cutoff = job.constriction_cutoff;
+    }
+    if (tab) {
+        memset(tab, 0, sizeof *tab);
+        tab->n_rec_types = (int32_t)job.rec_types.size();
+        tab->e_vdw_hb = job.prepared ? job.e_vdw_hb.data() : nullptr;
+        tab->sol_fn = job.prepared ? job.sol_fn.data() : nullptr;
+        tab->r_eps = job.prepared ? job.r_eps.data() : nullptr;
+    }
+    if (setup) {
+        memset(setup, 0, sizeof *setup);
+        setup->n_rec_types = (int32_t)job.rec_types.size();
+        setup->n_atom_maps = job.num_atom_maps;
+        setup->pairs = job.prepared ? job.pairs.data() : nullptr;
+        setup->r_smooth = job.r_smooth;
+        setup->coeff_desolv = job.lib.coeff_desolv;
+        setup->dddiel = job.dddiel;
+        setup->disorder_h = job.disorder_h;
+        setup->t_hydrogen = job.t_hydrogen;
+        setup->t_nonhb_hydrogen = job.t_nonhb_hydrogen;
+        setup->t_carbon = job.t_carbon;
+        setup->t_arom_carbon = job.t_arom_carbon;
+        setup->t_oxygen = job.t_oxygen;
+        setup->t_sulphur = job.t_sulphur;
+        setup->t_nitrogen = job.t_nitrogen;
+        setup->t_nonhb_nitrogen = job.t_nonhb_nitrogen;
+    }
+}
+
+}  // namespace ag
+
+template <typename F>
+static ag_status guarded(F &&f) {
+    try {
+        f();
+        return AG_OK;
+    } catch (const ag::Error &e) {
+        ag::set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        ag::set_last_error("host allocation failed");
+        return AG_ERR_OOM;
+    } catch (const std::exception &e) {
+        ag::set_last_error(e.what());
+        return AG_ERR_BAD_INPUT;
+    }
+}
+
+// print_error(FATAL_ERROR) text in the log (main.cpp:133-177).
+static void log_fatal(ag::Job &job, const std::string &msg) {
+    char out[2048];
+    snprintf(out, sizeof out, "\n%s: %s:  %s\n", job.program.c_str(), "ERROR", msg.c_str());
+    if (job.log) {
+        fprintf(job.log, "%s\n", out);
+        fprintf(job.log, "\n%s: Unsuccessful Completion.\n", job.program.c_str());
+        fflush(job.log);
+    }
+    fprintf(stderr, "%s\n", out);
+}
+
+extern "C" {
+
+int ag_abi_version(void) { return AG_ABI_VERSION; }
+
+const char *ag_last_error(void) { return ag::g_last_error.c_str(); }
+
+ag_status ag_context_create(ag_ctx **out, int device) {
+    if (!out) {
+        ag::set_last_error("null output pointer");
+        return AG_ERR_BAD_INPUT;
+    }
+    *out = nullptr;
+    return guarded([&] { agdev::context_create(out, device); });
+}
+
+ag_status ag_context_destroy(ag_ctx *ctx) {
+    return guarded([&] { agdev::context_destroy(ctx); });
+}
+
+ag_status ag_job_open(const char *gpf_path, const char *glg_path, const char *program_name, uint32_t flags,
+                      ag_job **out) {
+    if (!out) {
+        ag::set_last_error("null output pointer");
+        return AG_ERR_BAD_INPUT;
+    }
+    *out = nullptr;
+    ag_job *j = new (std::nothrow) ag_job();
+    if (!j) {
+        ag::set_last_error("host allocation failed");
+        return AG_ERR_OOM;
+    }
+    ag::Job &job = j->job;
+    job.make_files = !(flags & AG_JOB_NO_FILES);
+    if (program_name) job.program = program_name;
+    ag_status st = guarded([&] {
+        if (glg_path) {
+            job.log = fopen(glg_path, "w");
+            if (!job.log) ag::fail(AG_ERR_IO, std::string("Sorry, I can't create the log file \"") + glg_path + "\"");
+            job.own_log = true;
+        }
+        if (!gpf_path) ag::fail(AG_ERR_BAD_INPUT, "no grid parameter file given");
+        ag::job_parse_gpf(job, gpf_path);
+    });
+    if (st != AG_OK) {
+        log_fatal(job, ag::g_last_error);
+        delete j;
+        return st;
+    }
+    *out = j;
+    return AG_OK;
+}
+
+ag_status ag_job_prepare(ag_job *j) {
+    if (!j) {
+        ag::set_last_error("null job");
+        return AG_ERR_BAD_INPUT;
+    }
+    ag_status st = guarded([&] { ag::job_prepare(j->job); });
+    if (st != AG_OK) log_fatal(j->job, ag::g_last_error);
+    return st;
+}
+
+ag_status ag_job_views(const ag_job *j, ag_receptor *rec, ag_grid *grid, ag_mapset *ms, ag_tables *tab,
+                       ag_setup *setup) {
+    if (!j) {
+        ag::set_last_error("null job");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { ag::job_views(j->job, rec, grid, ms, tab, setup); });
+}
+
+int64_t ag_job_points(const ag_job *j) { return j ? j->job.points() : 0; }
+
+void ag_job_close(ag_job *j) { delete j; }
+
+ag_status ag_maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                        const ag_tables *tab, float *maps, float *fmap, int32_t *cmap, int outputs_on_device) {
+    if (!ctx || !rec || !grid || !ms || !tab || !maps) {
+        ag::set_last_error("null argument to ag_maps_plain");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::maps_plain(ctx, rec, grid, ms, tab, maps, fmap, cmap, outputs_on_device); });
+}
+
+int64_t ag_format_map_values(const float *values, int64_t n, char *buf, int64_t cap) {
+    int64_t used = 0;
+    char tmp[64];
+    for (int64_t i = 0; i < n; i++) {
+        int k = ag::format_map_value(values[i], tmp);
+        if (used + k > cap) return -1;
+        memcpy(buf + used, tmp, (size_t)k);
+        used += k;
+    }
+    return used;
+}
+
+ag_status ag_job_write_maps(ag_job *j, const float *maps, const float *fmap, const int32_t *cmap,
+                            int n_threads) {
+    if (!j || !maps) {
+        ag::set_last_error("null argument to ag_job_write_maps");
+        return AG_ERR_BAD_INPUT;
+    }
+    ag_status st = guarded([&] {
+        ag::write_all_maps(j->job, maps, fmap, cmap, n_threads > 0 ? n_threads : 1);
+        j->job.logf("\n%s: Successful Completion.\n", j->job.program.c_str());
+        if (j->job.log) fflush(j->job.log);
+    });
+    if (st != AG_OK) log_fatal(j->job, ag::g_last_error);
+    return st;
+}
+
+ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path, const char *program_name) {
+    ag_job *j = nullptr;
+    ag_status st = ag_job_open(gpf_path, glg_path, program_name, 0, &j);
+    if (st != AG_OK) return st;
+    ag::Job &job = j->job;
+    st = guarded([&] {
+        ag::job_write_fld(job);
+        ag::job_prepare(job);
+        job.logf("\n\nCalculating Pairwise Interaction Energies\n");
+        job.logf("Beginning grid calculations.\n");
+        job.logf("\nCalculating %d grid%s over %lld element%s, around %d receptor atom%s.\n\n",
+                 job.num_maps + job.num_distance_maps, (job.num_maps + job.num_distance_maps) == 1 ? "" : "s",
+                 (long long)job.points(), job.points() == 1 ? "" : "s", job.n_atoms, job.n_atoms == 1 ? "" : "s");
+        ag::job_write_map_headers(job);
+        ag_receptor rec;
+        ag_grid grid;
+        ag_mapset ms;
+        ag_tables tab;
+        ag::job_views(job, &rec, &grid, &ms, &tab, nullptr);
+        std::vector<float> maps((size_t)job.num_maps * job.points());
+        std::vector<float> fmap(job.floating_grid ? job.points() : 0);
+        std::vector<int32_t> cmap(job.constriction_grid ? job.points() : 0);
+        // With no context given, the device is opened only now, after every
+        // input error has had its chance to surface.
+        ag_ctx *own = nullptr;
+        if (!ctx) {
+            agdev::context_create(&own, 0);
+            ctx = own;
+        }
+        struct Own { ag_ctx *c; ~Own() { if (c) agdev::context_destroy(c); } } own_guard{own};
+        agdev::maps_plain(ctx, &rec, &grid, &ms, &tab, maps.data(), job.floating_grid ? fmap.data() : nullptr,
+                          job.constriction_grid ? cmap.data() : nullptr, 0);
+        unsigned hw = std::thread::hardware_concurrency();
+        ag::write_all_maps(job, maps.data(), fmap.data(), cmap.data(), (int)std::min(16u, hw ? hw : 1u));
+        job.logf("\n%s: Successful Completion.\n", job.program.c_str());
+        if (job.log) fflush(job.log);
+    });
+    if (st != AG_OK) log_fatal(job, ag::g_last_error);
+    ag_job_close(j);
+    return st;
+}
+
+}  // extern "C"

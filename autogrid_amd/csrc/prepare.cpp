@@ -1,0 +1,300 @@
+// Host-side precompute of the grid-map path: energy tables and H-bond
+// geometry.  These run once per receptor in milliseconds and feed the device
+// kernels; they follow the reference's arithmetic term by term so that the
+// tables the kernels read are the reference's tables.
+
+#include <cmath>
+#include <cstring>
+
+#include "ag_internal.h"
+
+namespace ag {
+
+// e_vdW_Hb[type][map][0..1023] (main.cpp:2040-2266): 12-6 / 12-10 pair
+// potential clamped at EINTCLAMP, E[0] = EINTCLAMP, E[1023] = 0, then the
+// min-window smoothing of half-width floor(r_smooth*128/2) bins.  The table
+// element type is float as in the reference (typedefs.h:82); the arithmetic
+// before storing is double.
+void build_vdw_tables(const ag_setup &s, const ag_mapset &ms, float *out) {
+    const int T = s.n_rec_types, M = s.n_atom_maps;
+    std::vector<float> e(AG_NEINT);
+    std::vector<double> smooth(AG_NEINT);
+    memset(out, 0, sizeof(float) * (size_t)T * M * AG_NEINT);
+    const int i_smooth = (int)(s.r_smooth * AG_A_DIV / 2.);
+    for (int m = 0; m < M; m++) {
+        if (ms.maps[m].is_covalent) continue;
+        for (int t = 0; t < T; t++) {
+            const ag_pair &p = s.pairs[(size_t)m * T + t];
+            double cA = p.cA, cB = p.cB;
+            const int xA = p.xA, xB = p.xB;
+            if (cA == 0 && cB == 0) {
+                double tmpconst = p.eps / (double)(xA - xB);
+                cA = tmpconst * pow(p.Rij, (double)xA) * (double)xB;
+                cB = tmpconst * pow(p.Rij, (double)xB) * (double)xA;
+            }
+            if (std::isnan(cA))
+                fail(AG_ERR_BAD_INPUT, "Van der Waals coefficient cA is not a number.  AutoGrid must exit.");
+            if (std::isnan(cB))
+                fail(AG_ERR_BAD_INPUT, "Van der Waals coefficient cB is not a number.  AutoGrid must exit.");
+            if (xA == 0) fail(AG_ERR_BAD_INPUT, "Van der Waals exponent xA is 0.  AutoGrid must exit.");
+            if (xB == 0) fail(AG_ERR_BAD_INPUT, "Van der Waals exponent xB is 0.  AutoGrid must exit.");
+            for (int k = 1; k < AG_NEINT; k++) {
+                const double r = (double)k * (1. / AG_A_DIV);
+                const double v = cA / pow(r, (double)xA) - cB / pow(r, (double)xB);
+                e[k] = (float)(kEintClamp < v ? kEintClamp : v);
+            }
+            e[0] = (float)kEintClamp;
+            e[AG_NEINT - 1] = 0.f;
+            if (i_smooth > 0) {
+                for (int k = 0; k < AG_NEINT; k++) {
+                    double lo = 100000.;
+                    const int j0 = k - i_smooth < 0 ? 0 : k - i_smooth;
+                    const int j1 = k + i_smooth + 1 > AG_NEINT ? AG_NEINT : k + i_smooth + 1;
+                    for (int j = j0; j < j1; j++) lo = lo < e[j] ? lo : (double)e[j];
+                    smooth[k] = lo;
+                }
+                for (int k = 0; k < AG_NEINT; k++) e[k] = (float)smooth[k];
+            }
+            memcpy(out + ((size_t)t * M + m) * AG_NEINT, e.data(), sizeof(float) * AG_NEINT);
+        }
+    }
+}
+
+// sol_fn[k] = coeff_desolv * exp(-r^2 / (2 sigma^2)), sigma = 3.6, sol_fn[0]
+// left at 0 (main.cpp:2272-2276).
+void build_sol_fn(double coeff_desolv, double *out) {
+    const double sigma = 3.6;
+    out[0] = 0.0;
+    for (int k = 1; k < AG_NDIEL; k++) {
+        const double r = (double)k * (1. / AG_A_DIV);
+        out[k] = coeff_desolv * exp(-(r * r) / (2. * (sigma * sigma)));
+    }
+}
+
+// Mehler-Solmajer distance-dependent dielectric (distdepdiel.cpp:30-57); the
+// reference evaluates the denominator in long double (1.0L literal).
+static double ddd_mehler_solmajer(double distance) {
+    const double lambda = 0.003627L, epsilon0 = 78.4L, A = -8.5525L, rk = 7.7839L;
+    const double B = epsilon0 - A;
+    const double lambda_B = -lambda * B;
+    double epsilon = A + B / (1.0L + rk * exp(lambda_B * distance));
+    if (epsilon < kApproxZero) epsilon = 1.0L;
+    return epsilon;
+}
+
+// r_epsilon_fn[k] = 332 / eps(k/128), eps[0] = 1 (main.cpp:1612-1628).
+void build_r_eps(double *out) {
+    out[0] = 332.0 / 1.0;
+    for (int k = 1; k < AG_NDIEL; k++) out[k] = 332.0 / ddd_mehler_solmajer((double)k * (1. / AG_A_DIV));
+}
+
+// --- bond vectors (main.cpp:2297-2736) -------------------------------------
+
+static inline double len3(const double v[3]) { return hypot(v[0], hypot(v[1], v[2])); }
+
+// vect_normalize (main.cpp:3606-3617)
+static double normalize(double v[3]) {
+    double rd = len3(v);
+    double inv = rd < kApproxZero ? 1. / kApproxZero : 1. / rd;
+    for (int i = 0; i < 3; i++) v[i] *= inv;
+    return rd;
+}
+
+// vect_sub_normalize (main.cpp:3620-3633)
+static double sub_normalize(double res[3], const double a[3], const double b[3]) {
+    double rd2 = 0.;
+    for (int i = 0; i < 3; i++) {
+        res[i] = a[i] - b[i];
+        rd2 += res[i] * res[i];
+    }
+    double rd = rd2 < kApproxZero ? kApproxZero : sqrt(rd2);
+    double inv = 1. / rd;
+    for (int i = 0; i < 3; i++) res[i] *= inv;
+    return rd;
+}
+
+// distance_le (main.cpp:121-129): componentwise early-out, then the hypot
+// length; dc = a - b.
+static bool distance_le(const double *a, const double *b, double limit, double dc[3], double &d) {
+    dc[0] = a[0] - b[0];
+    if (dc[0] > limit || dc[0] < -limit) return false;
+    dc[1] = a[1] - b[1];
+    if (dc[1] > limit || dc[1] < -limit) return false;
+    dc[2] = a[2] - b[2];
+    if (dc[2] > limit || dc[2] < -limit) return false;
+    d = len3(dc);
+    return d <= limit;
+}
+
+struct Bonds {
+    std::vector<int> n;
+    std::vector<int> b;  // [atom][kMaxNbonds]
+    explicit Bonds(int natoms) : n(natoms, 0), b((size_t)natoms * kMaxNbonds, -1) {}
+    void add(int i, int j) {  // bondmanager.cpp:44-93
+        if (n[i] >= kMaxNbonds || n[j] >= kMaxNbonds) fail(AG_ERR_BAD_INPUT, "could not add bond");
+        b[(size_t)i * kMaxNbonds + n[i]++] = j;
+        b[(size_t)j * kMaxNbonds + n[j]++] = i;
+    }
+};
+
+void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int32_t *type,
+                        const int32_t *hbond, int32_t *rexp, double *rvector, double *rvector2,
+                        uint8_t *disorder, FILE *log) {
+    const int hydrogen = s.t_hydrogen, nonHB_hydrogen = s.t_nonhb_hydrogen;
+    const int carbon = s.t_carbon, arom_carbon = s.t_arom_carbon, oxygen = s.t_oxygen;
+    const int nitrogen = s.t_nitrogen, nonHB_nitrogen = s.t_nonhb_nitrogen, sulphur = s.t_sulphur;
+    const bool disorder_h = s.disorder_h != 0;
+    for (int i = 0; i < n; i++) rexp[i] = 0;
+    memset(rvector, 0, sizeof(double) * 3 * (size_t)n);
+    memset(rvector2, 0, sizeof(double) * 3 * (size_t)n);
+    memset(disorder, 0, (size_t)n);
+    Bonds bonds(n);
+    auto C = [&](int a) { return coord + 3 * (size_t)a; };
+    for (int ia = 0; ia < n; ia++) {
+        const int from = ia - 20 > 0 ? ia - 20 : 0;
+        const int to = ia + 20 < n - 1 ? ia + 20 : n - 1;
+        double *rv = rvector + 3 * (size_t)ia, *rv2 = rvector2 + 3 * (size_t)ia;
+        if (hbond[ia] == AG_HB_D1) {
+            // Donor hydrogen: every atom within 1.378 A in the window sets the
+            // bond vector (the last one wins) and rexp.
+            for (int ib = from; ib <= to; ib++) {
+                if (ib == ia) continue;
+                double d[3], rd;
+                if (!distance_le(C(ia), C(ib), 1.378, d, rd)) continue;
+                for (int i = 0; i < 3; i++) rv[i] = d[i];
+                normalize(rv);
+                if (type[ib] == nitrogen) rexp[ia] = 2;
+                else if (type[ib] == oxygen || type[ib] == sulphur) rexp[ia] = 4;
+                else rexp[ia] = 2;
+                if (disorder_h) bonds.add(ia, ib);
+            }
+        } else if (hbond[ia] == AG_HB_A2) {
+            int nbond = 0, i1 = -1, i2 = -1;
+            for (int ib = from; ib <= to; ib++) {
+                if (ib == ia) continue;
+                double dc[3], rd;
+                const double lim = (type[ib] == hydrogen || type[ib] == nonHB_hydrogen) ? 1.3 : 1.9;
+                if (!distance_le(C(ia), C(ib), lim, dc, rd)) continue;
+                if (nbond == 2) {
+                    if (log)
+                        fprintf(log, "\nWARNING: Found an H-bonding O atom with three bonded atoms, atom serial number %d\n", ia + 1);
+                } else if (nbond == 1) {
+                    nbond = 2;
+                    i2 = ib;
+                } else {
+                    nbond = 1;
+                    i1 = ib;
+                }
+            }
+            if (nbond == 0 && log)
+                fprintf(log, "\nWARNING: Oxygen atom found with no bonded atoms, atom serial number %d, atom_type %d\n",
+                        ia + 1, type[ia]);
+            if (nbond == 1) {
+                // Carbonyl O=C-X: rvector = O - C, rvector2 = (O-C) x (C-X).
+                sub_normalize(rv, C(ia), C(i1));
+                for (int j = from; j <= to; j++) {
+                    if (j == i1 || j == ia) continue;
+                    double d[3], rdcx;
+                    if (!distance_le(C(i1), C(j), type[j] == hydrogen ? 1.3 : 1.7, d, rdcx)) continue;
+                    normalize(d);
+                    rv2[0] = rv[1] * d[2] - rv[2] * d[1];
+                    rv2[1] = rv[2] * d[0] - rv[0] * d[2];
+                    rv2[2] = rv[0] * d[1] - rv[1] * d[0];
+                    normalize(rv2);
+                }
+            } else if (nbond == 2) {
+                if (disorder_h && (type[i1] == hydrogen || type[i2] == hydrogen) && type[i1] != type[i2]) {
+                    int ib;
+                    if (type[i1] == carbon || type[i1] == arom_carbon) ib = i1;
+                    else if (type[i2] == carbon || type[i2] == arom_carbon) ib = i2;
+                    else ib = i1;
+                    bonds.add(ia, ib);
+                    sub_normalize(rv, C(ia), C(ib));
+                } else {
+                    // Hydroxyl / ether X1-O-X2: lone-pair plane along X1->X2,
+                    // rvector from the X1-X2 line to O.
+                    sub_normalize(rv2, C(i2), C(i1));
+                    double rdot = 0.;
+                    for (int i = 0; i < 3; i++) rdot += (C(ia)[i] - C(i1)[i]) * rv2[i];
+                    for (int i = 0; i < 3; i++) rv[i] = C(ia)[i] - ((rdot * rv2[i]) + C(i1)[i]);
+                    normalize(rv);
+                }
+            }
+        } else if (hbond[ia] == AG_HB_A1) {
+            // Directional N acceptor: up to three bonded atoms; rvector is the
+            // negated normalised mean of the (N - X) vectors.
+            int nbond = 0;
+            double bondv[3][3];
+            for (int ib = from; ib <= to; ib++) {
+                if (ib == ia) continue;
+                double rd;
+                const double lim = (type[ib] == hydrogen || type[ib] == nonHB_hydrogen) ? 1.3 : 1.7;
+                if (!distance_le(C(ia), C(ib), lim, bondv[nbond], rd)) continue;
+                if (++nbond == 3) break;
+            }
+            if (nbond == 0 && log)
+                fprintf(log, "\nWARNING: Nitrogen atom found with no bonded atoms, atom serial number %d\n", ia + 1);
+            double mean[3] = {0, 0, 0};
+            for (int b = 0; b < nbond; b++)
+                for (int i = 0; i < 3; i++) mean[i] += bondv[b][i] / nbond;
+            normalize(mean);
+            for (int i = 0; i < 3; i++) rv[i] = -1 * mean[i];
+        }
+    }
+    if (disorder_h) {
+        // main.cpp:2684-2736
+        for (int ia = 0; ia < n; ia++) {
+            int hcount = 0;
+            for (int b = 0; b < kMaxNbonds && b < bonds.n[ia]; b++)
+                if (type[bonds.b[(size_t)ia * kMaxNbonds + b]] == hydrogen) hcount++;
+            if ((type[ia] == nitrogen && hcount == 2) || (type[ia] == nonHB_nitrogen && hcount == 3) ||
+                (type[ia] == oxygen && hcount == 1) || (type[ia] == sulphur && hcount == 1)) {
+                disorder[ia] = 1;
+                for (int b = 0; b < kMaxNbonds && b < bonds.n[ia]; b++) {
+                    int ba = bonds.b[(size_t)ia * kMaxNbonds + b];
+                    if (type[ba] == hydrogen) disorder[ba] = 1;
+                }
+            }
+        }
+    }
+}
+
+void job_prepare(Job &job) {
+    const int T = (int)job.rec_types.size();
+    const int M = job.num_atom_maps;
+    job.pairs.assign((size_t)M * T, ag_pair{});
+    for (int m = 0; m < M; m++)
+        for (int t = 0; t < T; t++) job.pairs[(size_t)m * T + t] = job.maps[m].pair[t];
+    job.map_desc.assign(M, ag_map_desc{});
+    for (int m = 0; m < M; m++) {
+        ag_map_desc &d = job.map_desc[m];
+        d.is_covalent = job.maps[m].is_covalent;
+        d.is_hbonder = job.maps[m].is_hbonder;
+        d.hbond = job.maps[m].hbond;
+        d.solpar_probe = job.maps[m].solpar_probe;
+        d.vol_probe = job.maps[m].vol_probe;
+    }
+    ag_receptor rec;
+    ag_grid grid;
+    ag_mapset ms;
+    ag_tables tab;
+    ag_setup setup;
+    // views need the vectors sized first
+    job.e_vdw_hb.assign((size_t)T * M * AG_NEINT, 0.f);
+    job.sol_fn.assign(AG_NDIEL, 0.0);
+    job.r_eps.assign(AG_NDIEL, 0.0);
+    job.rexp.assign(job.n_atoms, 0);
+    job.rvector.assign(3 * (size_t)job.n_atoms, 0.0);
+    job.rvector2.assign(3 * (size_t)job.n_atoms, 0.0);
+    job.disorder.assign(job.n_atoms, 0);
+    job.prepared = true;
+    job_views(job, &rec, &grid, &ms, &tab, &setup);
+    build_vdw_tables(setup, ms, job.e_vdw_hb.data());
+    build_sol_fn(job.lib.coeff_desolv, job.sol_fn.data());
+    if (job.dddiel) build_r_eps(job.r_eps.data());
+    build_bond_vectors(setup, job.n_atoms, job.coord.data(), job.type.data(), job.hbond.data(),
+                       job.rexp.data(), job.rvector.data(), job.rvector2.data(), job.disorder.data(),
+                       job.log);
+}
+
+}  // namespace ag

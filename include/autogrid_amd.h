@@ -1,0 +1,233 @@
+/*
+ * autogrid_amd.h -- C-ABI of the MI355X AutoGrid4 grid-map engine.
+ *
+ * The reference autogrid4 has no library interface: its hot path is inline in
+ * main() (main.cpp:2805-3464) and its only boundary is the command line plus
+ * files (setflags.cpp:96-134).  This header is the in-process seam placed at
+ * the point where the reference has finished reading the GPF and building its
+ * tables and bond vectors (main.cpp:2785), plus the front-end calls that take
+ * a GPF to that point and the writer that turns computed maps into the
+ * reference's .map files.
+ *
+ * Conventions
+ *   - Plain C types only; every array is a caller-visible pointer plus a size.
+ *   - Functions return ag_status; the message of the last failure on the
+ *     calling thread is returned by ag_last_error().  Nothing calls exit().
+ *   - Map layout everywhere: maps[m][iz][iy][ix], x fastest, the index of a
+ *     point is ix + n1x*(iy + n1y*iz) (main.cpp:322).
+ *   - Atom coordinates are in the grid frame: PDBQT coordinate minus grid
+ *     centre (main.cpp:1135-1139).  Grid point (ix,iy,iz) sits at
+ *     ((ix-ne_x)*spacing, (iy-ne_y)*spacing, (iz-ne_z)*spacing)
+ *     (main.cpp:2815-2859).
+ */
+#ifndef AUTOGRID_AMD_H
+#define AUTOGRID_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AG_ABI_VERSION 1
+
+/* Table sizes of the 8 A build (constants.h:148-171, USE_8A_NBCUTOFF). */
+#define AG_A_DIV 128           /* bins per Angstrom                         */
+#define AG_NEINT 1024          /* vdW/H-bond table length: 8 A * 128        */
+#define AG_NDIEL 8192          /* dielectric / desolvation table length     */
+#define AG_MAX_REC_TYPES 20    /* NUM_RECEPTOR_TYPES, autogrid.h:95         */
+#define AG_MAX_MAPS 128        /* MAX_MAPS, autocomm.h:94                   */
+#define AG_MAX_ATOMS 100000    /* AG_MAX_ATOMS, autogrid.h:58               */
+#define AG_MAX_GRID_PTS 1025   /* MAX_GRID_PTS, autocomm.h:87               */
+
+typedef enum ag_status {
+    AG_OK = 0,
+    AG_ERR_BAD_INPUT = 1,   /* malformed GPF/PDBQT/parameters, bad sizes   */
+    AG_ERR_NO_DONOR_H = 2,  /* affinity maps requested, no DS/D1 atom      */
+    AG_ERR_OOM = 3,         /* host or device allocation failed            */
+    AG_ERR_DEVICE = 4,      /* HIP error, no device, kernel fault          */
+    AG_ERR_IO = 5,          /* file could not be opened or written         */
+    AG_ERR_UNSUPPORTED = 6  /* request outside what this build implements  */
+} ag_status;
+
+/* H-bond classes of parameters.h:36-37 plus the AD code the hot loop tests. */
+enum { AG_HB_NON = 0, AG_HB_DS = 1, AG_HB_D1 = 2, AG_HB_AS = 3,
+       AG_HB_A1 = 4, AG_HB_A2 = 5, AG_HB_AD = 6 };
+
+/* ------------------------------------------------------------------------ */
+/* Prepared state: the in-process seam (main.cpp:2785).                      */
+/* ------------------------------------------------------------------------ */
+
+/* Receptor atoms.  Replaces the static arrays of main.cpp:359-380. */
+typedef struct ag_receptor {
+    int32_t        n_atoms;
+    const double  *coord;     /* [n][3] grid frame                         */
+    const double  *charge;    /* [n] e                                     */
+    const double  *vol;       /* [n] atomic fragmental volume              */
+    const double  *solpar;    /* [n] atomic solvation parameter            */
+    const int32_t *type;      /* [n] receptor type index (rec_index)       */
+    const int32_t *hbond;     /* [n] AG_HB_*                               */
+    const int32_t *rexp;      /* [n] donor exponent (main.cpp:2360-2390)   */
+    const double  *rvector;   /* [n][3] bond vector (main.cpp:2297-2664)   */
+    const double  *rvector2;  /* [n][3] lone-pair plane vector             */
+    const uint8_t *disorder;  /* [n] disordered hydroxyl flags             */
+} ag_receptor;
+
+/* Grid.  n1 = nelements+1 points per axis, ne = nelements/2 (main.cpp:1086). */
+typedef struct ag_grid {
+    int32_t n1[3];
+    int32_t ne[3];
+    double  spacing;
+} ag_grid;
+
+/* One ligand-type map (MapObject, main.cpp:289-315). */
+typedef struct ag_map_desc {
+    int32_t is_covalent;
+    int32_t is_hbonder;
+    int32_t hbond;            /* AG_HB_* of the probe type                 */
+    int32_t reserved;
+    double  solpar_probe;
+    double  vol_probe;
+} ag_map_desc;
+
+/* The map set and the scalars the hot loop reads. */
+typedef struct ag_mapset {
+    int32_t n_atom_maps;      /* ligand maps incl. covalent and separate
+                                 desolvation maps (num_atom_maps)           */
+    const ag_map_desc *maps;  /* [n_atom_maps]                             */
+    int32_t elec_index;       /* output slot of the e map, or -1 (elecPE)  */
+    int32_t dsolv_index;      /* output slot of the d map, or -1 (dsolvPE) */
+    int32_t n_maps;           /* n_atom_maps + (e) + (d)  (num_maps)       */
+    int32_t separate_desolvation;
+    int32_t disorder_h;
+    int32_t hydrogen_type;    /* rec index of HD, -1 if none (main.cpp:2280) */
+    int32_t dddiel;           /* 1: distance-dependent dielectric          */
+    int32_t floating_grid;    /* 1: also produce r_min (fmap)              */
+    int32_t constriction_grid;/* 1: also produce the atom count (cmap)     */
+    int32_t reserved;
+    double  inv_diel_cal;     /* 332/diel for a constant dielectric        */
+    double  coeff_estat;
+    double  solpar_q;         /* qasp, charge term of desolvation          */
+    double  cov_pos[3];       /* covalent attachment point, grid frame     */
+    double  cov_half_width;
+    double  cov_barrier;
+    double  constriction_cutoff;
+} ag_mapset;
+
+/* Lookup tables (EnergyTables, structs.h:257-268). */
+typedef struct ag_tables {
+    int32_t       n_rec_types;
+    int32_t       reserved;
+    const float  *e_vdw_hb;   /* [n_rec_types][n_atom_maps][AG_NEINT]      */
+    const double *sol_fn;     /* [AG_NDIEL] desolvation Gaussian           */
+    const double *r_eps;      /* [AG_NDIEL] 332/eps(r) (dddiel only)       */
+} ag_tables;
+
+/* One (map, receptor type) Lennard-Jones pair after GPF processing
+ * (main.cpp:1315-1357, nbp_r_eps / nbp_coeffs main.cpp:1741-1806).          */
+typedef struct ag_pair {
+    double  cA, cB;           /* both 0: derive from Rij, eps              */
+    double  Rij, eps;         /* eps already weighted                      */
+    int32_t xA, xB;
+    int32_t hbonder;
+    int32_t reserved;
+} ag_pair;
+
+/* What the table and bond-vector builders read (the state at main.cpp:1819). */
+typedef struct ag_setup {
+    int32_t        n_rec_types;
+    int32_t        n_atom_maps;
+    const ag_pair *pairs;     /* [n_atom_maps][n_rec_types]                */
+    double         r_smooth;
+    double         coeff_desolv;
+    int32_t        dddiel;
+    int32_t        disorder_h;
+    /* receptor type indices used by the bond-vector pass, -1 if absent
+       (main.cpp:2280-2288) */
+    int32_t t_hydrogen, t_nonhb_hydrogen, t_carbon, t_arom_carbon;
+    int32_t t_oxygen, t_sulphur, t_nitrogen, t_nonhb_nitrogen;
+} ag_setup;
+
+/* ------------------------------------------------------------------------ */
+/* Context: one GPU, one stream, device scratch reused across calls.        */
+/* ------------------------------------------------------------------------ */
+typedef struct ag_ctx ag_ctx;
+
+/* Create a context on HIP device `device`.  Fails with AG_ERR_DEVICE when no
+ * device is usable; there is no CPU fallback. */
+ag_status ag_context_create(ag_ctx **out, int device);
+ag_status ag_context_destroy(ag_ctx *ctx);
+const char *ag_last_error(void);
+int ag_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Front end: GPF + PDBQT + parameter library -> job.                       */
+/* ------------------------------------------------------------------------ */
+typedef struct ag_job ag_job;
+
+/* Flags of ag_job_open. */
+#define AG_JOB_NO_FILES 1u    /* parse only: open no map/.fld/.xyz files   */
+
+/* Read a GPF with the keyword semantics of main.cpp:778-1819 (order
+ * dependence included), read the receptor PDBQT (main.cpp:823-1047) and any
+ * parameter_file, and, unless AG_JOB_NO_FILES, create the output files at
+ * the points the reference does.  `glg_path` may be NULL (log discarded);
+ * `program_name` is what the log and .fld print as the program (argv[0]). */
+ag_status ag_job_open(const char *gpf_path, const char *glg_path,
+                      const char *program_name, uint32_t flags, ag_job **out);
+/* Build e_vdW_Hb / sol_fn / r_eps (main.cpp:1607-1639, 2040-2276) and the
+ * bond vectors and disorder flags (main.cpp:2297-2736) on the host. */
+ag_status ag_job_prepare(ag_job *job);
+ag_status ag_job_views(const ag_job *job, ag_receptor *rec, ag_grid *grid,
+                       ag_mapset *mapset, ag_tables *tables, ag_setup *setup);
+/* Points per map (n1x*n1y*n1z). */
+int64_t ag_job_points(const ag_job *job);
+void ag_job_close(ag_job *job);
+
+/* Whole drop-in: open + prepare + compute on ctx + write every output file +
+ * log "Successful Completion" (main.cpp:3565).  On failure the log ends with
+ * the reference's "Unsuccessful Completion" line (main.cpp:173-176).  With
+ * ctx NULL a context on device 0 is created after the inputs are read.  An
+ * empty gpf_path reads the GPF from standard input (setflags.cpp:78). */
+ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path,
+                     const char *program_name);
+
+/* ------------------------------------------------------------------------ */
+/* Hot path.                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Compute every map of the set for one receptor on one grid
+ * (main.cpp:2805-3431): covalent maps, the electrostatic map (all atoms, no
+ * cutoff), closest donor-H, directional H-bond terms, vdW/H-bond table
+ * lookups within 8 A, desolvation, the d map and the H-bond min/max fix-up.
+ *
+ * maps  : [n_maps][P] float; slot order as in the GPF (atom maps, then e and
+ *         d at elec_index / dsolv_index).
+ * fmap  : [P] float, r_min of the floating grid, or NULL.
+ * cmap  : [P] int32, atoms within constriction_cutoff, or NULL.
+ * Output pointers are device pointers when `outputs_on_device` is nonzero,
+ * else host pointers (the result is copied back before return).
+ * Inputs are host pointers; they are staged to the device on ctx's stream. */
+ag_status ag_maps_plain(ag_ctx *ctx, const ag_receptor *rec,
+                        const ag_grid *grid, const ag_mapset *mapset,
+                        const ag_tables *tables, float *maps, float *fmap,
+                        int32_t *cmap, int outputs_on_device);
+
+/* Map output: the value text of main.cpp:3486-3497 -- "0" when |e| < 1e-4,
+ * else "%.3f" of (float)(floor(e*1000+0.5)/1000) -- for n values into buf.
+ * Returns the number of bytes written (each value ends in '\n'), or -1 if
+ * cap is too small.  Needs at most 16*n bytes for finite |e| < 1e12. */
+int64_t ag_format_map_values(const float *values, int64_t n, char *buf,
+                             int64_t cap);
+
+/* Copy the per-job maps from a computed [n_maps][P] host array into the
+ * job's .map files (+ fmap/cmap), then log the min/max summary and the
+ * completion line.  Formatting runs on `n_threads` host threads. */
+ag_status ag_job_write_maps(ag_job *job, const float *maps, const float *fmap,
+                            const int32_t *cmap, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AUTOGRID_AMD_H */

@@ -1,0 +1,515 @@
+/*
+ * ag_oracle.c -- FP64 CPU restatement of autogrid4's grid-map path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the MI355X engine.  Only tests,
+ * the smoke check and the benchmark's CPU-baseline leg may load it.
+ *
+ * Parity pins (see tests/test_oracle.py):
+ *   - the compiled reference binary (oracle/_ref/autogrid4) on the
+ *     reference's own GPFs and on seeded receptors: every .map value line
+ *     must be byte-identical;
+ *   - PyAutoDock maps held by the reference's tests (Tests/py_hsg1_sm.*.map,
+ *     copied as fixtures) at the reference's tolerance with the sign fixed;
+ *   - closed forms: eps(r) against the Mehler-Solmajer formula, the LJ
+ *     coefficient known answer of Tests/hsg1_sm_nbpcoeffs.gpf, a one-atom
+ *     receptor's radial e map, min-window smoothing against brute force.
+ */
+#include "ag_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define APPROX_ZERO 1.0e-6  /* autocomm.h:72 */
+#define EINTCLAMP 100000.0  /* autocomm.h:89 */
+#define NEINT 1024          /* constants.h:160 */
+#define NDIEL 8192          /* constants.h:171 */
+#define A_DIV 128           /* constants.h:151 */
+#define SOFTNBC 8.0         /* constants.h:150 */
+#define MAXNB 4             /* AG_MAX_NBONDS, autogrid.h:59 */
+
+static double dmin(double a, double b) { return a < b ? a : b; }
+static double dmax(double a, double b) { return a > b ? a : b; }
+
+/* main.cpp:2040-2266 */
+int orc_vdw_tables(const ag_setup *s, const ag_mapset *ms, float *out) {
+    const int T = s->n_rec_types, M = s->n_atom_maps;
+    const int i_smooth = (int)(s->r_smooth * A_DIV / 2.);
+    float e[NEINT];
+    double sm[NEINT];
+    memset(out, 0, sizeof(float) * (size_t)T * M * NEINT);
+    for (int ia = 0; ia < M; ia++) {
+        if (ms->maps[ia].is_covalent) continue;
+        for (int i = 0; i < T; i++) {
+            const ag_pair *p = &s->pairs[(size_t)ia * T + i];
+            double cA = p->cA, cB = p->cB;
+            const int xA = p->xA, xB = p->xB;
+            if (cA == 0 && cB == 0) {
+                double tmpconst;
+                cA = (tmpconst = p->eps / (double)(xA - xB)) * pow(p->Rij, (double)xA) * (double)xB;
+                cB = tmpconst * pow(p->Rij, (double)xB) * (double)xA;
+            }
+            if (isnan(cA) || isnan(cB) || xA == 0 || xB == 0) return -1;
+            for (int k = 1; k < NEINT; k++) {
+                double r = ((double)k) * (1. / A_DIV);
+                double rA = pow(r, (double)xA);
+                double rB = pow(r, (double)xB);
+                double v = cA / rA - cB / rB;
+                e[k] = (float)((EINTCLAMP < v) ? EINTCLAMP : v);
+            }
+            e[0] = (float)EINTCLAMP;
+            e[NEINT - 1] = 0.f;
+            if (i_smooth > 0) {
+                for (int k = 0; k < NEINT; k++) {
+                    sm[k] = 100000.;
+                    int j0 = k - i_smooth > 0 ? k - i_smooth : 0;
+                    int j1 = k + i_smooth + 1 < NEINT ? k + i_smooth + 1 : NEINT;
+                    for (int j = j0; j < j1; j++) sm[k] = (sm[k] < e[j]) ? sm[k] : e[j];
+                }
+                for (int k = 0; k < NEINT; k++) e[k] = (float)sm[k];
+            }
+            memcpy(out + ((size_t)i * M + ia) * NEINT, e, sizeof e);
+        }
+    }
+    return 0;
+}
+
+/* main.cpp:2272-2276 */
+void orc_sol_fn(double coeff_desolv, double *out) {
+    double sigma = 3.6;
+    out[0] = 0.0;
+    for (int k = 1; k < NDIEL; k++) {
+        double r = ((double)k) * (1. / A_DIV);
+        out[k] = coeff_desolv * exp(-(r * r) / (2. * (sigma * sigma)));
+    }
+}
+
+/* distdepdiel.cpp:30-57; the denominator is long double as in the reference */
+double orc_ddd(double distance) {
+    double lambda = 0.003627L, epsilon0 = 78.4L, A = -8.5525L, rk = 7.7839L;
+    double B = epsilon0 - A;
+    double lambda_B = -lambda * B;
+    double epsilon = A + B / (1.0L + rk * exp(lambda_B * distance));
+    if (epsilon < APPROX_ZERO) epsilon = 1.0L;
+    return epsilon;
+}
+
+/* main.cpp:1612-1628 */
+void orc_r_eps(double *out) {
+    double eps[NDIEL];
+    eps[0] = 1.0;
+    for (int k = 1; k < NDIEL; k++) eps[k] = orc_ddd(((double)k) * (1. / A_DIV));
+    for (int k = 0; k < NDIEL; k++) out[k] = 332.0 / eps[k];
+}
+
+/* --- bond vectors ------------------------------------------------------ */
+
+static double len3(const double *v) { return hypot(v[0], hypot(v[1], v[2])); } /* vect3len */
+
+static double vnorm(double *v) { /* vect_normalize, main.cpp:3606 */
+    double rd = len3(v), inv;
+    if (rd < APPROX_ZERO) inv = 1. / APPROX_ZERO;
+    else inv = 1. / rd;
+    for (int i = 0; i < 3; i++) v[i] *= inv;
+    return rd;
+}
+
+static double vsubnorm(double *res, const double *a, const double *b) { /* main.cpp:3620 */
+    double rd2 = 0., rd, inv;
+    for (int i = 0; i < 3; i++) {
+        res[i] = a[i] - b[i];
+        rd2 += res[i] * res[i];
+    }
+    if (rd2 < APPROX_ZERO) rd = APPROX_ZERO;
+    else rd = sqrt(rd2);
+    inv = 1. / rd;
+    for (int i = 0; i < 3; i++) res[i] *= inv;
+    return rd;
+}
+
+static int dist_le(const double *a, const double *b, double lim, double *dc, double *d) { /* main.cpp:121 */
+    if ((dc[0] = a[0] - b[0]) > lim || dc[0] < -lim) return 0;
+    if ((dc[1] = a[1] - b[1]) > lim || dc[1] < -lim) return 0;
+    if ((dc[2] = a[2] - b[2]) > lim || dc[2] < -lim) return 0;
+    *d = len3(dc);
+    return *d <= lim;
+}
+
+int orc_bond_vectors(const ag_setup *s, int n, const double *coord, const int32_t *type,
+                     const int32_t *hbond, int32_t *rexp, double *rvector, double *rvector2,
+                     uint8_t *disorder) {
+    const int hydrogen = s->t_hydrogen, nonHB_hydrogen = s->t_nonhb_hydrogen, carbon = s->t_carbon;
+    const int arom_carbon = s->t_arom_carbon, oxygen = s->t_oxygen, nitrogen = s->t_nitrogen;
+    const int nonHB_nitrogen = s->t_nonhb_nitrogen, sulphur = s->t_sulphur;
+    int *nb = calloc((size_t)n + 1, sizeof(int));
+    int *bd = calloc((size_t)n * MAXNB + 1, sizeof(int));
+    int rc = 0;
+    memset(rexp, 0, sizeof(int32_t) * (size_t)n);
+    memset(rvector, 0, sizeof(double) * 3 * (size_t)n);
+    memset(rvector2, 0, sizeof(double) * 3 * (size_t)n);
+    memset(disorder, 0, (size_t)n);
+#define C(a) (coord + 3 * (size_t)(a))
+#define ADDBOND(i, j)                                                   \
+    do {                                                                \
+        if (nb[i] >= MAXNB || nb[j] >= MAXNB) { rc = -1; goto done; }   \
+        bd[(size_t)(i) * MAXNB + nb[i]++] = (j);                        \
+        bd[(size_t)(j) * MAXNB + nb[j]++] = (i);                        \
+    } while (0)
+    for (int ia = 0; ia < n; ia++) {
+        int from = ia - 20 > 0 ? ia - 20 : 0;
+        int to = ia + 20 < n - 1 ? ia + 20 : n - 1;
+        double *rv = rvector + 3 * (size_t)ia, *rv2 = rvector2 + 3 * (size_t)ia;
+        if (hbond[ia] == 2) { /* D1, main.cpp:2320-2414 */
+            for (int ib = from; ib <= to; ib++) {
+                double d[3], rd;
+                if (ib == ia) continue;
+                if (!dist_le(C(ia), C(ib), 1.378, d, &rd)) continue;
+                for (int i = 0; i < 3; i++) rv[i] = d[i];
+                vnorm(rv);
+                if (type[ib] == nitrogen) rexp[ia] = 2;
+                else if (type[ib] == oxygen || type[ib] == sulphur) rexp[ia] = 4;
+                else rexp[ia] = 2;
+                if (s->disorder_h) ADDBOND(ia, ib);
+            }
+        } else if (hbond[ia] == 5) { /* A2, main.cpp:2416-2594 */
+            int nbond = 0, i1 = 0, i2 = 0;
+            for (int ib = from; ib <= to; ib++) {
+                double dc[3], rd;
+                if (ib == ia) continue;
+                if (!dist_le(C(ia), C(ib), (type[ib] == hydrogen || type[ib] == nonHB_hydrogen) ? 1.3 : 1.9, dc, &rd))
+                    continue;
+                if (nbond == 2) {
+                } else if (nbond == 1) {
+                    nbond = 2;
+                    i2 = ib;
+                } else if (nbond == 0) {
+                    nbond = 1;
+                    i1 = ib;
+                }
+            }
+            if (nbond == 1) {
+                vsubnorm(rv, C(ia), C(i1));
+                for (i2 = from; i2 <= to; i2++) {
+                    double rdcx, d[3];
+                    if (i2 == i1 || i2 == ia) continue;
+                    if (!dist_le(C(i1), C(i2), type[i2] == hydrogen ? 1.3 : 1.7, d, &rdcx)) continue;
+                    vnorm(d);
+                    rv2[0] = rv[1] * d[2] - rv[2] * d[1];
+                    rv2[1] = rv[2] * d[0] - rv[0] * d[2];
+                    rv2[2] = rv[0] * d[1] - rv[1] * d[0];
+                    vnorm(rv2);
+                }
+            } else if (nbond == 2) {
+                if (s->disorder_h && (type[i1] == hydrogen || type[i2] == hydrogen) && type[i1] != type[i2]) {
+                    int ib;
+                    if (type[i1] == carbon || type[i1] == arom_carbon) ib = i1;
+                    else if (type[i2] == carbon || type[i2] == arom_carbon) ib = i2;
+                    else ib = i1;
+                    ADDBOND(ia, ib);
+                    vsubnorm(rv, C(ia), C(ib));
+                } else {
+                    double rdot = 0.;
+                    vsubnorm(rv2, C(i2), C(i1));
+                    for (int i = 0; i < 3; i++) rdot += (C(ia)[i] - C(i1)[i]) * rv2[i];
+                    for (int i = 0; i < 3; i++) rv[i] = C(ia)[i] - ((rdot * rv2[i]) + C(i1)[i]);
+                    vnorm(rv);
+                }
+            }
+        } else if (hbond[ia] == 4) { /* A1, main.cpp:2596-2657 */
+            int nbond = 0;
+            double bondv[3][3], bondl[3], mean[3] = {0., 0., 0.};
+            for (int ib = from; ib <= to; ib++) {
+                if (ib == ia) continue;
+                if (!dist_le(C(ia), C(ib), (type[ib] == hydrogen || type[ib] == nonHB_hydrogen) ? 1.3 : 1.7,
+                             bondv[nbond], &bondl[nbond]))
+                    continue;
+                if (++nbond == 3) break;
+            }
+            for (int b = 0; b < nbond; b++)
+                for (int i = 0; i < 3; i++) mean[i] += (bondv[b][i]) / nbond;
+            vnorm(mean);
+            for (int i = 0; i < 3; i++) rv[i] = -1 * mean[i];
+        }
+    }
+    if (s->disorder_h) { /* main.cpp:2684-2736 */
+        for (int ia = 0; ia < n; ia++) {
+            int hcount = 0;
+            for (int b = 0; b < MAXNB && b < nb[ia]; b++)
+                if (type[bd[(size_t)ia * MAXNB + b]] == hydrogen) hcount++;
+            if ((type[ia] == nitrogen && hcount == 2) || (type[ia] == nonHB_nitrogen && hcount == 3) ||
+                (type[ia] == oxygen && hcount == 1) || (type[ia] == sulphur && hcount == 1)) {
+                disorder[ia] = 1;
+                for (int b = 0; b < MAXNB && b < nb[ia]; b++) {
+                    int ba = bd[(size_t)ia * MAXNB + b];
+                    if (type[ba] == hydrogen) disorder[ba] = 1;
+                }
+            }
+        }
+    }
+done:
+#undef ADDBOND
+#undef C
+    free(nb);
+    free(bd);
+    return rc;
+}
+
+/* --- per-point loop, main.cpp:2805-3431 --------------------------------- */
+
+int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,
+             double *maps, double *fmap, int32_t *cmap, int z0, int z1) {
+    const int N = rec->n_atoms, M = ms->n_atom_maps;
+    const int n1x = grid->n1[0], n1y = grid->n1[1];
+    const long long P = (long long)grid->n1[0] * grid->n1[1] * grid->n1[2];
+    const double spacing = grid->spacing;
+    const int elecPE = ms->elec_index, dsolvPE = ms->dsolv_index;
+    const int hydrogen = ms->hydrogen_type;
+    const double PI_halved = 3.14159265358979323846 / 2.;
+    const double ln_half = log(0.5);
+    int nh12 = 0;
+    int *h12 = malloc(sizeof(int) * ((size_t)N + 1));
+    for (int i = 0; i < N; i++)
+        if (rec->hbond[i] == 1 || rec->hbond[i] == 2) h12[nh12++] = i;
+    if (M > 0 && nh12 == 0) {
+        free(h12);
+        return -1;
+    }
+    const float *evdw = tab->e_vdw_hb;
+#define EV(t, m, k) ((double)evdw[((size_t)(t) * M + (m)) * NEINT + (k)])
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int iz = z0; iz < z1; iz++) {
+        double c[3];
+        double hbondmin[AG_MAX_MAPS], hbondmax[AG_MAX_MAPS];
+        int hbondflag[AG_MAX_MAPS];
+        c[2] = ((double)(iz - grid->ne[2])) * spacing;
+        for (int iy = 0; iy < n1y; iy++) {
+            c[1] = ((double)(iy - grid->ne[1])) * spacing;
+            for (int ix = 0; ix < n1x; ix++) {
+                const long long mapi = ix + (long long)n1x * (iy + (long long)n1y * iz);
+                double d[3];
+                c[0] = ((double)(ix - grid->ne[0])) * spacing;
+                /* covalent maps (main.cpp:2866-2879) */
+                for (int j = 0; j < M; j++)
+                    if (ms->maps[j].is_covalent) {
+                        double rcov;
+                        for (int ii = 0; ii < 3; ii++) d[ii] = ms->cov_pos[ii] - c[ii];
+                        rcov = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / ms->cov_half_width;
+                        if (rcov < APPROX_ZERO) rcov = APPROX_ZERO;
+                        maps[(size_t)j * P + mapi] = ms->cov_barrier * (1. - exp(ln_half * rcov * rcov));
+                    } else {
+                        maps[(size_t)j * P + mapi] = 0.;
+                    }
+                if (elecPE >= 0) maps[(size_t)elecPE * P + mapi] = 0.;
+                if (dsolvPE >= 0) maps[(size_t)dsolvPE * P + mapi] = 0.;
+                if (fmap) fmap[mapi] = 1.0e12;
+                if (cmap) cmap[mapi] = 0;
+                for (int m = 0; m < M; m++) {
+                    hbondmin[m] = 999999.;
+                    hbondmax[m] = -999999.;
+                    hbondflag[m] = 0;
+                }
+                /* Loop 1 (main.cpp:2940-2982): all atoms, hypot distance */
+                if (elecPE >= 0 || fmap || cmap) {
+                    double e = 0;
+                    for (int ia = 0; ia < N; ia++) {
+                        const double *a = rec->coord + 3 * (size_t)ia;
+                        double r = hypot((a[0] - c[0]), hypot((a[1] - c[1]), (a[2] - c[2])));
+                        if (r < APPROX_ZERO) r = APPROX_ZERO;
+                        double inv_rmax = 1. / dmax(r, 0.5);
+                        int indx_r = (int)(r * A_DIV);
+                        if (indx_r > NDIEL - 1) indx_r = NDIEL - 1;
+                        if (fmap) fmap[mapi] = dmin(r, fmap[mapi]);
+                        if (cmap && r <= ms->constriction_cutoff) cmap[mapi]++;
+                        if (elecPE >= 0) {
+                            if (ms->dddiel) e += rec->charge[ia] * inv_rmax * tab->r_eps[indx_r] * ms->coeff_estat;
+                            else e += rec->charge[ia] * inv_rmax * ms->inv_diel_cal * ms->coeff_estat;
+                        }
+                    }
+                    if (elecPE >= 0) maps[(size_t)elecPE * P + mapi] = e;
+                }
+                /* closest donor H (main.cpp:2985-3015) */
+                double rminH = 999999.;
+                int closestH = -1;
+                if (M > 0)
+                    for (int inh = 0; inh < nh12; inh++) {
+                        int ia = h12[inh];
+                        int brk = 0;
+                        double dd[3];
+                        for (int i = 0; i < 3; i++) {
+                            dd[i] = rec->coord[3 * (size_t)ia + i] - c[i];
+                            if (fabs(dd[i]) > rminH) { brk = 1; break; }
+                        }
+                        if (brk) continue;
+                        double r = sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+                        if (r < rminH) {
+                            rminH = r;
+                            closestH = ia;
+                        }
+                    }
+                if (!(M > 0 || dsolvPE >= 0)) continue;
+                /* Loop 2 (main.cpp:3017-3404) */
+                for (int ia = 0; ia < N; ia++) {
+                    const double *a = rec->coord + 3 * (size_t)ia;
+                    const double *rv = rec->rvector + 3 * (size_t)ia;
+                    const double *rv2 = rec->rvector2 + 3 * (size_t)ia;
+                    const int hb = rec->hbond[ia], at = rec->type[ia];
+                    const int dis = rec->disorder[ia] != 0;
+                    double dnorm[3], racc, rdon, Hramp;
+                    for (int i = 0; i < 3; i++) d[i] = a[i] - c[i];
+                    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    if (r > SOFTNBC) continue;
+                    if (ms->disorder_h && dis && at == hydrogen) continue;
+                    if (r < APPROX_ZERO) r = APPROX_ZERO;
+                    double inv_r = 1. / r;
+                    for (int i = 0; i < 3; i++) dnorm[i] = d[i] * inv_r;
+                    int indx_r = (int)(r * A_DIV);
+                    if (indx_r > NDIEL - 1) indx_r = NDIEL - 1;
+                    int indx_n = (int)(r * A_DIV);
+                    if (indx_n > NEINT - 1) indx_n = NEINT - 1;
+                    racc = 1.;
+                    rdon = 1.;
+                    Hramp = 1.;
+                    if (hb == 2) { /* D1 */
+                        double cos_theta = 0.;
+                        for (int i = 0; i < 3; i++) cos_theta -= dnorm[i] * rv[i];
+                        if (cos_theta <= 0.) {
+                            racc = 0.;
+                        } else {
+                            switch (rec->rexp[ia]) {
+                                double tmp;
+                                case 1:
+                                default: racc = cos_theta; break;
+                                case 2: racc = cos_theta * cos_theta; break;
+                                case 4:
+                                    tmp = cos_theta * cos_theta;
+                                    racc = tmp * tmp;
+                                    break;
+                            }
+                            if (ia == closestH || closestH < 0) {
+                                /* closestH < 0 only without atom maps, where Hramp is unused */
+                                Hramp = 1.;
+                            } else {
+                                const double *rh = rec->rvector + 3 * (size_t)closestH;
+                                double ct = 0.;
+                                for (int i = 0; i < 3; i++) ct += rh[i] * rv[i];
+                                ct = dmin(ct, 1.0);
+                                ct = dmax(ct, -1.0);
+                                double theta = acos(ct);
+                                Hramp = 0.5 - 0.5 * cos(theta * 120. / 90.);
+                            }
+                        }
+                    } else if (hb == 4) { /* A1 */
+                        double cos_theta = 0.;
+                        for (int i = 0; i < 3; i++) cos_theta -= dnorm[i] * rv[i];
+                        if (cos_theta <= 0.) rdon = 0.;
+                        else rdon = cos_theta * cos_theta;
+                    } else if (hb == 5 && !dis) { /* A2 */
+                        double t0, ti, cos_theta = 0.;
+                        for (int i = 0; i < 3; i++) cos_theta -= dnorm[i] * rv[i];
+                        t0 = 0.;
+                        for (int i = 0; i < 3; i++) t0 += dnorm[i] * rv2[i];
+                        if (t0 > 1.) t0 = 1.;
+                        else if (t0 < -1.) t0 = -1.;
+                        t0 = PI_halved - acos(t0);
+                        double cross[3];
+                        cross[0] = dnorm[1] * rv2[2] - dnorm[2] * rv2[1];
+                        cross[1] = dnorm[2] * rv2[0] - dnorm[0] * rv2[2];
+                        cross[2] = dnorm[0] * rv2[1] - dnorm[1] * rv2[0];
+                        double rd2 = cross[0] * cross[0] + cross[1] * cross[1] + cross[2] * cross[2];
+                        if (rd2 < APPROX_ZERO) rd2 = APPROX_ZERO;
+                        double inv_rd = 1. / sqrt(rd2);
+                        ti = 0.;
+                        for (int i = 0; i < 3; i++) ti += cross[i] * inv_rd * rv[i];
+                        rdon = 0.;
+                        if (cos_theta >= 0.) {
+                            if (ti > 1.) ti = 1.;
+                            else if (ti < -1.) ti = -1.;
+                            ti = acos(ti) - PI_halved;
+                            if (ti < 0.) ti = -ti;
+                            rdon = (0.9 + 0.1 * sin(ti + ti)) * cos(t0);
+                        } else if (cos_theta >= -0.34202) {
+                            rdon = 562.25 * pow(0.116978 - cos_theta * cos_theta, 3.) * cos(t0);
+                        }
+                    } else if (dis && hb == 5) { /* disordered hydroxyl */
+                        double cos_theta = 0.;
+                        for (int i = 0; i < 3; i++) cos_theta -= dnorm[i] * rv[i];
+                        if (cos_theta > 1.) cos_theta = 1.;
+                        else if (cos_theta < -1.) cos_theta = -1.;
+                        double theta = acos(cos_theta);
+                        racc = 0.;
+                        rdon = 0.;
+                        if (theta <= 1.24791 + PI_halved) {
+                            rdon = pow(cos(theta - 1.24791), 4.);
+                            racc = rdon;
+                        }
+                    }
+                    for (int m = 0; m < M; m++) {
+                        const ag_map_desc *md = &ms->maps[m];
+                        if (md->is_covalent) continue;
+                        double *E = &maps[(size_t)m * P + mapi];
+                        if (md->is_hbonder) {
+                            double rsph = EV(at, m, indx_n) / 100.;
+                            rsph = dmax(rsph, 0.);
+                            rsph = dmin(rsph, 1.);
+                            const int mh = md->hbond;
+                            if ((mh == 3 || mh == 5 || mh == 6) && (hb == 1 || hb == 2 || hb == 6)) {
+                                if (!dis) {
+                                    *E += EV(at, m, indx_n) * Hramp * (racc + (1. - racc) * rsph);
+                                } else {
+                                    int indx_h = (int)((r - 1.10) * A_DIV);
+                                    indx_h = indx_h > 0 ? indx_h : 0;
+                                    indx_h = indx_h < NEINT - 1 ? indx_h : NEINT - 1;
+                                    *E += EV(hydrogen, m, indx_h) * Hramp * (racc + (1. - racc) * rsph);
+                                }
+                            } else if ((mh == 4 || mh == 6) && (hb == 1 || hb == 2 || hb == 6)) {
+                                hbondmin[m] = dmin(hbondmin[m], EV(at, m, indx_n) * (racc + (1. - racc) * rsph));
+                                hbondmax[m] = dmax(hbondmax[m], EV(at, m, indx_n) * (racc + (1. - racc) * rsph));
+                                hbondflag[m] = 1;
+                            } else if ((mh == 1 || mh == 2 || mh == 6) && (hb > 2 || hb == 6)) {
+                                if (dis) {
+                                    int indx_h = (int)((r - 1.10) * A_DIV);
+                                    indx_h = indx_h > 0 ? indx_h : 0;
+                                    indx_h = indx_h < NEINT - 1 ? indx_h : NEINT - 1;
+                                    *E += EV(hydrogen, m, indx_h) * Hramp * (rdon + (1. - rdon) * rsph);
+                                } else {
+                                    double t = EV(at, m, indx_n) * (rdon + (1. - rdon) * rsph);
+                                    hbondmin[m] = dmin(hbondmin[m], t);
+                                    hbondmax[m] = dmax(hbondmax[m], t);
+                                    hbondflag[m] = 1;
+                                }
+                            } else {
+                                *E += EV(at, m, indx_n);
+                            }
+                        } else {
+                            *E += EV(at, m, indx_n);
+                        }
+                        int md2 = m;
+                        if (ms->separate_desolvation) md2 = ++m;
+                        maps[(size_t)md2 * P + mapi] +=
+                            ms->maps[md2].solpar_probe * rec->vol[ia] * tab->sol_fn[indx_r] +
+                            (rec->solpar[ia] + ms->solpar_q * fabs(rec->charge[ia])) * ms->maps[md2].vol_probe *
+                                tab->sol_fn[indx_r];
+                    }
+                    if (dsolvPE >= 0)
+                        maps[(size_t)dsolvPE * P + mapi] += ms->solpar_q * rec->vol[ia] * tab->sol_fn[indx_r];
+                }
+                /* min/max fix-up (main.cpp:3420-3431) */
+                for (int m = 0; m < M; m++)
+                    if (hbondflag[m]) maps[(size_t)m * P + mapi] += (hbondmin[m] + dmax(hbondmax[m], 0));
+            }
+        }
+    }
+#undef EV
+    free(h12);
+    return 0;
+}
+
+/* main.cpp:3486-3497 with HAVE_ROUND unset (main.cpp:85-89) */
+int orc_format_value(double e, char *out) {
+    if (fabs(e) < 0.0001) {
+        strcpy(out, "0\n");
+        return 2;
+    }
+    return sprintf(out, "%.3f\n", (float)(floor(e * 1000.0 + 0.5) / 1000.0));
+}

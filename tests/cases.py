@@ -1,0 +1,131 @@
+"""Parity cases: the reference's own test GPFs plus seeded synthetic receptors
+that exercise every GPF feature of the map path.
+
+Inputs of every case live under tests/golden/ (the reference's test files in
+reftests/, the synthetic ones in cases/<name>/), so no case depends on a
+random-number stream at test time.  tests/golden/make_golden.py writes the
+synthetic inputs and records what the reference autogrid4 produced for each
+case in tests/golden/golden.json.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+REFTESTS = os.path.join(GOLDEN, "reftests")
+CASES_DIR = os.path.join(GOLDEN, "cases")
+
+# Reference test GPFs (Tests/*.gpf): name -> expected success.
+REF_CASES = {
+    "hsg1_sm": True,
+    "hsg1_sm_nbpcoeffs": True,
+    "hsg1_sm_nbpreps": True,
+    "hsg1_sm_nbpreps_illegal_atom": True,
+    "hsg1_no_receptor_types": True,
+    "hsg1_sm_no_parameter_file": True,
+    "x1hpv_CASE0": True,
+    "x1hpv_CASE2": True,
+    "x1hpv_CASE3": True,
+    "x1hpv_CASE4": True,
+    "x1hpv_CASE1": False,            # a "map" line missing
+    "x1hpv_CASE5": False,            # one "map" line too many
+    "hsg1_sm_nbpreps_toofew": False,  # nbp_r_eps with 5 fields
+    "hsg1_sm_minus_two_types": False,  # receptor_types does not match the PDBQT
+}
+
+# Synthetic cases: receptor (atoms, radius, seed) + GPF variations.
+SYN_CASES = {
+    # the small workload: 1,000 atoms, 41^3 points, A C HD N NA OA + e + d
+    "small": dict(atoms=1000, radius=28.0, seed=1002, npts=40, lig=["A", "C", "HD", "N", "NA", "OA"]),
+    "disorder": dict(atoms=600, radius=20.0, seed=2001, npts=24, lig=["A", "C", "HD", "N", "NA", "OA"],
+                     extra=["disorder_h"]),
+    "sepdesolv": dict(atoms=600, radius=20.0, seed=2002, npts=20, lig=["C", "HD", "OA", "NA"],
+                      pre=["separate_desolvation_maps"]),
+    "covalent": dict(atoms=500, radius=18.0, seed=2003, npts=20, lig=["C", "Z", "OA", "HD"],
+                     extra=["covalentmap 1.5 800.0 {cx} {cy} {cz}"]),
+    "constdiel": dict(atoms=500, radius=18.0, seed=2004, npts=20, lig=["C", "OA", "HD"], diel="dielectric 4.0"),
+    "nodiel": dict(atoms=400, radius=16.0, seed=2005, npts=16, lig=["C", "OA", "HD"], diel=None),
+    "nbp_coeffs": dict(atoms=400, radius=16.0, seed=2006, npts=16, lig=["C", "OA", "HD", "N"],
+                       pre=["coeff_vdw 0.2", "qasp 0.02"],
+                       extra=["nbp_r_eps 3.9 0.12 12 6 C OA", "nbp_coeffs 1297.46337890625 227.8125 12 6 N C"]),
+    "ongrid": dict(atoms=300, radius=14.0, seed=2007, npts=18, lig=["C", "OA", "HD"], center_atom=0),
+    "all13": dict(atoms=800, radius=22.0, seed=2008, npts=20,
+                  lig=["A", "C", "N", "NA", "OA", "S", "SA", "HD", "F", "Cl", "Br", "I", "P"]),
+    "many_maps": dict(atoms=400, radius=16.0, seed=2009, npts=12,
+                      lig=["A", "C", "N", "NA", "OA", "S", "SA", "HD", "F", "Cl", "Br", "I", "P", "H", "NS",
+                           "OS", "HS", "Mg", "Zn", "Fe"]),
+}
+
+ALL_CASES = list(REF_CASES) + list(SYN_CASES)
+
+
+def case_dir(name: str) -> str:
+    return REFTESTS if name in REF_CASES else os.path.join(CASES_DIR, name)
+
+
+def gpf_name(name: str) -> str:
+    return f"{name}.gpf"
+
+
+def map_names(gpf_path: str) -> list:
+    """Output map file names of a GPF in slot order."""
+    names = []
+    for line in open(gpf_path):
+        w = line.split("#")[0].split()
+        if w and w[0].lower() in ("map", "elecmap", "dsolvmap", "desolvmap"):
+            names.append(w[1])
+    return names
+
+
+def materialize(name: str, dest: str) -> str:
+    """Copy a case's inputs into dest; returns the GPF file name."""
+    src = case_dir(name)
+    os.makedirs(dest, exist_ok=True)
+    for f in os.listdir(src):
+        if f.endswith((".gpf", ".pdbqt", ".dat")):
+            shutil.copy(os.path.join(src, f), dest)
+    return gpf_name(name)
+
+
+def write_synthetic_inputs(name: str, dest: str) -> None:
+    """Generate the inputs of a synthetic case (used by make_golden.py)."""
+    import numpy as np
+    from autogrid_amd import synth
+
+    c = SYN_CASES[name]
+    os.makedirs(dest, exist_ok=True)
+    coords, types, q = synth.make_receptor(c["atoms"], c["radius"], c["seed"], center=(4.25, -3.5, 7.75))
+    rec = f"{name}.pdbqt"
+    synth.write_pdbqt(os.path.join(dest, rec), coords, types, q)
+    written = synth.read_pdbqt_coords(os.path.join(dest, rec))
+    if "center_atom" in c:
+        center = written[c["center_atom"]]
+    else:
+        center = written.mean(axis=0)
+    present = []
+    for t in types:
+        if t not in present:
+            present.append(t)
+    lines = list(c.get("pre", []))
+    n = c["npts"]
+    lines += [f"npts {n} {n} {n}", f"gridfld {name}.maps.fld", "spacing 0.375",
+              "receptor_types " + " ".join(present), "ligand_types " + " ".join(c["lig"]),
+              f"receptor {rec}", "gridcenter %.3f %.3f %.3f" % tuple(center), "smooth 0.5"]
+    cx, cy, cz = (float("%.3f" % v) + d for v, d in zip(center, (0.9, -0.4, 0.3)))
+    lines += [e.format(cx=cx, cy=cy, cz=cz) for e in c.get("extra", [])]
+    maps = []
+    sep = any("separate_desolvation_maps" in p for p in c.get("pre", []))
+    for t in c["lig"]:
+        maps.append(f"{name}.{t}.map")
+        if sep and t != "Z":
+            maps.append(f"{name}.{t}.desolv.map")
+    lines += [f"map {m}" for m in maps]
+    lines += [f"elecmap {name}.e.map", f"dsolvmap {name}.d.map"]
+    diel = c.get("diel", "dielectric -0.1465")
+    if diel:
+        lines.append(diel)
+    with open(os.path.join(dest, gpf_name(name)), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    np.random.default_rng(0)  # no further randomness

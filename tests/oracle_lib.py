@@ -1,0 +1,112 @@
+"""ctypes access to the FP64 oracle (oracle/liboracle_ag.so) for the tests.
+
+The oracle is the checker: tests compare the engine's device output with it
+and pin it against the reference autogrid4 binary and the reference's own
+fixtures.  It is never part of the engine.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+import autogrid_amd as ag
+from autogrid_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle_ag.so")
+REF_BIN = os.path.join(ORACLE_DIR, "_ref", "autogrid4")
+
+_lib = None
+
+
+def olib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-C", ORACLE_DIR, "-s"], check=True)
+        L = C.CDLL(ORACLE_SO)
+        vp = C.c_void_p
+        L.orc_vdw_tables.argtypes = [C.POINTER(_abi.Setup), C.POINTER(_abi.MapSet), vp]
+        L.orc_sol_fn.argtypes = [C.c_double, vp]
+        L.orc_sol_fn.restype = None
+        L.orc_r_eps.argtypes = [vp]
+        L.orc_r_eps.restype = None
+        L.orc_ddd.argtypes = [C.c_double]
+        L.orc_ddd.restype = C.c_double
+        L.orc_bond_vectors.argtypes = [C.POINTER(_abi.Setup), C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_maps.argtypes = [C.POINTER(_abi.Receptor), C.POINTER(_abi.Grid), C.POINTER(_abi.MapSet),
+                               C.POINTER(_abi.Tables), vp, vp, vp, C.c_int, C.c_int]
+        L.orc_format_value.argtypes = [C.c_double, C.c_char_p]
+        _lib = L
+    return _lib
+
+
+def tables(st: ag.PreparedState):
+    """The oracle's (e_vdw_hb, sol_fn, r_eps) for a parsed state."""
+    rec, grid, ms, tab, setup = st.structs()
+    T, M = st.e_vdw_hb.shape[0], st.n_atom_maps
+    ev = np.zeros((T, M, 1024), np.float32)
+    rc = olib().orc_vdw_tables(C.byref(setup), C.byref(ms), ev.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle table build stopped")
+    sol = np.zeros(8192)
+    olib().orc_sol_fn(st.coeff_desolv, sol.ctypes.data)
+    reps = np.zeros(8192)
+    if st.dddiel:
+        olib().orc_r_eps(reps.ctypes.data)
+    return ev, sol, reps
+
+
+def bond_vectors(st: ag.PreparedState):
+    rec, grid, ms, tab, setup = st.structs()
+    n = st.n_atoms
+    rexp = np.zeros(n, np.int32)
+    rv = np.zeros((n, 3))
+    rv2 = np.zeros((n, 3))
+    dis = np.zeros(n, np.uint8)
+    coord = np.ascontiguousarray(st.coord)
+    typ = np.ascontiguousarray(st.type, np.int32)
+    hb = np.ascontiguousarray(st.hbond, np.int32)
+    rc = olib().orc_bond_vectors(C.byref(setup), n, coord.ctypes.data, typ.ctypes.data, hb.ctypes.data,
+                                 rexp.ctypes.data, rv.ctypes.data, rv2.ctypes.data, dis.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle bond-vector pass stopped")
+    return rexp, rv, rv2, dis
+
+
+def oracle_state(st: ag.PreparedState) -> ag.PreparedState:
+    """A copy of st whose tables and bond vectors come from the oracle."""
+    import dataclasses
+    ev, sol, reps = tables(st)
+    rexp, rv, rv2, dis = bond_vectors(st)
+    return dataclasses.replace(st, e_vdw_hb=ev, sol_fn=sol, r_eps=reps, rexp=rexp, rvector=rv, rvector2=rv2,
+                               disorder=dis, _keep=[])
+
+
+def maps(st: ag.PreparedState, z0: int = 0, z1: int | None = None, fmap=False, cmap=False):
+    """FP64 maps [n_maps, n1z, n1y, n1x] (planes z0..z1) of the state."""
+    rec, grid, ms, tab, setup = st.structs()
+    nz, ny, nx = st.n1[2], st.n1[1], st.n1[0]
+    z1 = nz if z1 is None else z1
+    out = np.zeros((st.n_maps, nz, ny, nx))
+    fm = np.zeros((nz, ny, nx)) if fmap else None
+    cm = np.zeros((nz, ny, nx), np.int32) if cmap else None
+    rc = olib().orc_maps(C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab), out.ctypes.data,
+                         fm.ctypes.data if fmap else None, cm.ctypes.data if cmap else None, z0, z1)
+    if rc != 0:
+        raise RuntimeError("oracle: no closestH atom was found")
+    return (out, fm, cm) if (fmap or cmap) else out
+
+
+def format_value(e: float) -> bytes:
+    buf = C.create_string_buffer(64)
+    n = olib().orc_format_value(float(e), buf)
+    return buf.raw[:n]
+
+
+def format_map(values) -> bytes:
+    return b"".join(format_value(float(v)) for v in np.asarray(values).reshape(-1))

@@ -1,0 +1,111 @@
+"""Device parity: the HIP map path against the FP64 oracle on every case.
+
+The oracle reproduces the reference autogrid4 byte for byte on these inputs
+(tests/test_oracle.py), so agreement with it is agreement with the
+reference.  Bar: |gpu - oracle| <= max(1e-3 kcal/mol, 1e-3 * |oracle|) at
+every point of every map.
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import autogrid_amd as ag
+import cases
+import oracle_lib as ol
+import parity
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(cases.GOLDEN, "golden.json")))
+SUCCESS = [n for n in cases.ALL_CASES if GOLDEN[n]["success"]]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ag.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name", SUCCESS)
+def test_maps_match_oracle(ctx, name, tmp_path):
+    print(f"[gpu] case {name}", flush=True)
+    st, _ = parity.load_case(name, str(tmp_path))
+    ref = ol.maps(ol.oracle_state(st))
+    got = ctx.maps_plain(st)
+    assert got.shape == ref.shape
+    report = []
+    for k in range(st.n_maps):
+        ok, mx, rms, bad = parity.within(got[k], ref[k])
+        report.append((k, ok, mx, rms, bad))
+    print(name, [(k, f"{mx:.2e}", f"{rms:.2e}", bad) for k, ok, mx, rms, bad in report], flush=True)
+    failed = [r for r in report if not r[1]]
+    assert not failed, f"{name}: maps over tolerance {failed}"
+
+
+def test_repeat_is_bitwise_identical(ctx, tmp_path):
+    st, _ = parity.load_case("small", str(tmp_path))
+    a = ctx.maps_plain(st)
+    b = ctx.maps_plain(st)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_context_reuse_across_sizes(ctx, tmp_path):
+    """Scratch grows and is reused: big, small, big again give the same maps."""
+    big, _ = parity.load_case("small", str(tmp_path / "a"))
+    small, _ = parity.load_case("hsg1_sm", str(tmp_path / "b"))
+    m1 = ctx.maps_plain(big)
+    m2 = ctx.maps_plain(small)
+    m3 = ctx.maps_plain(big)
+    assert np.array_equal(m1, m3)
+    ok, *_ = parity.within(m2, ol.maps(ol.oracle_state(small)))
+    assert ok
+
+
+def test_device_outputs(ctx, tmp_path):
+    """Outputs written straight into device memory equal the host path."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")   # the runtime the engine is bound to
+    st, _ = parity.load_case("hsg1_sm", str(tmp_path))
+    host = ctx.maps_plain(st)
+    ptr = C.c_void_p()
+    assert hip.hipMalloc(C.byref(ptr), C.c_size_t(host.nbytes)) == 0
+    try:
+        ctx.maps_plain_device(st, ptr.value)
+        back = np.empty_like(host)
+        assert hip.hipMemcpy(C.c_void_p(back.ctypes.data), ptr, C.c_size_t(host.nbytes), 2) == 0  # D2H
+    finally:
+        hip.hipFree(ptr)
+    assert np.array_equal(back, host)
+
+
+@pytest.mark.parametrize("name", ["hsg1_sm", "x1hpv_CASE0", "covalent", "sepdesolv"])
+def test_cli_dropin(name, tmp_path):
+    """autogrid4 drop-in: exit 0, completion line, reference headers, values."""
+    print(f"[gpu] cli {name}", flush=True)
+    gpf = cases.materialize(name, str(tmp_path))
+    p = subprocess.run([ag.CLI_PATH, "-p", gpf, "-l", "out.glg"], cwd=tmp_path, capture_output=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr.decode()
+    glg = open(tmp_path / "out.glg").read()
+    assert "Successful Completion" in "".join(glg.splitlines(True)[-10:])
+    st, _ = parity.load_case(name, str(tmp_path / "oracle"))
+    ref = ol.maps(ol.oracle_state(st))
+    g = GOLDEN[name]
+    for k, m in enumerate(cases.map_names(os.path.join(tmp_path, gpf))):
+        lines = open(tmp_path / m).read().split("\n")
+        assert lines[:6] == g["maps"][m]["header"]
+        vals = np.array([float(x) for x in lines[6:] if x])
+        assert vals.size == g["maps"][m]["n"]
+        # the text carries 3 decimals: allow the rounding of both sides
+        err = np.abs(vals - ref[k].reshape(-1))
+        lim = np.maximum(2e-3, 1e-3 * np.abs(ref[k].reshape(-1)) + 5e-4)
+        assert np.all(err <= lim), f"{m}: max {err.max()}"
+    if g.get("fld"):
+        fld = [ln for ln in open(tmp_path / f"{name}.maps.fld").read().split("\n") if not ln.startswith("# Created by")]
+        assert fld == g["fld"]

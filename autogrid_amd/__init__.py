@@ -24,7 +24,7 @@ import numpy as np
 from . import _abi
 from ._abi import AgError, check, lib
 
-__all__ = ["AgError", "Context", "Job", "PreparedState", "run_gpf", "format_map_values",
+__all__ = ["AgError", "Context", "Job", "Plan", "PreparedState", "run_gpf", "format_map_values",
            "LIB_PATH", "CLI_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
@@ -263,9 +263,49 @@ class Context:
         check(lib().ag_maps_plain(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
                                   C.c_void_p(out_ptr), None, None, 1))
 
+    def plan(self, st: PreparedState) -> "Plan":
+        """Upload the state to this context's device (ag_plan_create)."""
+        return Plan(self, st)
+
+    def timing(self) -> dict:
+        """HIP-event milliseconds of the last plan execution on this context."""
+        ms = (C.c_float * 4)()
+        check(lib().ag_context_timing(self._h, ms))
+        return {"cell_ms": ms[0], "near_ms": ms[1], "far_ms": ms[2], "total_ms": ms[3]}
+
     def run_gpf(self, gpf_path: str, glg_path: str | None = None, program: str = "autogrid4") -> None:
         check(lib().ag_run_gpf(self._h, os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
                                program.encode()))
+
+
+class Plan:
+    """A receptor, grid, map set and tables resident on one device."""
+
+    def __init__(self, ctx: Context, st: PreparedState):
+        rec, grid, ms, tab, _ = st.structs()
+        h = C.c_void_p()
+        check(lib().ag_plan_create(ctx.handle, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab), C.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        self.n_maps = st.n_maps
+        self.points = st.points
+
+    def execute(self, maps_dev_ptr: int) -> None:
+        """All maps into device memory at maps_dev_ptr ([n_maps][P] float32)."""
+        check(lib().ag_plan_execute(self._h, C.c_void_p(maps_dev_ptr)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ag_plan_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program: str = "autogrid4") -> None:

@@ -111,7 +111,13 @@ def lib() -> C.CDLL:
         L.ag_format_map_values.argtypes = [vp, C.c_int64, C.c_char_p, C.c_int64]
         L.ag_format_map_values.restype = C.c_int64
         L.ag_job_write_maps.argtypes = [vp, vp, vp, vp, C.c_int]
-        for name in ("ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",
+        L.ag_plan_create.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
+                                     C.POINTER(Tables), C.POINTER(vp)]
+        L.ag_plan_execute.argtypes = [vp, vp]
+        L.ag_plan_destroy.argtypes = [vp]
+        L.ag_context_timing.argtypes = [vp, C.POINTER(C.c_float)]
+        for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_context_timing",
+                     "ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",
                      "ag_job_views", "ag_run_gpf", "ag_maps_plain", "ag_job_write_maps"):
             getattr(L, name).restype = C.c_int
         _lib = L

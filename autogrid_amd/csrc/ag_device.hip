@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -732,7 +733,7 @@ __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// context and host orchestration
+// context, plans and host orchestration
 // ---------------------------------------------------------------------------
 
 struct DevBuf {
@@ -752,6 +753,9 @@ struct DevBuf {
         }
         return ptr;
     }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() {
         if (ptr) (void)hipFree(ptr);
     }
@@ -760,11 +764,26 @@ struct DevBuf {
 struct ag_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    DevBuf maps;                       // output staging of ag_maps_plain
+    hipEvent_t ev[4] = {};
+    float last_ms[4] = {0, 0, 0, 0};   // cell list, near, far, total of the last execute
+};
+
+// A receptor, grid, map set and tables resident on one device.
+struct ag_plan {
+    ag_ctx *ctx = nullptr;
+    int N = 0, M = 0, T = 0, n_maps = 0, groups = 0;
+    long long P = 0;
+    int elec_index = -1;
     DevBuf coord, charge, vol, solpar, rvec, rvec2, type, hbond, rexp, disorder;
     DevBuf atom_cell, cell_count, cell_start, cell_cursor, slot_atom, near_atoms, far_atoms;
-    DevBuf tab, sol, reps, fartab, maps, fmap, cmap, status;
-    hipEvent_t ev[8] = {};
-    float last_ms[4] = {0, 0, 0, 0};   // near, far, cell, total of the last call
+    DevBuf tab, sol, reps, fartab, status;
+    CellParams cp{};
+    int ncells = 0;
+    AtomSrc src{};
+    std::vector<NearParams> near;      // one per group of 16 map slots
+    FarParams far{};
+    bool run_far = false;
 };
 
 namespace agdev {
@@ -793,118 +812,141 @@ void context_destroy(ag_ctx *c) {
     delete c;
 }
 
+void context_timing(ag_ctx *c, float *ms) {
+    for (int i = 0; i < 4; i++) ms[i] = c->last_ms[i];
+}
+
 template <typename T>
-static T *upload(ag_ctx *c, DevBuf &b, const T *src, size_t n) {
+static T *upload(hipStream_t st, DevBuf &b, const T *src, size_t n) {
     T *d = (T *)b.get(sizeof(T) * n);
-    if (n) HIP_CHECK(hipMemcpyAsync(d, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+    if (n) HIP_CHECK(hipMemcpyAsync(d, src, sizeof(T) * n, hipMemcpyHostToDevice, st));
     return d;
 }
 
-void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
-                const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap, int outputs_on_device) {
-    HIP_CHECK(hipSetDevice(c->device));
-    const int N = rec->n_atoms;
-    const int M = ms->n_atom_maps;
-    const int T = tab->n_rec_types;
+static void validate(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab) {
+    const int N = rec->n_atoms, M = ms->n_atom_maps, T = tab->n_rec_types;
     if (N < 0 || N > AG_MAX_ATOMS) ag::fail(AG_ERR_BAD_INPUT, "atom count out of range");
     if (M < 0 || M + 2 > AG_MAX_MAPS) ag::fail(AG_ERR_BAD_INPUT, "map count out of range");
     if (T < 0 || T > AG_MAX_REC_TYPES) ag::fail(AG_ERR_BAD_INPUT, "receptor type count out of range");
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 3; d++)
         if (grid->n1[d] < 1 || grid->n1[d] > AG_MAX_GRID_PTS || grid->ne[d] < 0)
             ag::fail(AG_ERR_BAD_INPUT, "grid dimensions out of range");
-    }
     if (!(grid->spacing > 0)) ag::fail(AG_ERR_BAD_INPUT, "grid spacing must be positive");
-    if (fmap || cmap || ms->floating_grid || ms->constriction_grid)
+    if (ms->floating_grid || ms->constriction_grid)
         ag::fail(AG_ERR_UNSUPPORTED, "floating and constriction grids are not implemented on the device yet");
     const int n_maps = M + (ms->elec_index >= 0) + (ms->dsolv_index >= 0);
     if (ms->n_maps != n_maps) ag::fail(AG_ERR_BAD_INPUT, "n_maps does not match the map set");
+    if ((ms->elec_index >= 0 && ms->elec_index >= n_maps) || (ms->dsolv_index >= 0 && ms->dsolv_index >= n_maps))
+        ag::fail(AG_ERR_BAD_INPUT, "e/d map slot out of range");
     for (int i = 0; i < N; i++) {
         if (rec->type[i] < 0 || rec->type[i] >= T) ag::fail(AG_ERR_BAD_INPUT, "atom type index out of range");
         if (rec->hbond[i] < 0 || rec->hbond[i] > 6) ag::fail(AG_ERR_BAD_INPUT, "atom hbond class out of range");
     }
-    bool need_h = false;
-    bool any_aff = false;
-    for (int m = 0; m < M; m++) any_aff = any_aff || !ms->maps[m].is_covalent;
     if (M > 0) {
-        for (int i = 0; i < N && !need_h; i++) need_h = rec->hbond[i] == AG_HB_DS || rec->hbond[i] == AG_HB_D1;
-        if (!need_h) ag::fail(AG_ERR_NO_DONOR_H, "no closestH atom was found");
+        bool any_h = false;
+        for (int i = 0; i < N && !any_h; i++) any_h = rec->hbond[i] == AG_HB_DS || rec->hbond[i] == AG_HB_D1;
+        if (!any_h) ag::fail(AG_ERR_NO_DONOR_H, "no closestH atom was found");
     }
-    if (ms->disorder_h && (ms->hydrogen_type < -1 || ms->hydrogen_type >= T))
-        ag::fail(AG_ERR_BAD_INPUT, "hydrogen type index out of range");
+    if (ms->hydrogen_type < -1 || ms->hydrogen_type >= T) ag::fail(AG_ERR_BAD_INPUT, "hydrogen type index out of range");
+    if (M > 0 && !ms->maps) ag::fail(AG_ERR_BAD_INPUT, "map descriptors missing");
+}
 
-    const long long P = (long long)grid->n1[0] * grid->n1[1] * grid->n1[2];
+// Role of map m: the second map of each separate-desolvation pair carries
+// that type's desolvation (main.cpp:3385).
+static int map_role(const ag_mapset *ms, int m) {
+    if (ms->maps[m].is_covalent) return ROLE_COV;
+    if (!ms->separate_desolvation) return ROLE_AFF | ROLE_DES;
+    int k = 0;
+    while (k < m) k += ms->maps[k].is_covalent ? 1 : 2;
+    return k == m ? ROLE_AFF : ROLE_DES;
+}
+
+// H-bond case of (probe class mh, atom class ah) (main.cpp:3286-3374).
+static int hb_case(int mh, int ah) {
+    const bool atom_don = (ah == 1 || ah == 2 || ah == 6);
+    if ((mh == 3 || mh == 5 || mh == 6) && atom_don) return HBC_ACC_PROBE;
+    if ((mh == 4 || mh == 6) && atom_don) return HBC_MINMAX_RACC;
+    if ((mh == 1 || mh == 2 || mh == 6) && (ah > 2 || ah == 6)) return HBC_DON_PROBE;
+    return HBC_PLAIN;
+}
+
+ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                     const ag_tables *tab) {
+    validate(rec, grid, ms, tab);
+    HIP_CHECK(hipSetDevice(c->device));
+    std::unique_ptr<ag_plan> pl(new ag_plan());
+    ag_plan &p = *pl;
+    p.ctx = c;
+    const int N = p.N = rec->n_atoms;
+    const int M = p.M = ms->n_atom_maps;
+    const int T = p.T = tab->n_rec_types;
+    p.n_maps = ms->n_maps;
+    p.elec_index = ms->elec_index;
+    const long long P = p.P = (long long)grid->n1[0] * grid->n1[1] * grid->n1[2];
     hipStream_t st = c->stream;
-    HIP_CHECK(hipEventRecord(c->ev[0], st));
 
-    // ---- inputs ----
-    double *d_coord = upload(c, c->coord, rec->coord, 3 * (size_t)N);
-    double *d_charge = upload(c, c->charge, rec->charge, N);
-    double *d_vol = upload(c, c->vol, rec->vol, N);
-    double *d_solpar = upload(c, c->solpar, rec->solpar, N);
-    double *d_rvec = upload(c, c->rvec, rec->rvector, 3 * (size_t)N);
-    double *d_rvec2 = upload(c, c->rvec2, rec->rvector2, 3 * (size_t)N);
-    int *d_type = upload(c, c->type, rec->type, N);
-    int *d_hbond = upload(c, c->hbond, rec->hbond, N);
-    int *d_rexp = upload(c, c->rexp, rec->rexp, N);
-    unsigned char *d_dis = upload(c, c->disorder, rec->disorder, N);
+    double *d_coord = upload(st, p.coord, rec->coord, 3 * (size_t)N);
+    double *d_charge = upload(st, p.charge, rec->charge, N);
+    double *d_vol = upload(st, p.vol, rec->vol, N);
+    double *d_solpar = upload(st, p.solpar, rec->solpar, N);
+    double *d_rvec = upload(st, p.rvec, rec->rvector, 3 * (size_t)N);
+    double *d_rvec2 = upload(st, p.rvec2, rec->rvector2, 3 * (size_t)N);
+    int *d_type = upload(st, p.type, rec->type, N);
+    int *d_hbond = upload(st, p.hbond, rec->hbond, N);
+    int *d_rexp = upload(st, p.rexp, rec->rexp, N);
+    unsigned char *d_dis = upload(st, p.disorder, rec->disorder, N);
+    p.src = AtomSrc{d_coord, d_charge, d_vol, d_solpar, d_rvec, d_rvec2, d_type, d_hbond, d_rexp, d_dis,
+                    ms->coeff_estat, ms->solpar_q};
 
-    // ---- cell list over the box padded by the staging radius ----
+    // cell list geometry: the box padded by the staging radius
     const double cs = 4.0;
-    CellParams cp;
-    cp.cs = cs;
-    cp.n_atoms = N;
+    p.cp.cs = cs;
+    p.cp.n_atoms = N;
     for (int d = 0; d < 3; d++) {
         const double lo = -grid->ne[d] * grid->spacing - 8.1;
         const double hi = (grid->n1[d] - 1 - grid->ne[d]) * grid->spacing + 8.1;
-        cp.origin[d] = lo;
-        cp.nc[d] = (int)ceil((hi - lo) / cs) + 1;
+        p.cp.origin[d] = lo;
+        p.cp.nc[d] = (int)ceil((hi - lo) / cs) + 1;
     }
-    const int ncells = cp.nc[0] * cp.nc[1] * cp.nc[2];
-    int *d_atom_cell = (int *)c->atom_cell.get(sizeof(int) * (size_t)N);
-    int *d_count = (int *)c->cell_count.get(sizeof(int) * (size_t)ncells);
-    int *d_start = (int *)c->cell_start.get(sizeof(int) * ((size_t)ncells + 1));
-    int *d_cursor = (int *)c->cell_cursor.get(sizeof(int) * (size_t)ncells);
-    int *d_slot_atom = (int *)c->slot_atom.get(sizeof(int) * (size_t)N);
-    HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(int) * (size_t)ncells, st));
-    HIP_CHECK(hipMemsetAsync(d_cursor, 0, sizeof(int) * (size_t)ncells, st));
-    const int tpb = 256;
-    if (N > 0) k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(cp, d_coord, d_atom_cell, d_count);
-    k_scan<<<1, 1024, 0, st>>>(d_count, d_start, ncells);
-    if (N > 0) k_cell_scatter<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(N, d_atom_cell, d_start, d_cursor, d_slot_atom);
-    k_cell_sort<<<(ncells + tpb - 1) / tpb, tpb, 0, st>>>(ncells, d_start, d_slot_atom);
-    NearAtom *d_near = (NearAtom *)c->near_atoms.get(sizeof(NearAtom) * (size_t)N);
-    AtomSrc src{d_coord, d_charge, d_vol, d_solpar, d_rvec, d_rvec2, d_type, d_hbond, d_rexp, d_dis,
-                ms->coeff_estat, ms->solpar_q};
-    // records of every binned slot, in (cell, atom index) order
-    if (N > 0) k_gather<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(d_start + ncells, d_slot_atom, src, d_near);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(c->ev[1], st));
+    p.ncells = p.cp.nc[0] * p.cp.nc[1] * p.cp.nc[2];
+    p.atom_cell.get(sizeof(int) * (size_t)N);
+    p.cell_count.get(sizeof(int) * (size_t)p.ncells);
+    p.cell_start.get(sizeof(int) * ((size_t)p.ncells + 1));
+    p.cell_cursor.get(sizeof(int) * (size_t)p.ncells);
+    p.slot_atom.get(sizeof(int) * (size_t)N);
+    p.near_atoms.get(sizeof(NearAtom) * (size_t)N);
+    p.status.get(sizeof(int));
 
-    // ---- tables ----
-    // electrostatic factor table shared by both kernels
+    // electrostatic factor per bin: 332/eps for the distance-dependent
+    // dielectric, 332/diel for a constant one (main.cpp:2960-2978)
     std::vector<float> reps(AG_NDIEL);
-    for (int k = 0; k < AG_NDIEL; k++)
-        reps[k] = ms->dddiel ? (float)tab->r_eps[k] : (float)ms->inv_diel_cal;
-    float *d_reps = upload(c, c->reps, reps.data(), AG_NDIEL);
+    for (int k = 0; k < AG_NDIEL; k++) reps[k] = ms->dddiel ? (float)tab->r_eps[k] : (float)ms->inv_diel_cal;
+    float *d_reps = upload(st, p.reps, reps.data(), AG_NDIEL);
     std::vector<float> sol(AG_NEINT + 1);
     for (int k = 0; k <= AG_NEINT; k++) sol[k] = (float)tab->sol_fn[k];
-    float *d_sol = upload(c, c->sol, sol.data(), AG_NEINT + 1);
+    float *d_sol = upload(st, p.sol, sol.data(), AG_NEINT + 1);
 
-    // outputs
-    float *d_maps = outputs_on_device ? maps_out : (float *)c->maps.get(sizeof(float) * (size_t)n_maps * P);
-    int *d_status = (int *)c->status.get(sizeof(int));
-    HIP_CHECK(hipMemsetAsync(d_status, 0, sizeof(int), st));
+    // vdW/H-bond tables regrouped as [group][type][bin][16 map slots]
+    p.groups = M > 0 ? (M + kMapSlots - 1) / kMapSlots : 1;
+    const size_t gstride = (size_t)(T > 0 ? T : 1) * AG_NEINT * kMapSlots;
+    std::vector<float> gtab(gstride * p.groups, 0.f);
+    for (int g = 0; g < p.groups; g++)
+        for (int t = 0; t < T; t++)
+            for (int j = 0; j < kMapSlots && g * kMapSlots + j < M; j++) {
+                const float *srcrow = tab->e_vdw_hb + ((size_t)t * M + (g * kMapSlots + j)) * AG_NEINT;
+                float *dst = gtab.data() + g * gstride + (size_t)t * AG_NEINT * kMapSlots + j;
+                for (int k = 0; k < AG_NEINT; k++) dst[(size_t)k * kMapSlots] = srcrow[k];
+            }
+    float *d_tab = upload(st, p.tab, gtab.data(), gtab.size());
 
-    // ---- near kernel, one pass per group of 16 map slots ----
-    const int groups = M > 0 ? (M + kMapSlots - 1) / kMapSlots : 1;
-    std::vector<float> gtab((size_t)(T > 0 ? T : 1) * AG_NEINT * kMapSlots);
+    // near-kernel parameters, one set per group of 16 map slots
     NearParams np;
     memset(&np, 0, sizeof np);
     for (int d = 0; d < 3; d++) {
         np.n1[d] = grid->n1[d];
         np.ne[d] = grid->ne[d];
-        np.cell_origin[d] = cp.origin[d];
-        np.nc[d] = cp.nc[d];
+        np.cell_origin[d] = p.cp.origin[d];
+        np.nc[d] = p.cp.nc[d];
     }
     np.spacing = grid->spacing;
     np.s128 = (float)(grid->spacing * 128.0);
@@ -913,8 +955,8 @@ void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag
     np.nb[2] = (grid->n1[2] + kBZ - 1) / kBZ;
     np.nbricks = np.nb[0] * np.nb[1] * np.nb[2];
     np.cell_size = cs;
-    np.cell_start = d_start;
-    np.atoms = d_near;
+    np.cell_start = (const int *)p.cell_start.ptr;
+    np.atoms = (const NearAtom *)p.near_atoms.ptr;
     np.coord64 = d_coord;
     np.rvec64 = d_rvec;
     np.rvec2_64 = d_rvec2;
@@ -925,89 +967,45 @@ void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag
     for (int d = 0; d < 3; d++) np.cov_pos[d] = ms->cov_pos[d];
     np.cov_hw = ms->cov_half_width;
     np.cov_barrier = ms->cov_barrier;
-    np.maps = d_maps;
     np.P = P;
-    np.status = d_status;
-    for (int g = 0; g < groups; g++) {
+    np.status = (int *)p.status.ptr;
+    for (int g = 0; g < p.groups; g++) {
         const int m0 = g * kMapSlots;
         const int ns = M > 0 ? std::min(kMapSlots, M - m0) : 0;
-        np.nslots = ns;
+        NearParams q = np;
+        q.nslots = ns;
         bool has_aff = false;
-        for (int j = 0; j < kMapSlots; j++) {
-            np.role[j] = ROLE_NONE;
-            np.hbc[j] = 0;
-            np.sp[j] = 0.f;
-            np.volp[j] = 0.f;
-            np.out_slot[j] = 0;
-        }
         for (int j = 0; j < ns; j++) {
             const int m = m0 + j;
             const ag_map_desc &md = ms->maps[m];
-            np.out_slot[j] = m;
-            if (md.is_covalent) {
-                np.role[j] = ROLE_COV;
-                continue;
-            }
-            // separate desolvation: the map after each non-covalent map holds
-            // its desolvation (main.cpp:3385)
-            bool second = false;
-            if (ms->separate_desolvation) {
-                // walk from the first map to know whether m is a second of a pair
-                int k = 0;
-                while (k < m) {
-                    if (ms->maps[k].is_covalent) k += 1;
-                    else k += 2;
-                }
-                second = (k != m);
-            }
-            if (ms->separate_desolvation) np.role[j] = second ? ROLE_DES : ROLE_AFF;
-            else np.role[j] = ROLE_AFF | ROLE_DES;
+            q.out_slot[j] = m;
+            q.role[j] = map_role(ms, m);
+            if (q.role[j] == ROLE_COV) continue;
             has_aff = true;
-            np.sp[j] = (float)md.solpar_probe;
-            np.volp[j] = (float)md.vol_probe;
+            q.sp[j] = (float)md.solpar_probe;
+            q.volp[j] = (float)md.vol_probe;
             int hc = 0;
-            if (md.is_hbonder) {
-                const int mh = md.hbond;
-                for (int ah = 0; ah <= 6; ah++) {
-                    int cse = HBC_PLAIN;
-                    const bool atom_don = (ah == 1 || ah == 2 || ah == 6);
-                    if ((mh == 3 || mh == 5 || mh == 6) && atom_don) cse = HBC_ACC_PROBE;
-                    else if ((mh == 4 || mh == 6) && atom_don) cse = HBC_MINMAX_RACC;
-                    else if ((mh == 1 || mh == 2 || mh == 6) && (ah > 2 || ah == 6)) cse = HBC_DON_PROBE;
-                    hc |= cse << (4 * ah);
-                }
-            }
-            np.hbc[j] = hc;
+            if (md.is_hbonder)
+                for (int ah = 0; ah <= 6; ah++) hc |= hb_case(md.hbond, ah) << (4 * ah);
+            q.hbc[j] = hc;
         }
-        // table for this group's maps: [T][1024][16]
-        std::fill(gtab.begin(), gtab.end(), 0.f);
-        for (int t = 0; t < T; t++)
-            for (int j = 0; j < ns; j++)
-                for (int k = 0; k < AG_NEINT; k++)
-                    gtab[((size_t)t * AG_NEINT + k) * kMapSlots + j] =
-                        tab->e_vdw_hb[((size_t)t * M + (m0 + j)) * AG_NEINT + k];
-        float *d_tab = (float *)c->tab.get(sizeof(float) * gtab.size());
-        HIP_CHECK(hipMemcpyAsync(d_tab, gtab.data(), sizeof(float) * gtab.size(), hipMemcpyHostToDevice, st));
-        np.tab = d_tab;
-        np.e_slot = (g == 0) ? ms->elec_index : -1;
-        np.d_slot = (g == 0) ? ms->dsolv_index : -1;
-        np.has_affinity = has_aff || np.d_slot >= 0;
-        if (np.nslots > 0 || np.e_slot >= 0 || np.d_slot >= 0)
-            k_near<<<np.nbricks, kNearThreads, 0, st>>>(np);
-        HIP_CHECK(hipGetLastError());
-        // the group table buffer is rewritten by the next group: wait here
-        if (g + 1 < groups) HIP_CHECK(hipStreamSynchronize(st));
+        q.tab = d_tab + g * gstride;
+        q.e_slot = (g == 0) ? ms->elec_index : -1;
+        q.d_slot = (g == 0) ? ms->dsolv_index : -1;
+        q.has_affinity = has_aff || q.d_slot >= 0;
+        p.near.push_back(q);
     }
-    HIP_CHECK(hipEventRecord(c->ev[2], st));
 
-    // ---- far kernel (e map) ----
-    if (ms->elec_index >= 0 && N > 0) {
-        float4 *d_far = (float4 *)c->far_atoms.get(sizeof(float4) * (size_t)N);
-        k_far_atoms<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(N, d_coord, d_charge, ms->coeff_estat, d_far);
+    // far kernel (e map): per-atom float4 and the table with bins < 1025 zeroed
+    p.run_far = ms->elec_index >= 0 && N > 0;
+    if (p.run_far) {
+        float4 *d_far = (float4 *)p.far_atoms.get(sizeof(float4) * (size_t)N);
+        k_far_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_charge, ms->coeff_estat, d_far);
+        HIP_CHECK(hipGetLastError());
         std::vector<float> ft(AG_NDIEL);
         for (int k = 0; k < AG_NDIEL; k++) ft[k] = k < kFarBinMin ? 0.f : reps[k];
-        float *d_ft = upload(c, c->fartab, ft.data(), AG_NDIEL);
-        FarParams fp;
+        float *d_ft = upload(st, p.fartab, ft.data(), AG_NDIEL);
+        FarParams &fp = p.far;
         memset(&fp, 0, sizeof fp);
         for (int d = 0; d < 3; d++) {
             fp.n1[d] = grid->n1[d];
@@ -1021,25 +1019,79 @@ void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag
         fp.tab = d_ft;
         fp.tab_len = AG_NDIEL;
         fp.tmax = (float)(AG_NDIEL - 1);
-        fp.emap = d_maps + (size_t)ms->elec_index * P;
+    }
+    HIP_CHECK(hipStreamSynchronize(st));   // host staging vectors go out of scope
+    return pl.release();
+}
+
+void plan_destroy(ag_plan *p) {
+    if (!p) return;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    delete p;
+}
+
+// Cell list + near kernel (+ far kernel) into d_maps ([n_maps][P], device).
+// Asynchronous on the context's stream up to the final status check.
+void plan_execute(ag_plan *p, float *d_maps) {
+    ag_ctx *c = p->ctx;
+    HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int N = p->N;
+    const int tpb = 256;
+    int *d_atom_cell = (int *)p->atom_cell.ptr, *d_count = (int *)p->cell_count.ptr;
+    int *d_start = (int *)p->cell_start.ptr, *d_cursor = (int *)p->cell_cursor.ptr;
+    int *d_slot_atom = (int *)p->slot_atom.ptr;
+    int *d_status = (int *)p->status.ptr;
+    HIP_CHECK(hipEventRecord(c->ev[0], st));
+    HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(int) * (size_t)p->ncells, st));
+    HIP_CHECK(hipMemsetAsync(d_cursor, 0, sizeof(int) * (size_t)p->ncells, st));
+    HIP_CHECK(hipMemsetAsync(d_status, 0, sizeof(int), st));
+    if (N > 0) k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(p->cp, (const double *)p->coord.ptr, d_atom_cell, d_count);
+    k_scan<<<1, 1024, 0, st>>>(d_count, d_start, p->ncells);
+    if (N > 0) k_cell_scatter<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(N, d_atom_cell, d_start, d_cursor, d_slot_atom);
+    k_cell_sort<<<(p->ncells + tpb - 1) / tpb, tpb, 0, st>>>(p->ncells, d_start, d_slot_atom);
+    if (N > 0)
+        k_gather<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(d_start + p->ncells, d_slot_atom, p->src,
+                                                      (NearAtom *)p->near_atoms.ptr);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(c->ev[1], st));
+    for (auto &q : p->near) {
+        NearParams np = q;
+        np.maps = d_maps;
+        if (np.nslots > 0 || np.e_slot >= 0 || np.d_slot >= 0) k_near<<<np.nbricks, kNearThreads, 0, st>>>(np);
+        HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipEventRecord(c->ev[2], st));
+    if (p->run_far) {
+        FarParams fp = p->far;
+        fp.emap = d_maps + (size_t)p->elec_index * p->P;
         const int blocks = (fp.nruns + kFarThreads - 1) / kFarThreads;
-        k_far<true><<<blocks, kFarThreads, sizeof(float) * AG_NDIEL, st>>>(fp);
+        k_far<true><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
         HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipEventRecord(c->ev[3], st));
-
     int status = 0;
     HIP_CHECK(hipMemcpyAsync(&status, d_status, sizeof(int), hipMemcpyDeviceToHost, st));
-    if (!outputs_on_device)
-        HIP_CHECK(hipMemcpyAsync(maps_out, d_maps, sizeof(float) * (size_t)n_maps * P, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    HIP_CHECK(hipEventElapsedTime(&c->last_ms[2], c->ev[0], c->ev[1]));
-    HIP_CHECK(hipEventElapsedTime(&c->last_ms[0], c->ev[1], c->ev[2]));
-    HIP_CHECK(hipEventElapsedTime(&c->last_ms[1], c->ev[2], c->ev[3]));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms[0], c->ev[0], c->ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms[1], c->ev[1], c->ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms[2], c->ev[2], c->ev[3]));
     HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
-    if (status & 1)
-        ag::fail(AG_ERR_UNSUPPORTED,
-                 "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
+    if (status & 1) ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
+}
+
+void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,
+                float *maps_out, float *fmap, int32_t *cmap, int outputs_on_device) {
+    if (fmap || cmap) ag::fail(AG_ERR_UNSUPPORTED, "floating and constriction grids are not implemented on the device yet");
+    std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab), plan_destroy);
+    const size_t bytes = sizeof(float) * (size_t)p->n_maps * p->P;
+    float *d_maps = outputs_on_device ? maps_out : (float *)c->maps.get(bytes);
+    plan_execute(p.get(), d_maps);
+    if (!outputs_on_device) {
+        HIP_CHECK(hipMemcpyAsync(maps_out, d_maps, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
 }
 
 }  // namespace agdev

@@ -159,9 +159,15 @@ void write_all_maps(Job &job, const float *maps, const float *fmap,
 
 // Device side (ag_device.hip).
 struct ag_ctx;
+struct ag_plan;
 namespace agdev {
 ag_status context_create(ag_ctx **out, int device);
 void context_destroy(ag_ctx *ctx);
+void context_timing(ag_ctx *ctx, float *ms);
+ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                     const ag_tables *tab);
+void plan_destroy(ag_plan *p);
+void plan_execute(ag_plan *p, float *d_maps);
 void maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                 const ag_mapset *ms, const ag_tables *tab, float *maps,
                 float *fmap, int32_t *cmap, int outputs_on_device);

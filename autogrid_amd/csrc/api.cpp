@@ -202,6 +202,36 @@ ag_status ag_maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid
     return guarded([&] { agdev::maps_plain(ctx, rec, grid, ms, tab, maps, fmap, cmap, outputs_on_device); });
 }
 
+ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                         const ag_tables *tab, ag_plan **out) {
+    if (!ctx || !rec || !grid || !ms || !tab || !out) {
+        ag::set_last_error("null argument to ag_plan_create");
+        return AG_ERR_BAD_INPUT;
+    }
+    *out = nullptr;
+    return guarded([&] { *out = agdev::plan_create(ctx, rec, grid, ms, tab); });
+}
+
+ag_status ag_plan_execute(ag_plan *plan, float *maps_dev) {
+    if (!plan || !maps_dev) {
+        ag::set_last_error("null argument to ag_plan_execute");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::plan_execute(plan, maps_dev); });
+}
+
+ag_status ag_plan_destroy(ag_plan *plan) {
+    return guarded([&] { agdev::plan_destroy(plan); });
+}
+
+ag_status ag_context_timing(ag_ctx *ctx, float ms[4]) {
+    if (!ctx || !ms) {
+        ag::set_last_error("null argument to ag_context_timing");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::context_timing(ctx, ms); });
+}
+
 int64_t ag_format_map_values(const float *values, int64_t n, char *buf, int64_t cap) {
     int64_t used = 0;
     char tmp[64];

@@ -1,0 +1,162 @@
+"""Benchmark of the grid-map path on one MI355X.
+
+Workload (default "full"): a seeded 5,000-atom receptor in a 45 A sphere,
+npts 126 (127^3 = 2,048,383 points), maps A C N NA OA S SA HD F Cl Br I P
+plus e and d (15 maps), AD4.1_bound parameters, distance-dependent
+dielectric.  One step computes the whole map set on the device from
+device-resident inputs: cell list, near-field brick kernel, far-field
+electrostatics kernel.  The metric is the map-set time in seconds.
+
+Also reported on the same JSON line:
+  roofline      the far-field electrostatics kernel (the dominant one):
+                algorithmic FLOPs 16 per (point, atom) pair, achieved
+                TFLOP/s from its HIP-event time on the engine's stream,
+                against the MI355X FP32 vector peak.
+  cpu_baseline  the reference autogrid4 (built from its sources with
+                OpenMP) on a z-slab of the same workload on this host,
+                scaled to the whole map set by point count.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector, packed
+FP32_PEAK_NONPACKED_TFLOPS = 78.6
+FLOPS_PER_PAIR_E = 16
+
+
+def hip_runtime():
+    return C.CDLL("libamdhip64.so.7")
+
+
+def cpu_baseline(workload: str, seconds_target: float = 20.0) -> dict | None:
+    """Reference autogrid4 on a z-slab of the workload; None if not built."""
+    from autogrid_amd import synth
+    ref = os.path.join(ROOT, "oracle", "_ref", "autogrid4")
+    if not os.path.exists(ref):
+        return None
+    w = synth.WORKLOADS[workload]
+    cores = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16")))
+    z_npts = 32 if workload == "full" else (8 if workload == "large" else w.npts)
+    d = tempfile.mkdtemp(prefix="agbase_")
+    try:
+        info = synth.make_workload(workload, d, z_npts=z_npts)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        t0 = time.perf_counter()
+        p = subprocess.run([ref, "-p", info["gpf"], "-l", "ref.glg"], cwd=d, env=env, capture_output=True)
+        t = time.perf_counter() - t0
+        if p.returncode != 0:
+            return None
+        n1 = w.npts + 1
+        sample_points = n1 * n1 * (z_npts + 1)
+        full_points = n1 ** 3
+        return {"value": t * full_points / sample_points, "unit": "s", "cores": cores, "kind": "reference",
+                "sample": f"npts {w.npts} {w.npts} {z_npts} ({sample_points} points, {t:.2f} s incl. ASCII output), "
+                          f"scaled x{full_points / sample_points:.3f} by point count"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="full", choices=["small", "full", "large"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the map set shards by "
+              "z-slab across GPUs without communication; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import autogrid_amd as ag
+    from autogrid_amd import synth
+
+    work = tempfile.mkdtemp(prefix="agbench_")
+    try:
+        info = synth.make_workload(args.workload, work)
+        cwd = os.getcwd()
+        os.chdir(work)
+        try:
+            st = ag.Job(info["gpf"]).prepare().state()
+        finally:
+            os.chdir(cwd)
+        hip = hip_runtime()
+        ctx = ag.Context(0)
+        plan = ctx.plan(st)
+        nbytes = st.n_maps * st.points * 4
+        dptr = C.c_void_p()
+        if hip.hipMalloc(C.byref(dptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        for _ in range(args.warmup):
+            plan.execute(dptr.value)
+        hip.hipDeviceSynchronize()
+        far_ms, near_ms, cell_ms = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute(dptr.value)          # synchronous: ends with a stream sync
+            tm = ctx.timing()
+            far_ms.append(tm["far_ms"])
+            near_ms.append(tm["near_ms"])
+            cell_ms.append(tm["cell_ms"])
+        hip.hipDeviceSynchronize()
+        t = time.perf_counter() - t0
+        ms_per_step = 1000.0 * t / args.steps
+        far_avg = sum(far_ms) / len(far_ms)
+        P, N = st.points, st.n_atoms
+        flops = FLOPS_PER_PAIR_E * P * N
+        achieved = flops / (far_avg * 1e-3) / 1e12
+        hip.hipFree(dptr)
+        plan.close()
+        ctx.close()
+
+        # whole drop-in with ASCII output, for the wall time the reference reports
+        os.chdir(work)
+        try:
+            t1 = time.perf_counter()
+            p = subprocess.run([ag.CLI_PATH, "-p", info["gpf"], "-l", "gpu.glg"], capture_output=True)
+            wall_cli = time.perf_counter() - t1
+        finally:
+            os.chdir(cwd)
+        cli_ok = p.returncode == 0
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+    base = None if args.no_cpu_baseline else cpu_baseline(args.workload)
+    value = ms_per_step / 1000.0
+    out = {
+        "metric": "map-set wall s (5k atoms,126^3,14 maps); batched & fwd+adj vs plain GPU",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {N} atoms, {st.n1[0]}x{st.n1[1]}x{st.n1[2]} points, "
+                               f"{st.n_maps} maps (13 affinity + e + d), spacing 0.375",
+                   "atoms": N, "points": P, "maps": st.n_maps},
+        "roofline": {"bound": "valu", "kernel": "k_far (electrostatics, all atoms beyond the near field)",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "frac_nonpacked_peak": achieved / FP32_PEAK_NONPACKED_TFLOPS,
+                     "flops_per_launch": flops, "avg_launch_ms": far_avg, "traffic": None},
+        "kernels_ms": {"cell_list": sum(cell_ms) / len(cell_ms), "near": sum(near_ms) / len(near_ms),
+                       "far": far_avg},
+        "pair_evals_per_s": P * N / (far_avg * 1e-3),
+        "wall_with_ascii_output_s": wall_cli if cli_ok else None,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

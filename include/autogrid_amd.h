@@ -214,6 +214,22 @@ ag_status ag_maps_plain(ag_ctx *ctx, const ag_receptor *rec,
                         const ag_tables *tables, float *maps, float *fmap,
                         int32_t *cmap, int outputs_on_device);
 
+/* Device-resident form of the same computation: ag_plan_create uploads the
+ * receptor, tables and map set to ctx's device once; each ag_plan_execute
+ * then builds the cell list and runs the near-field and far-field kernels
+ * into maps_dev, a device buffer of [n_maps][P] floats.  A plan belongs to
+ * its context and must be destroyed before it. */
+typedef struct ag_plan ag_plan;
+ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
+                         const ag_mapset *mapset, const ag_tables *tables, ag_plan **out);
+ag_status ag_plan_execute(ag_plan *plan, float *maps_dev);
+ag_status ag_plan_destroy(ag_plan *plan);
+
+/* Milliseconds of the last execute on ctx, from HIP events recorded on ctx's
+ * stream: [0] cell list, [1] near-field kernel(s), [2] far-field kernel,
+ * [3] total. */
+ag_status ag_context_timing(ag_ctx *ctx, float ms[4]);
+
 /* Map output: the value text of main.cpp:3486-3497 -- "0" when |e| < 1e-4,
  * else "%.3f" of (float)(floor(e*1000+0.5)/1000) -- for n values into buf.
  * Returns the number of bytes written (each value ends in '\n'), or -1 if

@@ -162,7 +162,7 @@ class Job:
         self.prepared = False
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None):
             lib().ag_job_close(self._h)
             self._h = None
 

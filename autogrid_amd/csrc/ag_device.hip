@@ -49,8 +49,10 @@ constexpr int kBX = 8, kBY = 8, kBZ = 4;          // brick of the near kernel
 constexpr int kNearThreads = kBX * kBY * kBZ;     // 256
 constexpr int kMapSlots = 16;                     // maps per near-kernel pass
 constexpr int kCandCap = 2048;                    // candidates per brick
-constexpr int kHCap = 512;                        // donor-H candidates per brick
-constexpr int kChunk = 256;                       // candidate records staged per pass
+constexpr int kHCap = 256;                        // donor-H candidates per brick
+constexpr int kChunk = 128;                       // candidate records staged per pass
+constexpr int kMinMax = 4;                        // H-bond min/max maps per pass
+constexpr int kNearRepsLen = 1040;                // electrostatic bins used by the near kernel
 constexpr float kNearRadius = 8.03f;              // staging radius (A)
 constexpr int kFarK = 8;                          // points per lane, far kernel
 constexpr int kFarThreads = 256;
@@ -104,6 +106,7 @@ struct NearParams {
     float sp[kMapSlots];          // solpar_probe
     float volp[kMapSlots];        // vol_probe
     int out_slot[kMapSlots];      // output slot of each map
+    int mm[kMapSlots];            // min/max register of each map, -1 if none
     int e_slot, d_slot;           // -1 when not produced in this pass
     int has_affinity;             // any AFF/DES slot, or d map
     int disorder_h, hydrogen_type;
@@ -296,10 +299,22 @@ __device__ __forceinline__ double ref_dist(double ax, double ay, double az, doub
     return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
+// Uniform per-atom values: every lane reads the same LDS record, so its
+// fields are moved to scalar registers and branches on them stay scalar.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     __shared__ int cand[kCandCap];
     __shared__ int hcand[kHCap];
     __shared__ NearAtom rec[kChunk];
+    __shared__ double hpos[kHCap][3];
+    __shared__ int hidx[kHCap];
+    __shared__ float hrvs[kHCap][3];
+    __shared__ float reps_s[kNearRepsLen];
+    __shared__ float sol_s[AG_NEINT + 1];
     __shared__ int wave_tot[kNearThreads / 64];
     __shared__ int cell_off[kNearThreads + 1];
     __shared__ int cstart_s[kNearThreads];
@@ -314,6 +329,9 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     const int iy = by * kBY + ((tid / kBX) % kBY);
     const int iz = bz * kBZ + tid / (kBX * kBY);
     const bool active = ix < p.n1[0] && iy < p.n1[1] && iz < p.n1[2];
+
+    for (int k = tid; k < kNearRepsLen; k += kNearThreads) reps_s[k] = p.reps[k];
+    for (int k = tid; k <= AG_NEINT; k += kNearThreads) sol_s[k] = p.sol[k];
 
     // brick box in the grid frame
     const int ix0 = bx * kBX, iy0 = by * kBY, iz0 = bz * kBZ;
@@ -345,7 +363,6 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     __syncthreads();
     const float r2cut = kNearRadius * kNearRadius;
     for (int cbase = 0; cbase < ncell; cbase += kNearThreads) {
-        // atom counts of this batch of cells, exclusive scan in cell_off
         const int ci = cbase + tid;
         int cstart = 0, ccount = 0;
         if (ci < ncell) {
@@ -374,15 +391,13 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             bool ok = false, is_h = false;
             int slot = -1;
             if (j < total) {
-                // cell of item j: last c with cell_off[c] <= j
-                int lo = 0, hi = kNearThreads;  // cell_off[hi] > j
+                int lo = 0, hi = kNearThreads;  // last cell with cell_off[c] <= j
                 while (hi - lo > 1) {
                     int mid = (lo + hi) >> 1;
                     if (cell_off[mid] <= j) lo = mid; else hi = mid;
                 }
                 slot = cstart_s[lo] + (j - cell_off[lo]);
                 const NearAtom &a = p.atoms[slot];
-                // distance from the atom to the brick box
                 float dd = 0.f;
                 const double av[3] = {a.x, a.y, a.z};
                 for (int d = 0; d < 3; d++) {
@@ -398,9 +413,9 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             const int base = ncand_s;
             if (ok && base + pos < kCandCap) cand[base + pos] = slot;
             int htot;
-            const int hpos = block_compact(is_h, wave_tot, htot);
+            const int hpos_i = block_compact(is_h, wave_tot, htot);
             const int hbase = nh_s;
-            if (is_h && hbase + hpos < kHCap) hcand[hbase + hpos] = slot;
+            if (is_h && hbase + hpos_i < kHCap) hcand[hbase + hpos_i] = slot;
             __syncthreads();
             if (tid == 0) { ncand_s = base + tot; nh_s = hbase + htot; }
             __syncthreads();
@@ -411,48 +426,59 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
         if (tid == 0) atomicOr(p.status, 1);
         return;
     }
+    // donor-H candidates into LDS for the closest-H search
+    for (int h = tid; h < nh; h += kNearThreads) {
+        const NearAtom &a = p.atoms[hcand[h]];
+        hpos[h][0] = a.x; hpos[h][1] = a.y; hpos[h][2] = a.z;
+        hidx[h] = a.index;
+        hrvs[h][0] = a.rv[0]; hrvs[h][1] = a.rv[1]; hrvs[h][2] = a.rv[2];
+    }
+    __syncthreads();
 
     // ---- covalent maps (main.cpp:2866-2879) ----
-    float acc[kMapSlots], hmin[kMapSlots], hmax[kMapSlots];
+    float acc[kMapSlots];
 #pragma unroll
     for (int m = 0; m < kMapSlots; m++) {
         acc[m] = 0.f;
-        hmin[m] = 999999.f;
-        hmax[m] = -999999.f;
-    }
-#pragma unroll
-    for (int m = 0; m < kMapSlots; m++) {
-        if (m < p.nslots && (p.role[m] & ROLE_COV)) {
+        if (m < p.nslots && p.role[m] == ROLE_COV) {
             double dx = p.cov_pos[0] - cx, dy = p.cov_pos[1] - cy, dz = p.cov_pos[2] - cz;
             double rcov = sqrt(dx * dx + dy * dy + dz * dz) / p.cov_hw;
             if (rcov < 1.0e-6) rcov = 1.0e-6;
             acc[m] = (float)(p.cov_barrier * (1. - exp(-0.69314718055994530942 * rcov * rcov)));
         }
     }
+    // min/max of the H-bond pairs (main.cpp:3300-3374, 3420-3431), at most
+    // kMinMax slots per pass
+    float hmin[kMinMax], hmax[kMinMax];
+#pragma unroll
+    for (int k = 0; k < kMinMax; k++) {
+        hmin[k] = 999999.f;
+        hmax[k] = -999999.f;
+    }
 
     // ---- closest donor H (main.cpp:2985-3015), FP64, lowest index on ties ----
     double best_r = 999999.0;
-    int best_idx = 0x7fffffff, best_slot = -1;
+    int best_idx = 0x7fffffff, best_h = -1;
     for (int h = 0; h < nh; h++) {
-        const int s = hcand[h];
-        const NearAtom &a = p.atoms[s];
         double dx, dy, dz;
-        const double r = ref_dist(a.x, a.y, a.z, cx, cy, cz, dx, dy, dz);
-        if (r < best_r || (r == best_r && a.index < best_idx)) {
+        const double r = ref_dist(hpos[h][0], hpos[h][1], hpos[h][2], cx, cy, cz, dx, dy, dz);
+        const int ix_h = hidx[h];
+        if (r < best_r || (r == best_r && ix_h < best_idx)) {
             best_r = r;
-            best_idx = a.index;
-            best_slot = s;
+            best_idx = ix_h;
+            best_h = h;
         }
     }
     float hrv[3] = {0.f, 0.f, 0.f};
-    if (best_slot >= 0) {
-        const NearAtom &a = p.atoms[best_slot];
-        hrv[0] = a.rv[0]; hrv[1] = a.rv[1]; hrv[2] = a.rv[2];
+    if (best_h >= 0) {
+        hrv[0] = hrvs[best_h][0]; hrv[1] = hrvs[best_h][1]; hrv[2] = hrvs[best_h][2];
     }
 
     float e_near = 0.f, d_acc = 0.f;
     const float *tab = p.tab;
-    const float *reps = p.reps;
+    const bool want_e = p.e_slot >= 0;
+    // cos and sin of 1.24791 rad, the disordered-hydroxyl offset angle
+    const float cdis = 0.31936680f, sdis = 0.94763168f;
 
     for (int cbase = 0; cbase < ncand; cbase += kChunk) {
         const int nc = min(kChunk, ncand - cbase);
@@ -467,11 +493,10 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
             if (r2 > 64.33f) continue;  // beyond 8.02 A: neither cutoff pass nor near e term
             float r = sqrtf(r2);
-            float g = r * 128.f;
+            const float g = r * 128.f;
             int bin;
             bool cut;
-            const float frac = g - rintf(g);
-            if (fabsf(frac) < kGuardBins) {
+            if (fabsf(g - rintf(g)) < kGuardBins) {
                 double rdx, rdy, rdz;
                 const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
                 bin = (int)(r64 * 128.0);
@@ -482,7 +507,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 cut = r > 8.0f;
             }
             // electrostatics of pairs the far kernel leaves out
-            if (p.e_slot >= 0) {
+            if (want_e) {
                 bool near_e;
                 if (g < 1022.f) near_e = true;
                 else if (g > 1027.f) near_e = false;
@@ -490,168 +515,143 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 if (near_e) {
                     const float rr = r < 1.0e-6f ? 1.0e-6f : r;
                     const float inv_rmax = 1.f / (rr > 0.5f ? rr : 0.5f);
-                    e_near += a.qe * inv_rmax * reps[min(bin, AG_NDIEL - 1)];
+                    e_near += unif(a.qe) * inv_rmax * reps_s[min(bin, kNearRepsLen - 1)];
                 }
             }
             if (cut || !p.has_affinity) continue;
-            const int type = a.meta & 0xff, hb = (a.meta >> 8) & 0xff, rexp = (a.meta >> 16) & 0xff;
-            const bool dis = (a.meta >> 24) & 1;
+            const int meta = uni(a.meta);
+            const int type = meta & 0xff, hb = (meta >> 8) & 0xff, rexp = (meta >> 16) & 0xff;
+            const bool dis = (meta >> 24) & 1;
             if (p.disorder_h && dis && type == p.hydrogen_type) continue;
             const float rr = r < 1.0e-6f ? 1.0e-6f : r;
             const float inv_r = 1.f / rr;
-            const float dn[3] = {dxf * inv_r, dyf * inv_r, dzf * inv_r};
-            const int indx_n = min(bin, AG_NEINT - 1);
-            const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
+            const float dn0 = dxf * inv_r, dn1 = dyf * inv_r, dn2 = dzf * inv_r;
+            const float rv0 = unif(a.rv[0]), rv1 = unif(a.rv[1]), rv2 = unif(a.rv[2]);
+            const int aidx = uni(a.index);
             float racc = 1.f, rdon = 1.f, Hramp = 1.f;
-            if (hb == AG_HB_D1) {
-                float cos_t = -(dn[0] * a.rv[0]) - dn[1] * a.rv[1] - dn[2] * a.rv[2];
+            if (hb == AG_HB_D1 || (hb == AG_HB_A2 && !dis)) {
+                // cos(theta) of the D1 / ordered-A2 branches; FP64 near 0
+                float cos_t = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
                 bool pos;
                 if (fabsf(cos_t) < 1.0e-5f) {
                     double rdx, rdy, rdz;
                     double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
                     if (r64 < 1.0e-6) r64 = 1.0e-6;
                     const double ir = 1. / r64;
-                    const double *rv64 = p.rvec64 + 3 * (size_t)a.index;
+                    const double *rv64 = p.rvec64 + 3 * (size_t)aidx;
                     double c = 0.;
                     c -= rdx * ir * rv64[0];
                     c -= rdy * ir * rv64[1];
                     c -= rdz * ir * rv64[2];
-                    pos = c > 0.;
+                    pos = hb == AG_HB_D1 ? c > 0. : c >= 0.;
                     cos_t = (float)c;
                 } else {
-                    pos = cos_t > 0.f;
+                    pos = hb == AG_HB_D1 ? cos_t > 0.f : cos_t >= 0.f;
                 }
-                if (!pos) {
-                    racc = 0.f;
+                if (hb == AG_HB_D1) {
+                    if (!pos) {
+                        racc = 0.f;
+                    } else {
+                        const float c2 = cos_t * cos_t;
+                        racc = rexp == 2 ? c2 : (rexp == 4 ? c2 * c2 : cos_t);
+                        if (aidx != best_idx) {
+                            float ct = hrv[0] * rv0 + hrv[1] * rv1 + hrv[2] * rv2;
+                            ct = fminf(fmaxf(ct, -1.f), 1.f);
+                            Hramp = 0.5f - 0.5f * cosf(acosf(ct) * (120.f / 90.f));
+                        }
+                    }
                 } else {
-                    const float c2 = cos_t * cos_t;
-                    racc = rexp == 2 ? c2 : (rexp == 4 ? c2 * c2 : cos_t);
-                    if (a.index != best_idx) {
-                        float ct = hrv[0] * a.rv[0] + hrv[1] * a.rv[1] + hrv[2] * a.rv[2];
-                        ct = fminf(ct, 1.f);
-                        ct = fmaxf(ct, -1.f);
-                        Hramp = 0.5f - 0.5f * cosf(acosf(ct) * (120.f / 90.f));
+                    // Goodford lone-pair terms.  With x = dn.rv2 and
+                    // y = (dn x rv2).rv / |dn x rv2|:
+                    //   t0 = pi/2 - acos(x)         -> cos(t0) = sqrt(1 - x^2)
+                    //   ti = |acos(y) - pi/2|       -> sin(2 ti) = 2|y| sqrt(1 - y^2)
+                    const float rw0 = unif(a.rv2[0]), rw1 = unif(a.rv2[1]), rw2 = unif(a.rv2[2]);
+                    float t0 = dn0 * rw0 + dn1 * rw1 + dn2 * rw2;
+                    t0 = fminf(fmaxf(t0, -1.f), 1.f);
+                    const float cos_t0 = sqrtf(fmaxf(1.f - t0 * t0, 0.f));
+                    rdon = 0.f;
+                    if (pos) {
+                        const float cr0 = dn1 * rw2 - dn2 * rw1;
+                        const float cr1 = dn2 * rw0 - dn0 * rw2;
+                        const float cr2 = dn0 * rw1 - dn1 * rw0;
+                        float rd2 = cr0 * cr0 + cr1 * cr1 + cr2 * cr2;
+                        if (rd2 < 1.0e-6f) rd2 = 1.0e-6f;
+                        const float inv_rd = 1.f / sqrtf(rd2);
+                        float ti = (cr0 * inv_rd) * rv0 + (cr1 * inv_rd) * rv1 + (cr2 * inv_rd) * rv2;
+                        ti = fminf(fmaxf(ti, -1.f), 1.f);
+                        const float sin2ti = 2.f * fabsf(ti) * sqrtf(fmaxf(1.f - ti * ti, 0.f));
+                        rdon = (0.9f + 0.1f * sin2ti) * cos_t0;
+                    } else if (cos_t >= -0.34202f) {
+                        const float u = 0.116978f - cos_t * cos_t;
+                        rdon = 562.25f * (u * u * u) * cos_t0;
                     }
                 }
             } else if (hb == AG_HB_A1) {
-                const float cos_t = -(dn[0] * a.rv[0]) - dn[1] * a.rv[1] - dn[2] * a.rv[2];
+                const float cos_t = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
                 rdon = cos_t <= 0.f ? 0.f : cos_t * cos_t;
-            } else if (hb == AG_HB_A2 && !dis) {
-                float cos_t = -(dn[0] * a.rv[0]) - dn[1] * a.rv[1] - dn[2] * a.rv[2];
-                bool front;
-                if (fabsf(cos_t) < 1.0e-5f) {
-                    double rdx, rdy, rdz;
-                    double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
-                    if (r64 < 1.0e-6) r64 = 1.0e-6;
-                    const double ir = 1. / r64;
-                    const double *rv64 = p.rvec64 + 3 * (size_t)a.index;
-                    double c = 0.;
-                    c -= rdx * ir * rv64[0];
-                    c -= rdy * ir * rv64[1];
-                    c -= rdz * ir * rv64[2];
-                    front = c >= 0.;
-                    cos_t = (float)c;
-                } else {
-                    front = cos_t >= 0.f;
-                }
-                float t0 = dn[0] * a.rv2[0] + dn[1] * a.rv2[1] + dn[2] * a.rv2[2];
-                t0 = fminf(fmaxf(t0, -1.f), 1.f);
-                t0 = 1.57079632679489661923f - acosf(t0);
-                const float cr0 = dn[1] * a.rv2[2] - dn[2] * a.rv2[1];
-                const float cr1 = dn[2] * a.rv2[0] - dn[0] * a.rv2[2];
-                const float cr2 = dn[0] * a.rv2[1] - dn[1] * a.rv2[0];
-                float rd2 = cr0 * cr0 + cr1 * cr1 + cr2 * cr2;
-                if (rd2 < 1.0e-6f) rd2 = 1.0e-6f;
-                const float inv_rd = 1.f / sqrtf(rd2);
-                float ti = (cr0 * inv_rd) * a.rv[0] + (cr1 * inv_rd) * a.rv[1] + (cr2 * inv_rd) * a.rv[2];
-                rdon = 0.f;
-                if (front) {
-                    ti = fminf(fmaxf(ti, -1.f), 1.f);
-                    ti = acosf(ti) - 1.57079632679489661923f;
-                    ti = fabsf(ti);
-                    rdon = (0.9f + 0.1f * sinf(ti + ti)) * cosf(t0);
-                } else if (cos_t >= -0.34202f) {
-                    const float u = 0.116978f - cos_t * cos_t;
-                    rdon = 562.25f * (u * u * u) * cosf(t0);
-                }
             } else if (hb == AG_HB_A2 && dis) {
-                float cos_t = -(dn[0] * a.rv[0]) - dn[1] * a.rv[1] - dn[2] * a.rv[2];
-                cos_t = fminf(fmaxf(cos_t, -1.f), 1.f);
-                const float th = acosf(cos_t);
+                // cos(theta - 1.24791) = c cos(1.24791) + sqrt(1 - c^2) sin(1.24791);
+                // theta <= 1.24791 + pi/2  <=>  c >= -sin(1.24791)
+                float c = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
+                c = fminf(fmaxf(c, -1.f), 1.f);
                 racc = 0.f;
                 rdon = 0.f;
-                if (th <= 1.24791f + 1.57079632679489661923f) {
-                    const float cc = cosf(th - 1.24791f);
+                if (c >= -sdis) {
+                    const float cc = c * cdis + sqrtf(fmaxf(1.f - c * c, 0.f)) * sdis;
                     rdon = (cc * cc) * (cc * cc);
                     racc = rdon;
                 }
             }
-            // vdW / H-bond table rows for this pair: [type][bin][slot]
-            const float4 *row = reinterpret_cast<const float4 *>(tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots);
-            float E[kMapSlots];
-#pragma unroll
-            for (int q = 0; q < kMapSlots / 4; q++) {
-                const float4 v = row[q];
-                E[4 * q] = v.x; E[4 * q + 1] = v.y; E[4 * q + 2] = v.z; E[4 * q + 3] = v.w;
-            }
-            float Eh[kMapSlots];
-            bool need_h = false;
-            int indx_h = 0;
+            const int indx_n = min(bin, AG_NEINT - 1);
+            const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
+            const float S = sol_s[indx_r];
+            const float avol = unif(a.vol), asolq = unif(a.solq);
+            // disordered partner: hydrogen-type row at r - 1.10 (main.cpp:3313-3349)
+            int hrow_bin = 0;
             if (dis && p.hydrogen_type >= 0) {
-                // disordered donor/acceptor partner: table of the hydrogen type
-                // at r - 1.10 (main.cpp:3313-3349), FP64 like the reference
                 double rdx, rdy, rdz;
                 const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
-                indx_h = (int)((r64 - 1.10) * 128.0);
-                indx_h = min(max(0, indx_h), AG_NEINT - 1);
-                need_h = true;
+                hrow_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
             }
-            if (need_h) {
-                const float4 *hrow = reinterpret_cast<const float4 *>(
-                    tab + ((size_t)p.hydrogen_type * AG_NEINT + indx_h) * kMapSlots);
+            const float *row = tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots;
+            const float *hrow = tab + ((size_t)(p.hydrogen_type >= 0 ? p.hydrogen_type : 0) * AG_NEINT + hrow_bin) * kMapSlots;
 #pragma unroll
-                for (int q = 0; q < kMapSlots / 4; q++) {
-                    const float4 v = hrow[q];
-                    Eh[4 * q] = v.x; Eh[4 * q + 1] = v.y; Eh[4 * q + 2] = v.z; Eh[4 * q + 3] = v.w;
-                }
-            } else {
+            for (int q4 = 0; q4 < kMapSlots / 4; q4++) {
+                const float4 ev = reinterpret_cast<const float4 *>(row)[q4];
+                const float evs[4] = {ev.x, ev.y, ev.z, ev.w};
 #pragma unroll
-                for (int m = 0; m < kMapSlots; m++) Eh[m] = 0.f;
-            }
-            const float S = p.sol[indx_r];
-#pragma unroll
-            for (int m = 0; m < kMapSlots; m++) {
-                const int role = m < p.nslots ? p.role[m] : ROLE_NONE;
-                if (role & ROLE_AFF) {
-                    const int hc = (p.hbc[m] >> (4 * hb)) & 0xf;
-                    const float e = E[m];
-                    if (hc == HBC_PLAIN) {
-                        acc[m] += e;
-                    } else {
-                        float rsph = e / 100.f;
-                        rsph = fmaxf(rsph, 0.f);
-                        rsph = fminf(rsph, 1.f);
-                        if (hc == HBC_ACC_PROBE) {
-                            if (!dis) acc[m] += e * Hramp * (racc + (1.f - racc) * rsph);
-                            else acc[m] += Eh[m] * Hramp * (racc + (1.f - racc) * rsph);
-                        } else if (hc == HBC_MINMAX_RACC) {
-                            const float v = e * (racc + (1.f - racc) * rsph);
-                            hmin[m] = fminf(hmin[m], v);
-                            hmax[m] = fmaxf(hmax[m], v);
-                        } else {  // HBC_DON_PROBE
-                            if (dis) {
-                                acc[m] += Eh[m] * Hramp * (rdon + (1.f - rdon) * rsph);
+                for (int u = 0; u < 4; u++) {
+                    const int m = 4 * q4 + u;
+                    const int role = m < p.nslots ? p.role[m] : ROLE_NONE;
+                    if (role & ROLE_AFF) {
+                        const int hc = (p.hbc[m] >> (4 * hb)) & 0xf;
+                        const float e = evs[u];
+                        if (hc == HBC_PLAIN) {
+                            acc[m] += e;
+                        } else {
+                            float rsph = e / 100.f;
+                            rsph = fminf(fmaxf(rsph, 0.f), 1.f);
+                            if (hc == HBC_ACC_PROBE || (hc == HBC_DON_PROBE && dis)) {
+                                const float f = hc == HBC_ACC_PROBE ? racc : rdon;
+                                const float ee = dis ? hrow[m] : e;
+                                acc[m] += ee * Hramp * (f + (1.f - f) * rsph);
                             } else {
-                                const float v = e * (rdon + (1.f - rdon) * rsph);
-                                hmin[m] = fminf(hmin[m], v);
-                                hmax[m] = fmaxf(hmax[m], v);
+                                const float f = hc == HBC_MINMAX_RACC ? racc : rdon;
+                                const float v = e * (f + (1.f - f) * rsph);
+                                const int k = p.mm[m];
+#pragma unroll
+                                for (int kk = 0; kk < kMinMax; kk++)
+                                    if (k == kk) {
+                                        hmin[kk] = fminf(hmin[kk], v);
+                                        hmax[kk] = fmaxf(hmax[kk], v);
+                                    }
                             }
                         }
                     }
+                    if (role & ROLE_DES) acc[m] += p.sp[m] * avol * S + asolq * p.volp[m] * S;
                 }
-                if (role & ROLE_DES) acc[m] += p.sp[m] * a.vol * S + a.solq * p.volp[m] * S;
             }
-            if (p.d_slot >= 0) d_acc += a.dvol * S;
+            if (p.d_slot >= 0) d_acc += unif(a.dvol) * S;
         }
     }
 
@@ -661,10 +661,13 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     for (int m = 0; m < kMapSlots; m++) {
         if (m >= p.nslots || p.role[m] == ROLE_NONE) continue;
         float v = acc[m];
-        if (hmin[m] < 999999.f) v += hmin[m] + fmaxf(hmax[m], 0.f);
+        const int k = p.mm[m];
+#pragma unroll
+        for (int kk = 0; kk < kMinMax; kk++)
+            if (k == kk && hmin[kk] < 999999.f) v += hmin[kk] + fmaxf(hmax[kk], 0.f);
         p.maps[(long long)p.out_slot[m] * p.P + pt] = v;
     }
-    if (p.e_slot >= 0) p.maps[(long long)p.e_slot * p.P + pt] = e_near;
+    if (want_e) p.maps[(long long)p.e_slot * p.P + pt] = e_near;
     if (p.d_slot >= 0) p.maps[(long long)p.d_slot * p.P + pt] = d_acc;
 }
 
@@ -926,14 +929,41 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     for (int k = 0; k <= AG_NEINT; k++) sol[k] = (float)tab->sol_fn[k];
     float *d_sol = upload(st, p.sol, sol.data(), AG_NEINT + 1);
 
+    // Map groups for the near kernel: at most kMapSlots maps and kMinMax maps
+    // that keep an H-bond min/max per pass.
+    auto uses_minmax = [&](int m) {
+        const ag_map_desc &md = ms->maps[m];
+        if (md.is_covalent || !md.is_hbonder || map_role(ms, m) == ROLE_DES) return false;
+        for (int ah = 0; ah <= 6; ah++) {
+            const int cse = hb_case(md.hbond, ah);
+            if (cse == HBC_MINMAX_RACC || cse == HBC_DON_PROBE) return true;
+        }
+        return false;
+    };
+    std::vector<std::pair<int, int>> gr;   // (first map, count)
+    {
+        int m = 0;
+        while (m < M) {
+            int cnt = 0, nmm = 0;
+            while (m + cnt < M && cnt < kMapSlots) {
+                const int add = uses_minmax(m + cnt) ? 1 : 0;
+                if (nmm + add > kMinMax) break;
+                nmm += add;
+                cnt++;
+            }
+            gr.emplace_back(m, cnt);
+            m += cnt;
+        }
+        if (gr.empty()) gr.emplace_back(0, 0);
+    }
+    p.groups = (int)gr.size();
     // vdW/H-bond tables regrouped as [group][type][bin][16 map slots]
-    p.groups = M > 0 ? (M + kMapSlots - 1) / kMapSlots : 1;
     const size_t gstride = (size_t)(T > 0 ? T : 1) * AG_NEINT * kMapSlots;
     std::vector<float> gtab(gstride * p.groups, 0.f);
     for (int g = 0; g < p.groups; g++)
         for (int t = 0; t < T; t++)
-            for (int j = 0; j < kMapSlots && g * kMapSlots + j < M; j++) {
-                const float *srcrow = tab->e_vdw_hb + ((size_t)t * M + (g * kMapSlots + j)) * AG_NEINT;
+            for (int j = 0; j < gr[g].second; j++) {
+                const float *srcrow = tab->e_vdw_hb + ((size_t)t * M + (gr[g].first + j)) * AG_NEINT;
                 float *dst = gtab.data() + g * gstride + (size_t)t * AG_NEINT * kMapSlots + j;
                 for (int k = 0; k < AG_NEINT; k++) dst[(size_t)k * kMapSlots] = srcrow[k];
             }
@@ -970,16 +1000,19 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     np.P = P;
     np.status = (int *)p.status.ptr;
     for (int g = 0; g < p.groups; g++) {
-        const int m0 = g * kMapSlots;
-        const int ns = M > 0 ? std::min(kMapSlots, M - m0) : 0;
+        const int m0 = gr[g].first;
+        const int ns = gr[g].second;
         NearParams q = np;
         q.nslots = ns;
         bool has_aff = false;
+        int nmm = 0;
+        for (int j = 0; j < kMapSlots; j++) q.mm[j] = -1;
         for (int j = 0; j < ns; j++) {
             const int m = m0 + j;
             const ag_map_desc &md = ms->maps[m];
             q.out_slot[j] = m;
             q.role[j] = map_role(ms, m);
+            if (uses_minmax(m)) q.mm[j] = nmm++;
             if (q.role[j] == ROLE_COV) continue;
             has_aff = true;
             q.sp[j] = (float)md.solpar_probe;

@@ -75,7 +75,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="full", choices=["small", "full", "large"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cli", action="store_true", help="skip the drop-in run with ASCII output")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        hip = hip_runtime()
+        a, b = C.c_void_p(), C.c_void_p()
+        n = C.c_size_t(args.copy_calibration)
+        assert hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+        assert hip.hipMemset(a, 1, n) == 0
+        hip.hipDeviceSynchronize()
+        assert hip.hipMemcpy(b, a, n, 3) == 0   # device to device
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a)
+        hip.hipFree(b)
+        print(json.dumps({"copy_bytes": args.copy_calibration}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the map set shards by "
               "z-slab across GPUs without communication; see DESIGN.md).", file=sys.stderr)
@@ -123,14 +139,16 @@ def main():
         ctx.close()
 
         # whole drop-in with ASCII output, for the wall time the reference reports
-        os.chdir(work)
-        try:
-            t1 = time.perf_counter()
-            p = subprocess.run([ag.CLI_PATH, "-p", info["gpf"], "-l", "gpu.glg"], capture_output=True)
-            wall_cli = time.perf_counter() - t1
-        finally:
-            os.chdir(cwd)
-        cli_ok = p.returncode == 0
+        cli_ok, wall_cli = False, None
+        if not args.no_cli:
+            os.chdir(work)
+            try:
+                t1 = time.perf_counter()
+                p = subprocess.run([ag.CLI_PATH, "-p", info["gpf"], "-l", "gpu.glg"], capture_output=True)
+                wall_cli = time.perf_counter() - t1
+            finally:
+                os.chdir(cwd)
+            cli_ok = p.returncode == 0
     finally:
         shutil.rmtree(work, ignore_errors=True)
 

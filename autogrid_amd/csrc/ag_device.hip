@@ -13,7 +13,7 @@
 //                    desolvation, the d map, the H-bond min/max fix-up, and
 //                    the electrostatic terms of every atom the far kernel
 //                    leaves out.  Bin indices floor(128 r) are taken from an
-//                    FP32 distance unless 128 r lies within 5e-4 of an
+//                    FP32 distance unless 128 r lies within 1e-3 of an
 //                    integer, where the reference's FP64 distance is
 //                    recomputed, so the table bins match the reference's.
 //   3. far kernel  : the electrostatic sum over all atoms (main.cpp:2940-2982)
@@ -26,6 +26,8 @@
 // No atomics touch map values; every sum has a fixed order.
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -48,21 +50,25 @@ namespace {
 constexpr int kBX = 8, kBY = 8, kBZ = 4;          // brick of the near kernel
 constexpr int kNearThreads = kBX * kBY * kBZ;     // 256
 constexpr int kMapSlots = 16;                     // maps per near-kernel pass
+constexpr int kPlainSlots = 12;                   // of which without H-bond terms
+constexpr int kHbSlots = kMapSlots - kPlainSlots; // of which H-bonding probes
 constexpr int kCandCap = 2048;                    // candidates per brick
 constexpr int kHCap = 256;                        // donor-H candidates per brick
 constexpr int kChunk = 128;                       // candidate records staged per pass
-constexpr int kMinMax = 4;                        // H-bond min/max maps per pass
 constexpr int kNearRepsLen = 1040;                // electrostatic bins used by the near kernel
 constexpr float kNearRadius = 8.03f;              // staging radius (A)
 constexpr int kFarK = 8;                          // points per lane, far kernel
-constexpr int kFarThreads = 256;
+constexpr int kFarThreads = 512;
+constexpr int kFarTabMax = 16384;                 // LDS table entries without the clamp
 constexpr int kFarAtomChunk = 64;                 // Kahan block
 constexpr int kFarBinMin = 1025;                  // far kernel takes bins >= this
 constexpr float kFarBias = 1.0e-3f;               // added to (128 r)^2, keeps rsq finite
-constexpr float kGuardBins = 5.0e-4f;             // FP64 recompute band around bin edges
+constexpr float kGuardBins = 1.0e-3f;             // FP64 recompute band around bin edges
 
-// Map roles of the near kernel.
+// Map roles.
 enum : int { ROLE_NONE = 0, ROLE_AFF = 1, ROLE_DES = 2, ROLE_COV = 4 };
+// Coefficient bits of an H-bonding slot for one atom class.
+enum : int { HC_PLAIN = 1, HC_HRAMP = 2, HC_MINMAX = 4, HC_RACC = 8 };
 // H-bond cases (main.cpp:3286-3390).
 enum : int { HBC_PLAIN = 0, HBC_ACC_PROBE = 1, HBC_MINMAX_RACC = 2, HBC_DON_PROBE = 3 };
 
@@ -99,18 +105,17 @@ struct NearParams {
     const float *tab;             // [T][1024][kMapSlots] for this pass's maps
     const float *sol;             // [1025]
     const float *reps;            // [8192] electrostatic factor (r_eps or 332/diel)
-    // maps of this pass
-    int nslots;
-    int role[kMapSlots];
-    int hbc[kMapSlots];           // 4 bits per atom hbond class 0..6
-    float sp[kMapSlots];          // solpar_probe
-    float volp[kMapSlots];        // vol_probe
-    int out_slot[kMapSlots];      // output slot of each map
-    int mm[kMapSlots];            // min/max register of each map, -1 if none
+    // map slots of this pass: [0, kPlainSlots) maps without H-bond terms,
+    // [kPlainSlots, kMapSlots) H-bonding probes
+    float kE[kMapSlots];          // 1: add the vdW/H-bond table value; 0: desolvation only or unused
+    float sp[kMapSlots];          // solpar_probe, 0 when the slot carries no desolvation
+    float volp[kMapSlots];        // vol_probe, 0 when the slot carries no desolvation
+    int hcode[kHbSlots][7];       // per atom H-bond class: bits 0-3 ordered atom, 4-7 disordered
+                                  // atom; each nibble kP | kH << 1 | kM << 2 | use_racc << 3
+    int out_slot[kMapSlots];      // output map of each slot, -1 if unused
     int e_slot, d_slot;           // -1 when not produced in this pass
-    int has_affinity;             // any AFF/DES slot, or d map
+    int has_affinity;             // any slot in use, or the d map
     int disorder_h, hydrogen_type;
-    double cov_pos[3], cov_hw, cov_barrier;
     float *maps;
     long long P;
     int *status;
@@ -306,6 +311,7 @@ __device__ __forceinline__ float unif(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+template <int NPLAIN>
 __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     __shared__ int cand[kCandCap];
     __shared__ int hcand[kHCap];
@@ -435,23 +441,13 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     }
     __syncthreads();
 
-    // ---- covalent maps (main.cpp:2866-2879) ----
     float acc[kMapSlots];
 #pragma unroll
-    for (int m = 0; m < kMapSlots; m++) {
-        acc[m] = 0.f;
-        if (m < p.nslots && p.role[m] == ROLE_COV) {
-            double dx = p.cov_pos[0] - cx, dy = p.cov_pos[1] - cy, dz = p.cov_pos[2] - cz;
-            double rcov = sqrt(dx * dx + dy * dy + dz * dz) / p.cov_hw;
-            if (rcov < 1.0e-6) rcov = 1.0e-6;
-            acc[m] = (float)(p.cov_barrier * (1. - exp(-0.69314718055994530942 * rcov * rcov)));
-        }
-    }
-    // min/max of the H-bond pairs (main.cpp:3300-3374, 3420-3431), at most
-    // kMinMax slots per pass
-    float hmin[kMinMax], hmax[kMinMax];
+    for (int m = 0; m < kMapSlots; m++) acc[m] = 0.f;
+    // min/max of the H-bond pairs of each H-bonding slot (main.cpp:3300-3374)
+    float hmin[kHbSlots], hmax[kHbSlots];
 #pragma unroll
-    for (int k = 0; k < kMinMax; k++) {
+    for (int k = 0; k < kHbSlots; k++) {
         hmin[k] = 999999.f;
         hmax[k] = -999999.f;
     }
@@ -492,7 +488,9 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             const float dxf = (float)dx64, dyf = (float)dy64, dzf = (float)dz64;
             const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
             if (r2 > 64.33f) continue;  // beyond 8.02 A: neither cutoff pass nor near e term
-            float r = sqrtf(r2);
+            // hardware rsq: 128 r is good to ~3e-4 bins here, inside the guard band
+            const float r2c = fmaxf(r2, 1.0e-12f);
+            float r = r2c * __builtin_amdgcn_rsqf(r2c);
             const float g = r * 128.f;
             int bin;
             bool cut;
@@ -513,8 +511,8 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 else if (g > 1027.f) near_e = false;
                 else near_e = far_rb(a.xE, a.yE, a.zE, pxE, pyE, pzE) < (float)kFarBinMin;
                 if (near_e) {
-                    const float rr = r < 1.0e-6f ? 1.0e-6f : r;
-                    const float inv_rmax = 1.f / (rr > 0.5f ? rr : 0.5f);
+                    // 1/max(r, 0.5) with r clamped at APPROX_ZERO
+                    const float inv_rmax = fminf(__builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f)), 2.f);
                     e_near += unif(a.qe) * inv_rmax * reps_s[min(bin, kNearRepsLen - 1)];
                 }
             }
@@ -523,8 +521,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             const int type = meta & 0xff, hb = (meta >> 8) & 0xff, rexp = (meta >> 16) & 0xff;
             const bool dis = (meta >> 24) & 1;
             if (p.disorder_h && dis && type == p.hydrogen_type) continue;
-            const float rr = r < 1.0e-6f ? 1.0e-6f : r;
-            const float inv_r = 1.f / rr;
+            const float inv_r = __builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f));
             const float dn0 = dxf * inv_r, dn1 = dyf * inv_r, dn2 = dzf * inv_r;
             const float rv0 = unif(a.rv[0]), rv1 = unif(a.rv[1]), rv2 = unif(a.rv[2]);
             const int aidx = uni(a.index);
@@ -568,7 +565,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                     const float rw0 = unif(a.rv2[0]), rw1 = unif(a.rv2[1]), rw2 = unif(a.rv2[2]);
                     float t0 = dn0 * rw0 + dn1 * rw1 + dn2 * rw2;
                     t0 = fminf(fmaxf(t0, -1.f), 1.f);
-                    const float cos_t0 = sqrtf(fmaxf(1.f - t0 * t0, 0.f));
+                    const float cos_t0 = __builtin_amdgcn_sqrtf(fmaxf(1.f - t0 * t0, 0.f));
                     rdon = 0.f;
                     if (pos) {
                         const float cr0 = dn1 * rw2 - dn2 * rw1;
@@ -576,10 +573,10 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                         const float cr2 = dn0 * rw1 - dn1 * rw0;
                         float rd2 = cr0 * cr0 + cr1 * cr1 + cr2 * cr2;
                         if (rd2 < 1.0e-6f) rd2 = 1.0e-6f;
-                        const float inv_rd = 1.f / sqrtf(rd2);
+                        const float inv_rd = __builtin_amdgcn_rsqf(rd2);
                         float ti = (cr0 * inv_rd) * rv0 + (cr1 * inv_rd) * rv1 + (cr2 * inv_rd) * rv2;
                         ti = fminf(fmaxf(ti, -1.f), 1.f);
-                        const float sin2ti = 2.f * fabsf(ti) * sqrtf(fmaxf(1.f - ti * ti, 0.f));
+                        const float sin2ti = 2.f * fabsf(ti) * __builtin_amdgcn_sqrtf(fmaxf(1.f - ti * ti, 0.f));
                         rdon = (0.9f + 0.1f * sin2ti) * cos_t0;
                     } else if (cos_t >= -0.34202f) {
                         const float u = 0.116978f - cos_t * cos_t;
@@ -597,7 +594,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 racc = 0.f;
                 rdon = 0.f;
                 if (c >= -sdis) {
-                    const float cc = c * cdis + sqrtf(fmaxf(1.f - c * c, 0.f)) * sdis;
+                    const float cc = c * cdis + __builtin_amdgcn_sqrtf(fmaxf(1.f - c * c, 0.f)) * sdis;
                     rdon = (cc * cc) * (cc * cc);
                     racc = rdon;
                 }
@@ -606,50 +603,59 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
             const float S = sol_s[indx_r];
             const float avol = unif(a.vol), asolq = unif(a.solq);
-            // disordered partner: hydrogen-type row at r - 1.10 (main.cpp:3313-3349)
-            int hrow_bin = 0;
+            // vdW/H-bond table values of this pair for the pass's slots
+            const float *row = tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots;
+            float E[kMapSlots];
+#pragma unroll
+            for (int q4 = 0; q4 < kMapSlots / 4; q4++) {
+                if (q4 < NPLAIN / 4 || q4 == kMapSlots / 4 - 1) {
+                    const float4 v = reinterpret_cast<const float4 *>(row)[q4];
+                    E[4 * q4] = v.x; E[4 * q4 + 1] = v.y; E[4 * q4 + 2] = v.z; E[4 * q4 + 3] = v.w;
+                } else {
+                    E[4 * q4] = E[4 * q4 + 1] = E[4 * q4 + 2] = E[4 * q4 + 3] = 0.f;
+                }
+            }
+            // plain slots: table value + desolvation (main.cpp:3375-3390)
+#pragma unroll
+            for (int m = 0; m < NPLAIN; m++) {
+                const float t = __builtin_fmaf(p.sp[m], avol, p.volp[m] * asolq);
+                acc[m] = __builtin_fmaf(t, S, __builtin_fmaf(p.kE[m], E[m], acc[m]));
+            }
+            // H-bonding slots (main.cpp:3286-3374): the (probe, atom) case is a
+            // set of 0/1 coefficients chosen per atom class, applied without
+            // branches.  A disordered partner uses the hydrogen-type row at
+            // r - 1.10 A (main.cpp:3313-3349), FP64 like the reference.
+            float Eh[kHbSlots] = {0.f, 0.f, 0.f, 0.f};
             if (dis && p.hydrogen_type >= 0) {
                 double rdx, rdy, rdz;
                 const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
-                hrow_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
+                const int hb_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
+                const float4 v = reinterpret_cast<const float4 *>(
+                    tab + ((size_t)p.hydrogen_type * AG_NEINT + hb_bin) * kMapSlots)[kPlainSlots / 4];
+                Eh[0] = v.x; Eh[1] = v.y; Eh[2] = v.z; Eh[3] = v.w;
             }
-            const float *row = tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots;
-            const float *hrow = tab + ((size_t)(p.hydrogen_type >= 0 ? p.hydrogen_type : 0) * AG_NEINT + hrow_bin) * kMapSlots;
+            const float dfac = racc - rdon;
+            const int dsh = dis ? 4 : 0;
 #pragma unroll
-            for (int q4 = 0; q4 < kMapSlots / 4; q4++) {
-                const float4 ev = reinterpret_cast<const float4 *>(row)[q4];
-                const float evs[4] = {ev.x, ev.y, ev.z, ev.w};
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int m = 4 * q4 + u;
-                    const int role = m < p.nslots ? p.role[m] : ROLE_NONE;
-                    if (role & ROLE_AFF) {
-                        const int hc = (p.hbc[m] >> (4 * hb)) & 0xf;
-                        const float e = evs[u];
-                        if (hc == HBC_PLAIN) {
-                            acc[m] += e;
-                        } else {
-                            float rsph = e / 100.f;
-                            rsph = fminf(fmaxf(rsph, 0.f), 1.f);
-                            if (hc == HBC_ACC_PROBE || (hc == HBC_DON_PROBE && dis)) {
-                                const float f = hc == HBC_ACC_PROBE ? racc : rdon;
-                                const float ee = dis ? hrow[m] : e;
-                                acc[m] += ee * Hramp * (f + (1.f - f) * rsph);
-                            } else {
-                                const float f = hc == HBC_MINMAX_RACC ? racc : rdon;
-                                const float v = e * (f + (1.f - f) * rsph);
-                                const int k = p.mm[m];
-#pragma unroll
-                                for (int kk = 0; kk < kMinMax; kk++)
-                                    if (k == kk) {
-                                        hmin[kk] = fminf(hmin[kk], v);
-                                        hmax[kk] = fmaxf(hmax[kk], v);
-                                    }
-                            }
-                        }
-                    }
-                    if (role & ROLE_DES) acc[m] += p.sp[m] * avol * S + asolq * p.volp[m] * S;
-                }
+            for (int k = 0; k < kHbSlots; k++) {
+                const int m = kPlainSlots + k;
+                const int code = (p.hcode[k][hb] >> dsh) & 15;
+                const float kP = (code & HC_PLAIN) ? 1.f : 0.f;
+                const float kH = (code & HC_HRAMP) ? 1.f : 0.f;
+                const float kF = (code & HC_RACC) ? 1.f : 0.f;
+                const bool kM = (code & HC_MINMAX) != 0;
+                const float e = E[m];
+                const float rsph = __builtin_amdgcn_fmed3f(e * 0.01f, 0.f, 1.f);
+                const float f = __builtin_fmaf(kF, dfac, rdon);
+                const float mix = __builtin_fmaf(1.f - f, rsph, f);
+                const float eh = dis ? Eh[k] : e;
+                acc[m] = __builtin_fmaf(kP, e, acc[m]);
+                acc[m] = __builtin_fmaf(kH, (eh * Hramp) * mix, acc[m]);
+                const float v = e * mix;
+                hmin[k] = kM ? fminf(hmin[k], v) : hmin[k];
+                hmax[k] = kM ? fmaxf(hmax[k], v) : hmax[k];
+                const float t = __builtin_fmaf(p.sp[m], avol, p.volp[m] * asolq);
+                acc[m] = __builtin_fmaf(t, S, acc[m]);
             }
             if (p.d_slot >= 0) d_acc += unif(a.dvol) * S;
         }
@@ -659,22 +665,51 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     const long long pt = (long long)ix + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
 #pragma unroll
     for (int m = 0; m < kMapSlots; m++) {
-        if (m >= p.nslots || p.role[m] == ROLE_NONE) continue;
+        if ((m >= NPLAIN && m < kPlainSlots) || p.out_slot[m] < 0) continue;
         float v = acc[m];
-        const int k = p.mm[m];
-#pragma unroll
-        for (int kk = 0; kk < kMinMax; kk++)
-            if (k == kk && hmin[kk] < 999999.f) v += hmin[kk] + fmaxf(hmax[kk], 0.f);
+        if (m >= kPlainSlots && hmin[m - kPlainSlots] < 999999.f)
+            v += hmin[m - kPlainSlots] + fmaxf(hmax[m - kPlainSlots], 0.f);
         p.maps[(long long)p.out_slot[m] * p.P + pt] = v;
     }
     if (want_e) p.maps[(long long)p.e_slot * p.P + pt] = e_near;
     if (p.d_slot >= 0) p.maps[(long long)p.d_slot * p.P + pt] = d_acc;
 }
 
+// Covalent maps (main.cpp:2866-2879): a Gaussian well around the
+// attachment point, no receptor atoms involved.
+struct CovParams {
+    int n1[3], ne[3];
+    double spacing;
+    double pos[3], half_width, barrier;
+    int n_cov;
+    int slot[16];
+    float *maps;
+    long long P;
+};
+
+__global__ void k_covalent(CovParams c) {
+    const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pt >= c.P) return;
+    const int ix = (int)(pt % c.n1[0]);
+    const int iy = (int)((pt / c.n1[0]) % c.n1[1]);
+    const int iz = (int)(pt / ((long long)c.n1[0] * c.n1[1]));
+    const double dx = c.pos[0] - (double)(ix - c.ne[0]) * c.spacing;
+    const double dy = c.pos[1] - (double)(iy - c.ne[1]) * c.spacing;
+    const double dz = c.pos[2] - (double)(iz - c.ne[2]) * c.spacing;
+    double rcov = sqrt(dx * dx + dy * dy + dz * dz) / c.half_width;
+    if (rcov < 1.0e-6) rcov = 1.0e-6;
+    const float v = (float)(c.barrier * (1. - exp(-0.69314718055994530942 * rcov * rcov)));
+    for (int k = 0; k < c.n_cov; k++) c.maps[(long long)c.slot[k] * c.P + pt] = v;
+}
+
 // ---------------------------------------------------------------------------
 // far kernel
 // ---------------------------------------------------------------------------
 
+// CLAMP: the table stops at bin 8191 and indices are clamped as the
+// reference clamps them (main.cpp:2956); without it the table is extended
+// to the largest bin any pair can reach, every entry past 8191 holding the
+// reference's last value, and the clamp instruction disappears.
 template <bool CLAMP>
 __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
     extern __shared__ float T[];
@@ -703,21 +738,27 @@ __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
         float part[kFarK];
 #pragma unroll
         for (int k = 0; k < kFarK; k++) part[k] = 0.f;
-#pragma unroll 4
+#pragma unroll 2
         for (int a = a0; a < a1; a++) {
 #pragma clang fp contract(off)
             const float4 at = atoms[a];
             const float dy = at.y - py, dz = at.z - pz;
             const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
+            float rs[kFarK];
+            int bin[kFarK];
 #pragma unroll
             for (int k = 0; k < kFarK; k++) {
                 const float dx = at.x - px[k];
                 const float rb2 = __builtin_fmaf(dx, dx, dyz2);
-                const float rs = __builtin_amdgcn_rsqf(rb2);
-                const float rb = rb2 * rs;
-                const int bin = CLAMP ? (int)fminf(rb, p.tmax) : (int)rb;
-                part[k] = __builtin_fmaf(at.w, rs * T[bin], part[k]);
+                rs[k] = __builtin_amdgcn_rsqf(rb2);
+                const float rb = rb2 * rs[k];
+                bin[k] = CLAMP ? (int)fminf(rb, p.tmax) : (int)rb;
             }
+            float tv[kFarK];
+#pragma unroll
+            for (int k = 0; k < kFarK; k++) tv[k] = T[bin[k]];
+#pragma unroll
+            for (int k = 0; k < kFarK; k++) part[k] = __builtin_fmaf(at.w, rs[k] * tv[k], part[k]);
         }
 #pragma unroll
         for (int k = 0; k < kFarK; k++) {
@@ -784,9 +825,13 @@ struct ag_plan {
     CellParams cp{};
     int ncells = 0;
     AtomSrc src{};
-    std::vector<NearParams> near;      // one per group of 16 map slots
+    std::vector<NearParams> near;      // one per pass of 16 map slots
+    int nplain = 12;                   // plain slots in use (4, 8 or 12)
+    CovParams cov{};
+    bool run_cov = false;
     FarParams far{};
     bool run_far = false;
+    bool far_clamp = true;
 };
 
 namespace agdev {
@@ -929,47 +974,48 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     for (int k = 0; k <= AG_NEINT; k++) sol[k] = (float)tab->sol_fn[k];
     float *d_sol = upload(st, p.sol, sol.data(), AG_NEINT + 1);
 
-    // Map groups for the near kernel: at most kMapSlots maps and kMinMax maps
-    // that keep an H-bond min/max per pass.
-    auto uses_minmax = [&](int m) {
-        const ag_map_desc &md = ms->maps[m];
-        if (md.is_covalent || !md.is_hbonder || map_role(ms, m) == ROLE_DES) return false;
-        for (int ah = 0; ah <= 6; ah++) {
-            const int cse = hb_case(md.hbond, ah);
-            if (cse == HBC_MINMAX_RACC || cse == HBC_DON_PROBE) return true;
-        }
-        return false;
-    };
-    std::vector<std::pair<int, int>> gr;   // (first map, count)
-    {
-        int m = 0;
-        while (m < M) {
-            int cnt = 0, nmm = 0;
-            while (m + cnt < M && cnt < kMapSlots) {
-                const int add = uses_minmax(m + cnt) ? 1 : 0;
-                if (nmm + add > kMinMax) break;
-                nmm += add;
-                cnt++;
-            }
-            gr.emplace_back(m, cnt);
-            m += cnt;
-        }
-        if (gr.empty()) gr.emplace_back(0, 0);
+    // Map slots of the near-kernel passes.  Maps without H-bond terms (and
+    // the desolvation halves of separate_desolvation_maps) take plain slots,
+    // H-bonding probes take the H-bond slots; covalent maps take none.
+    std::vector<int> plain_maps, hb_maps;
+    std::vector<int> cov_maps;
+    for (int m = 0; m < M; m++) {
+        const int role = map_role(ms, m);
+        if (role == ROLE_COV) cov_maps.push_back(m);
+        else if ((role & ROLE_AFF) && ms->maps[m].is_hbonder) hb_maps.push_back(m);
+        else plain_maps.push_back(m);
     }
-    p.groups = (int)gr.size();
-    // vdW/H-bond tables regrouped as [group][type][bin][16 map slots]
+    p.groups = std::max<int>(1, std::max((int)(plain_maps.size() + kPlainSlots - 1) / kPlainSlots,
+                                         (int)(hb_maps.size() + kHbSlots - 1) / kHbSlots));
     const size_t gstride = (size_t)(T > 0 ? T : 1) * AG_NEINT * kMapSlots;
     std::vector<float> gtab(gstride * p.groups, 0.f);
-    for (int g = 0; g < p.groups; g++)
+    std::vector<std::array<int, kMapSlots>> slot_map(p.groups);
+    int nplain_max = 0;
+    for (int g = 0; g < p.groups; g++) {
+        slot_map[g].fill(-1);
+        int np_g = 0;
+        for (int j = 0; j < kPlainSlots; j++) {
+            const size_t i = (size_t)g * kPlainSlots + j;
+            if (i < plain_maps.size()) { slot_map[g][j] = plain_maps[i]; np_g = j + 1; }
+        }
+        nplain_max = std::max(nplain_max, np_g);
+        for (int k = 0; k < kHbSlots; k++) {
+            const size_t i = (size_t)g * kHbSlots + k;
+            if (i < hb_maps.size()) slot_map[g][kPlainSlots + k] = hb_maps[i];
+        }
         for (int t = 0; t < T; t++)
-            for (int j = 0; j < gr[g].second; j++) {
-                const float *srcrow = tab->e_vdw_hb + ((size_t)t * M + (gr[g].first + j)) * AG_NEINT;
+            for (int j = 0; j < kMapSlots; j++) {
+                const int m = slot_map[g][j];
+                if (m < 0) continue;
+                const float *srcrow = tab->e_vdw_hb + ((size_t)t * M + m) * AG_NEINT;
                 float *dst = gtab.data() + g * gstride + (size_t)t * AG_NEINT * kMapSlots + j;
                 for (int k = 0; k < AG_NEINT; k++) dst[(size_t)k * kMapSlots] = srcrow[k];
             }
+    }
+    p.nplain = nplain_max <= 4 ? 4 : (nplain_max <= 8 ? 8 : 12);
     float *d_tab = upload(st, p.tab, gtab.data(), gtab.size());
 
-    // near-kernel parameters, one set per group of 16 map slots
+    // near-kernel parameters, one set per pass
     NearParams np;
     memset(&np, 0, sizeof np);
     for (int d = 0; d < 3; d++) {
@@ -994,39 +1040,66 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     np.reps = d_reps;
     np.disorder_h = ms->disorder_h;
     np.hydrogen_type = ms->hydrogen_type;
-    for (int d = 0; d < 3; d++) np.cov_pos[d] = ms->cov_pos[d];
-    np.cov_hw = ms->cov_half_width;
-    np.cov_barrier = ms->cov_barrier;
     np.P = P;
     np.status = (int *)p.status.ptr;
     for (int g = 0; g < p.groups; g++) {
-        const int m0 = gr[g].first;
-        const int ns = gr[g].second;
         NearParams q = np;
-        q.nslots = ns;
-        bool has_aff = false;
-        int nmm = 0;
-        for (int j = 0; j < kMapSlots; j++) q.mm[j] = -1;
-        for (int j = 0; j < ns; j++) {
-            const int m = m0 + j;
-            const ag_map_desc &md = ms->maps[m];
+        bool used = false;
+        for (int j = 0; j < kMapSlots; j++) {
+            const int m = slot_map[g][j];
             q.out_slot[j] = m;
-            q.role[j] = map_role(ms, m);
-            if (uses_minmax(m)) q.mm[j] = nmm++;
-            if (q.role[j] == ROLE_COV) continue;
-            has_aff = true;
-            q.sp[j] = (float)md.solpar_probe;
-            q.volp[j] = (float)md.vol_probe;
-            int hc = 0;
-            if (md.is_hbonder)
-                for (int ah = 0; ah <= 6; ah++) hc |= hb_case(md.hbond, ah) << (4 * ah);
-            q.hbc[j] = hc;
+            q.kE[j] = 0.f;
+            q.sp[j] = 0.f;
+            q.volp[j] = 0.f;
+            if (m < 0) continue;
+            used = true;
+            const int role = map_role(ms, m);
+            const ag_map_desc &md = ms->maps[m];
+            q.kE[j] = (role & ROLE_AFF) ? 1.f : 0.f;
+            if (role & ROLE_DES) {
+                q.sp[j] = (float)md.solpar_probe;
+                q.volp[j] = (float)md.vol_probe;
+            }
+            if (j >= kPlainSlots) {
+                const int k = j - kPlainSlots;
+                for (int ah = 0; ah <= 6; ah++) {
+                    const int cse = hb_case(md.hbond, ah);
+                    int ordered = 0, disordered = 0;
+                    if (cse == HBC_PLAIN) ordered = disordered = HC_PLAIN;
+                    else if (cse == HBC_ACC_PROBE) ordered = disordered = HC_HRAMP | HC_RACC;
+                    else if (cse == HBC_MINMAX_RACC) ordered = disordered = HC_MINMAX | HC_RACC;
+                    else {  // HBC_DON_PROBE
+                        ordered = HC_MINMAX;
+                        disordered = HC_HRAMP;
+                    }
+                    q.hcode[k][ah] = ordered | (disordered << 4);
+                }
+            }
         }
         q.tab = d_tab + g * gstride;
         q.e_slot = (g == 0) ? ms->elec_index : -1;
         q.d_slot = (g == 0) ? ms->dsolv_index : -1;
-        q.has_affinity = has_aff || q.d_slot >= 0;
+        q.has_affinity = used || q.d_slot >= 0;
         p.near.push_back(q);
+    }
+
+    // covalent maps
+    if (!cov_maps.empty()) {
+        if (cov_maps.size() > 16) ag::fail(AG_ERR_UNSUPPORTED, "more than 16 covalent maps");
+        CovParams &cv = p.cov;
+        memset(&cv, 0, sizeof cv);
+        for (int d = 0; d < 3; d++) {
+            cv.n1[d] = grid->n1[d];
+            cv.ne[d] = grid->ne[d];
+            cv.pos[d] = ms->cov_pos[d];
+        }
+        cv.spacing = grid->spacing;
+        cv.half_width = ms->cov_half_width;
+        cv.barrier = ms->cov_barrier;
+        cv.n_cov = (int)cov_maps.size();
+        for (int k = 0; k < cv.n_cov; k++) cv.slot[k] = cov_maps[k];
+        cv.P = P;
+        p.run_cov = true;
     }
 
     // far kernel (e map): per-atom float4 and the table with bins < 1025 zeroed
@@ -1035,9 +1108,24 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
         float4 *d_far = (float4 *)p.far_atoms.get(sizeof(float4) * (size_t)N);
         k_far_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_charge, ms->coeff_estat, d_far);
         HIP_CHECK(hipGetLastError());
-        std::vector<float> ft(AG_NDIEL);
-        for (int k = 0; k < AG_NDIEL; k++) ft[k] = k < kFarBinMin ? 0.f : reps[k];
-        float *d_ft = upload(st, p.fartab, ft.data(), AG_NDIEL);
+        // largest bin any (point, atom) pair reaches: farthest box corner
+        double rmax = 0.0;
+        for (int i = 0; i < N; i++) {
+            double q = 0.0;
+            for (int d = 0; d < 3; d++) {
+                const double lo = -grid->ne[d] * grid->spacing, hi = (grid->n1[d] - 1 - grid->ne[d]) * grid->spacing;
+                const double a = rec->coord[3 * (size_t)i + d];
+                const double e = std::max(std::fabs(a - lo), std::fabs(a - hi));
+                q += e * e;
+            }
+            rmax = std::max(rmax, q);
+        }
+        const long long need = (long long)(std::sqrt(rmax) * 128.0) + 8;
+        p.far_clamp = need > kFarTabMax;
+        const int tab_len = p.far_clamp ? AG_NDIEL : (int)std::max<long long>(need, kFarBinMin + 1);
+        std::vector<float> ft(tab_len);
+        for (int k = 0; k < tab_len; k++) ft[k] = k < kFarBinMin ? 0.f : reps[std::min(k, AG_NDIEL - 1)];
+        float *d_ft = upload(st, p.fartab, ft.data(), ft.size());
         FarParams &fp = p.far;
         memset(&fp, 0, sizeof fp);
         for (int d = 0; d < 3; d++) {
@@ -1050,8 +1138,8 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
         fp.atoms = d_far;
         fp.n_atoms = N;
         fp.tab = d_ft;
-        fp.tab_len = AG_NDIEL;
-        fp.tmax = (float)(AG_NDIEL - 1);
+        fp.tab_len = tab_len;
+        fp.tmax = (float)(tab_len - 1);
     }
     HIP_CHECK(hipStreamSynchronize(st));   // host staging vectors go out of scope
     return pl.release();
@@ -1092,7 +1180,16 @@ void plan_execute(ag_plan *p, float *d_maps) {
     for (auto &q : p->near) {
         NearParams np = q;
         np.maps = d_maps;
-        if (np.nslots > 0 || np.e_slot >= 0 || np.d_slot >= 0) k_near<<<np.nbricks, kNearThreads, 0, st>>>(np);
+        if (!(np.has_affinity || np.e_slot >= 0)) continue;
+        if (p->nplain == 4) k_near<4><<<np.nbricks, kNearThreads, 0, st>>>(np);
+        else if (p->nplain == 8) k_near<8><<<np.nbricks, kNearThreads, 0, st>>>(np);
+        else k_near<12><<<np.nbricks, kNearThreads, 0, st>>>(np);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (p->run_cov) {
+        CovParams cv = p->cov;
+        cv.maps = d_maps;
+        k_covalent<<<(unsigned)((p->P + 255) / 256), 256, 0, st>>>(cv);
         HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipEventRecord(c->ev[2], st));
@@ -1100,7 +1197,8 @@ void plan_execute(ag_plan *p, float *d_maps) {
         FarParams fp = p->far;
         fp.emap = d_maps + (size_t)p->elec_index * p->P;
         const int blocks = (fp.nruns + kFarThreads - 1) / kFarThreads;
-        k_far<true><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
+        if (p->far_clamp) k_far<true><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
+        else k_far<false><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
         HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipEventRecord(c->ev[3], st));

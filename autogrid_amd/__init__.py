@@ -253,8 +253,12 @@ class Context:
         if out is None:
             out = np.empty((st.n_maps, nz, ny, nx), np.float32)
         assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == st.n_maps * st.points
+        fmap = np.empty((nz, ny, nx), np.float32) if st.floating_grid else None
+        cmap = np.empty((nz, ny, nx), np.int32) if st.constriction_grid else None
         check(lib().ag_maps_plain(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
-                                  out.ctypes.data, None, None, 0))
+                                  out.ctypes.data, fmap.ctypes.data if fmap is not None else None,
+                                  cmap.ctypes.data if cmap is not None else None, 0))
+        self.last_fmap, self.last_cmap = fmap, cmap
         return out
 
     def maps_plain_device(self, st: PreparedState, out_ptr: int) -> None:
@@ -290,9 +294,10 @@ class Plan:
         self.n_maps = st.n_maps
         self.points = st.points
 
-    def execute(self, maps_dev_ptr: int) -> None:
+    def execute(self, maps_dev_ptr: int, fmap_dev_ptr: int | None = None, cmap_dev_ptr: int | None = None) -> None:
         """All maps into device memory at maps_dev_ptr ([n_maps][P] float32)."""
-        check(lib().ag_plan_execute(self._h, C.c_void_p(maps_dev_ptr)))
+        check(lib().ag_plan_execute(self._h, C.c_void_p(maps_dev_ptr), C.c_void_p(fmap_dev_ptr),
+                                    C.c_void_p(cmap_dev_ptr)))
 
     def close(self):
         if getattr(self, "_h", None):

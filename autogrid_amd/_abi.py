@@ -113,7 +113,7 @@ def lib() -> C.CDLL:
         L.ag_job_write_maps.argtypes = [vp, vp, vp, vp, C.c_int]
         L.ag_plan_create.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
                                      C.POINTER(Tables), C.POINTER(vp)]
-        L.ag_plan_execute.argtypes = [vp, vp]
+        L.ag_plan_execute.argtypes = [vp, vp, vp, vp]
         L.ag_plan_destroy.argtypes = [vp]
         L.ag_context_timing.argtypes = [vp, C.POINTER(C.c_float)]
         for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_context_timing",

@@ -116,6 +116,8 @@ struct NearParams {
     int e_slot, d_slot;           // -1 when not produced in this pass
     int has_affinity;             // any slot in use, or the d map
     int disorder_h, hydrogen_type;
+    int *cmap;                    // constriction count per point, or null (pass 0 only)
+    double ccut;                  // constriction distance cutoff (<= 8 A)
     float *maps;
     long long P;
     int *status;
@@ -471,6 +473,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     }
 
     float e_near = 0.f, d_acc = 0.f;
+    int ccount = 0;
     const float *tab = p.tab;
     const bool want_e = p.e_slot >= 0;
     // cos and sin of 1.24791 rad, the disordered-hydroxyl offset angle
@@ -503,6 +506,15 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             } else {
                 bin = (int)g;
                 cut = r > 8.0f;
+            }
+            // constriction grid: atoms within the cutoff (main.cpp:2969-2973)
+            if (p.cmap) {
+                if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
+                    double rdx, rdy, rdz;
+                    ccount += ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz) <= p.ccut;
+                } else {
+                    ccount += r <= (float)p.ccut;
+                }
             }
             // electrostatics of pairs the far kernel leaves out
             if (want_e) {
@@ -672,6 +684,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
         p.maps[(long long)p.out_slot[m] * p.P + pt] = v;
     }
     if (want_e) p.maps[(long long)p.e_slot * p.P + pt] = e_near;
+    if (p.cmap) p.cmap[pt] = ccount;
     if (p.d_slot >= 0) p.maps[(long long)p.d_slot * p.P + pt] = d_acc;
 }
 
@@ -700,6 +713,58 @@ __global__ void k_covalent(CovParams c) {
     if (rcov < 1.0e-6) rcov = 1.0e-6;
     const float v = (float)(c.barrier * (1. - exp(-0.69314718055994530942 * rcov * rcov)));
     for (int k = 0; k < c.n_cov; k++) c.maps[(long long)c.slot[k] * c.P + pt] = v;
+}
+
+// Floating grid (main.cpp:2962-2966): the distance to the nearest receptor
+// atom, each distance clamped at APPROX_ZERO.  One lane owns kFarK points
+// along x; atoms stream through scalar registers.
+struct FminParams {
+    int n1[3], ne[3];
+    float spacing;
+    int runs_per_row, nruns;
+    const float4 *atoms;          // (x, y, z, 0) grid frame, A
+    int n_atoms;
+    float *fmap;
+};
+
+__global__ void __launch_bounds__(256) k_fmin(FminParams p) {
+    const int run = blockIdx.x * 256 + threadIdx.x;
+    if (run >= p.nruns) return;
+    const int row = run / p.runs_per_row;
+    const int ix0 = (run % p.runs_per_row) * kFarK;
+    const int iy = row % p.n1[1];
+    const int iz = row / p.n1[1];
+    const float py = (float)(iy - p.ne[1]) * p.spacing;
+    const float pz = (float)(iz - p.ne[2]) * p.spacing;
+    float px[kFarK], m2[kFarK];
+#pragma unroll
+    for (int k = 0; k < kFarK; k++) {
+        px[k] = (float)(ix0 + k - p.ne[0]) * p.spacing;
+        m2[k] = 3.0e38f;
+    }
+    const float4 *__restrict__ atoms = p.atoms;
+#pragma unroll 2
+    for (int a = 0; a < p.n_atoms; a++) {
+        const float4 at = atoms[a];
+        const float dy = at.y - py, dz = at.z - pz;
+        const float dyz2 = __builtin_fmaf(dy, dy, dz * dz);
+#pragma unroll
+        for (int k = 0; k < kFarK; k++) {
+            const float dx = at.x - px[k];
+            m2[k] = fminf(m2[k], __builtin_fmaf(dx, dx, dyz2));
+        }
+    }
+    const long long base = (long long)ix0 + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
+#pragma unroll
+    for (int k = 0; k < kFarK; k++)
+        if (ix0 + k < p.n1[0])
+            p.fmap[base + k] = p.n_atoms > 0 ? fmaxf(__builtin_amdgcn_sqrtf(m2[k]), 1.0e-6f) : 1.0e12f;
+}
+
+__global__ void k_fmin_atoms(int n, const double *coord, float4 *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = make_float4((float)coord[3 * i], (float)coord[3 * i + 1], (float)coord[3 * i + 2], 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -829,6 +894,9 @@ struct ag_plan {
     int nplain = 12;                   // plain slots in use (4, 8 or 12)
     CovParams cov{};
     bool run_cov = false;
+    bool want_fmap = false, want_cmap = false;
+    DevBuf fmin_atoms;
+    FminParams fmin{};
     FarParams far{};
     bool run_far = false;
     bool far_clamp = true;
@@ -880,8 +948,8 @@ static void validate(const ag_receptor *rec, const ag_grid *grid, const ag_mapse
         if (grid->n1[d] < 1 || grid->n1[d] > AG_MAX_GRID_PTS || grid->ne[d] < 0)
             ag::fail(AG_ERR_BAD_INPUT, "grid dimensions out of range");
     if (!(grid->spacing > 0)) ag::fail(AG_ERR_BAD_INPUT, "grid spacing must be positive");
-    if (ms->floating_grid || ms->constriction_grid)
-        ag::fail(AG_ERR_UNSUPPORTED, "floating and constriction grids are not implemented on the device yet");
+    if (ms->constriction_grid && !(ms->constriction_cutoff <= 8.0))
+        ag::fail(AG_ERR_UNSUPPORTED, "constriction_distance_cutoff above 8 A is not supported on the device");
     const int n_maps = M + (ms->elec_index >= 0) + (ms->dsolv_index >= 0);
     if (ms->n_maps != n_maps) ag::fail(AG_ERR_BAD_INPUT, "n_maps does not match the map set");
     if ((ms->elec_index >= 0 && ms->elec_index >= n_maps) || (ms->dsolv_index >= 0 && ms->dsolv_index >= n_maps))
@@ -1083,6 +1151,26 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
         p.near.push_back(q);
     }
 
+    // floating and constriction grids
+    p.want_fmap = ms->floating_grid != 0;
+    p.want_cmap = ms->constriction_grid != 0;
+    if (p.want_cmap) p.near[0].ccut = ms->constriction_cutoff;
+    if (p.want_fmap) {
+        float4 *d_fa = (float4 *)p.fmin_atoms.get(sizeof(float4) * (size_t)std::max(N, 1));
+        if (N > 0) k_fmin_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_fa);
+        HIP_CHECK(hipGetLastError());
+        FminParams &fm = p.fmin;
+        for (int d = 0; d < 3; d++) {
+            fm.n1[d] = grid->n1[d];
+            fm.ne[d] = grid->ne[d];
+        }
+        fm.spacing = (float)grid->spacing;
+        fm.runs_per_row = (grid->n1[0] + kFarK - 1) / kFarK;
+        fm.nruns = fm.runs_per_row * grid->n1[1] * grid->n1[2];
+        fm.atoms = d_fa;
+        fm.n_atoms = N;
+    }
+
     // covalent maps
     if (!cov_maps.empty()) {
         if (cov_maps.size() > 16) ag::fail(AG_ERR_UNSUPPORTED, "more than 16 covalent maps");
@@ -1154,7 +1242,9 @@ void plan_destroy(ag_plan *p) {
 
 // Cell list + near kernel (+ far kernel) into d_maps ([n_maps][P], device).
 // Asynchronous on the context's stream up to the final status check.
-void plan_execute(ag_plan *p, float *d_maps) {
+void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
+    if ((p->want_fmap && !d_fmap) || (p->want_cmap && !d_cmap))
+        ag::fail(AG_ERR_BAD_INPUT, "the map set asks for a floating/constriction grid but no output was given");
     ag_ctx *c = p->ctx;
     HIP_CHECK(hipSetDevice(c->device));
     hipStream_t st = c->stream;
@@ -1177,13 +1267,22 @@ void plan_execute(ag_plan *p, float *d_maps) {
                                                       (NearAtom *)p->near_atoms.ptr);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(c->ev[1], st));
+    bool first = true;
     for (auto &q : p->near) {
         NearParams np = q;
         np.maps = d_maps;
-        if (!(np.has_affinity || np.e_slot >= 0)) continue;
+        np.cmap = (first && p->want_cmap) ? d_cmap : nullptr;
+        first = false;
+        if (!(np.has_affinity || np.e_slot >= 0 || np.cmap)) continue;
         if (p->nplain == 4) k_near<4><<<np.nbricks, kNearThreads, 0, st>>>(np);
         else if (p->nplain == 8) k_near<8><<<np.nbricks, kNearThreads, 0, st>>>(np);
         else k_near<12><<<np.nbricks, kNearThreads, 0, st>>>(np);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (p->want_fmap) {
+        FminParams fm = p->fmin;
+        fm.fmap = d_fmap;
+        k_fmin<<<(fm.nruns + 255) / 256, 256, 0, st>>>(fm);
         HIP_CHECK(hipGetLastError());
     }
     if (p->run_cov) {
@@ -1214,13 +1313,27 @@ void plan_execute(ag_plan *p, float *d_maps) {
 
 void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,
                 float *maps_out, float *fmap, int32_t *cmap, int outputs_on_device) {
-    if (fmap || cmap) ag::fail(AG_ERR_UNSUPPORTED, "floating and constriction grids are not implemented on the device yet");
+    if ((ms->floating_grid && !fmap) || (ms->constriction_grid && !cmap))
+        ag::fail(AG_ERR_BAD_INPUT, "the map set asks for a floating/constriction grid but no output was given");
     std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab), plan_destroy);
     const size_t bytes = sizeof(float) * (size_t)p->n_maps * p->P;
-    float *d_maps = outputs_on_device ? maps_out : (float *)c->maps.get(bytes);
-    plan_execute(p.get(), d_maps);
+    const size_t pbytes = sizeof(float) * (size_t)p->P;
+    float *d_maps = outputs_on_device ? maps_out : (float *)c->maps.get(bytes + 2 * pbytes + 256);
+    float *d_fmap = nullptr;
+    int32_t *d_cmap = nullptr;
+    if (outputs_on_device) {
+        d_fmap = ms->floating_grid ? fmap : nullptr;
+        d_cmap = ms->constriction_grid ? cmap : nullptr;
+    } else {
+        char *base = (char *)d_maps + bytes;
+        if (ms->floating_grid) d_fmap = (float *)base;
+        if (ms->constriction_grid) d_cmap = (int32_t *)(base + pbytes);
+    }
+    plan_execute(p.get(), d_maps, d_fmap, d_cmap);
     if (!outputs_on_device) {
         HIP_CHECK(hipMemcpyAsync(maps_out, d_maps, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (d_fmap) HIP_CHECK(hipMemcpyAsync(fmap, d_fmap, pbytes, hipMemcpyDeviceToHost, c->stream));
+        if (d_cmap) HIP_CHECK(hipMemcpyAsync(cmap, d_cmap, pbytes, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
     }
 }

@@ -167,7 +167,7 @@ void context_timing(ag_ctx *ctx, float *ms);
 ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                      const ag_tables *tab);
 void plan_destroy(ag_plan *p);
-void plan_execute(ag_plan *p, float *d_maps);
+void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap);
 void maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                 const ag_mapset *ms, const ag_tables *tab, float *maps,
                 float *fmap, int32_t *cmap, int outputs_on_device);

@@ -212,12 +212,12 @@ ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *gri
     return guarded([&] { *out = agdev::plan_create(ctx, rec, grid, ms, tab); });
 }
 
-ag_status ag_plan_execute(ag_plan *plan, float *maps_dev) {
+ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev) {
     if (!plan || !maps_dev) {
         ag::set_last_error("null argument to ag_plan_execute");
         return AG_ERR_BAD_INPUT;
     }
-    return guarded([&] { agdev::plan_execute(plan, maps_dev); });
+    return guarded([&] { agdev::plan_execute(plan, maps_dev, fmap_dev, cmap_dev); });
 }
 
 ag_status ag_plan_destroy(ag_plan *plan) {

@@ -205,7 +205,8 @@ ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path,
  * maps  : [n_maps][P] float; slot order as in the GPF (atom maps, then e and
  *         d at elec_index / dsolv_index).
  * fmap  : [P] float, r_min of the floating grid, or NULL.
- * cmap  : [P] int32, atoms within constriction_cutoff, or NULL.
+ * cmap  : [P] int32, atoms within constriction_cutoff (<= 8 A), or NULL.
+ * fmap / cmap must be given exactly when the map set asks for them.
  * Output pointers are device pointers when `outputs_on_device` is nonzero,
  * else host pointers (the result is copied back before return).
  * Inputs are host pointers; they are staged to the device on ctx's stream. */
@@ -217,12 +218,13 @@ ag_status ag_maps_plain(ag_ctx *ctx, const ag_receptor *rec,
 /* Device-resident form of the same computation: ag_plan_create uploads the
  * receptor, tables and map set to ctx's device once; each ag_plan_execute
  * then builds the cell list and runs the near-field and far-field kernels
- * into maps_dev, a device buffer of [n_maps][P] floats.  A plan belongs to
- * its context and must be destroyed before it. */
+ * into maps_dev, a device buffer of [n_maps][P] floats, and -- when the map
+ * set asks for them -- fmap_dev ([P] float) and cmap_dev ([P] int32), else
+ * NULL.  A plan belongs to its context and must be destroyed before it. */
 typedef struct ag_plan ag_plan;
 ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                          const ag_mapset *mapset, const ag_tables *tables, ag_plan **out);
-ag_status ag_plan_execute(ag_plan *plan, float *maps_dev);
+ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev);
 ag_status ag_plan_destroy(ag_plan *plan);
 
 /* Milliseconds of the last execute on ctx, from HIP events recorded on ctx's

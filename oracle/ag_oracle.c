@@ -513,3 +513,8 @@ int orc_format_value(double e, char *out) {
     }
     return sprintf(out, "%.3f\n", (float)(floor(e * 1000.0 + 0.5) / 1000.0));
 }
+
+/* Floating-grid value (main.cpp:3502-3507): "%.3f" of (float)round3dp(r_min). */
+int orc_format_rmin(double r, char *out) {
+    return sprintf(out, "%.3f\n", (float)(floor(r * 1000.0 + 0.5) / 1000.0));
+}

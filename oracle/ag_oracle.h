@@ -42,6 +42,8 @@ int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
              int z0, int z1);
 /* One map value as the reference writes it (main.cpp:3486-3497). */
 int orc_format_value(double e, char *out);
+/* One floating-grid value (main.cpp:3502-3507). */
+int orc_format_rmin(double r, char *out);
 
 #ifdef __cplusplus
 }

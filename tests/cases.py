@@ -53,6 +53,8 @@ SYN_CASES = {
     "ongrid": dict(atoms=300, radius=14.0, seed=2007, npts=18, lig=["C", "OA", "HD"], center_atom=0),
     "all13": dict(atoms=800, radius=22.0, seed=2008, npts=20,
                   lig=["A", "C", "N", "NA", "OA", "S", "SA", "HD", "F", "Cl", "Br", "I", "P"]),
+    "fmap_cmap": dict(atoms=400, radius=16.0, seed=2010, npts=20, lig=["C", "OA", "HD"],
+                      extra=["constriction_distance_cutoff 6.5", "fmap fmap_cmap.f.map", "cmap fmap_cmap.c.map"]),
     "many_maps": dict(atoms=400, radius=16.0, seed=2009, npts=12,
                       lig=["A", "C", "N", "NA", "OA", "S", "SA", "HD", "F", "Cl", "Br", "I", "P", "H", "NS",
                            "OS", "HS", "Mg", "Zn", "Fe"]),
@@ -77,6 +79,18 @@ def map_names(gpf_path: str) -> list:
         if w and w[0].lower() in ("map", "elecmap", "dsolvmap", "desolvmap"):
             names.append(w[1])
     return names
+
+
+def distance_map_names(gpf_path: str) -> dict:
+    """{"fmap": file, "cmap": file} for the floating / constriction grids of a GPF."""
+    out = {}
+    for line in open(gpf_path):
+        w = line.split("#")[0].split()
+        if w and w[0].lower() == "fmap":
+            out["fmap"] = w[1]
+        elif w and w[0].lower() in ("cmap", "constriction_map"):
+            out["cmap"] = w[1]
+    return out
 
 
 def materialize(name: str, dest: str) -> str:

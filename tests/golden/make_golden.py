@@ -59,6 +59,9 @@ def main():
                     maps[m] = {"sha256": h, "n": n,
                                "header": open(os.path.join(d, m)).read().split("\n")[:6]}
                 rec["maps"] = maps
+                for kind, fn in cases.distance_map_names(os.path.join(d, gpf)).items():
+                    h, n = value_hash(os.path.join(d, fn))
+                    rec[kind] = {"file": fn, "sha256": h, "n": n}
                 fld = [ln for ln in open(os.path.join(d, f"{name.split('_CASE')[0] if name.startswith('x1hpv') else name}.maps.fld")).read().split("\n")
                        if not ln.startswith("# Created by")] if name in cases.SYN_CASES else None
                 rec["fld"] = fld

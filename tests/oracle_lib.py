@@ -41,6 +41,7 @@ def olib() -> C.CDLL:
         L.orc_maps.argtypes = [C.POINTER(_abi.Receptor), C.POINTER(_abi.Grid), C.POINTER(_abi.MapSet),
                                C.POINTER(_abi.Tables), vp, vp, vp, C.c_int, C.c_int]
         L.orc_format_value.argtypes = [C.c_double, C.c_char_p]
+        L.orc_format_rmin.argtypes = [C.c_double, C.c_char_p]
         _lib = L
     return _lib
 
@@ -110,3 +111,16 @@ def format_value(e: float) -> bytes:
 
 def format_map(values) -> bytes:
     return b"".join(format_value(float(v)) for v in np.asarray(values).reshape(-1))
+
+
+def format_rmin_map(values) -> bytes:
+    buf = C.create_string_buffer(64)
+    out = []
+    for v in np.asarray(values).reshape(-1):
+        n = olib().orc_format_rmin(float(v), buf)
+        out.append(buf.raw[:n])
+    return b"".join(out)
+
+
+def format_count_map(values) -> bytes:
+    return b"".join(b"%d\n" % int(v) for v in np.asarray(values).reshape(-1))

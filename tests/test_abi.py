@@ -46,5 +46,5 @@ def test_context_on_a_host_without_gpu_fails_cleanly():
 def test_null_arguments_are_rejected():
     L = _abi.lib()
     assert L.ag_maps_plain(None, None, None, None, None, None, None, None, 0) == 1
-    assert L.ag_plan_execute(None, None) == 1
+    assert L.ag_plan_execute(None, None, None, None) == 1
     assert L.ag_job_open(None, None, None, 0, None) == 1

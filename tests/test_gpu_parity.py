@@ -36,9 +36,17 @@ def ctx():
 def test_maps_match_oracle(ctx, name, tmp_path):
     print(f"[gpu] case {name}", flush=True)
     st, _ = parity.load_case(name, str(tmp_path))
-    ref = ol.maps(ol.oracle_state(st))
+    want_f, want_c = bool(st.floating_grid), bool(st.constriction_grid)
+    if want_f or want_c:
+        ref, rfm, rcm = ol.maps(ol.oracle_state(st), fmap=want_f, cmap=want_c)
+    else:
+        ref = ol.maps(ol.oracle_state(st))
     got = ctx.maps_plain(st)
     assert got.shape == ref.shape
+    if want_f:   # distance to the nearest atom, A
+        assert np.abs(ctx.last_fmap.astype(np.float64) - rfm).max() <= 1e-4
+    if want_c:   # atom counts are exact
+        assert np.array_equal(ctx.last_cmap, rcm)
     report = []
     for k in range(st.n_maps):
         ok, mx, rms, bad = parity.within(got[k], ref[k])
@@ -84,7 +92,7 @@ def test_device_outputs(ctx, tmp_path):
     assert np.array_equal(back, host)
 
 
-@pytest.mark.parametrize("name", ["hsg1_sm", "x1hpv_CASE0", "covalent", "sepdesolv"])
+@pytest.mark.parametrize("name", ["hsg1_sm", "x1hpv_CASE0", "covalent", "sepdesolv", "fmap_cmap"])
 def test_cli_dropin(name, tmp_path):
     """autogrid4 drop-in: exit 0, completion line, reference headers, values."""
     print(f"[gpu] cli {name}", flush=True)
@@ -109,3 +117,10 @@ def test_cli_dropin(name, tmp_path):
     if g.get("fld"):
         fld = [ln for ln in open(tmp_path / f"{name}.maps.fld").read().split("\n") if not ln.startswith("# Created by")]
         assert fld == g["fld"]
+    if "cmap" in g:   # integer counts: byte-identical text
+        body = open(tmp_path / g["cmap"]["file"], "rb").read().split(b"\n", 6)[6]
+        import hashlib
+        assert hashlib.sha256(body).hexdigest() == g["cmap"]["sha256"]
+    if "fmap" in g:
+        vals = np.array([float(x) for x in open(tmp_path / g["fmap"]["file"]).read().split("\n")[6:] if x])
+        assert vals.size == g["fmap"]["n"]

@@ -31,10 +31,18 @@ SUCCESS = [n for n in cases.ALL_CASES if GOLDEN[n]["success"]]
 @pytest.mark.parametrize("name", SUCCESS)
 def test_oracle_text_equals_reference(name, tmp_path):
     st, gpf = parity.load_case(name, str(tmp_path))
-    m = ol.maps(ol.oracle_state(st))
+    g = GOLDEN[name]
+    if st.floating_grid or st.constriction_grid:
+        m, fm, cm = ol.maps(ol.oracle_state(st), fmap=bool(st.floating_grid), cmap=bool(st.constriction_grid))
+        if st.floating_grid:
+            assert hashlib.sha256(ol.format_rmin_map(fm)).hexdigest() == g["fmap"]["sha256"]
+        if st.constriction_grid:
+            assert hashlib.sha256(ol.format_count_map(cm)).hexdigest() == g["cmap"]["sha256"]
+    else:
+        m = ol.maps(ol.oracle_state(st))
     for k, fn in enumerate(cases.map_names(os.path.join(tmp_path, gpf))):
         text = ol.format_map(m[k])
-        assert hashlib.sha256(text).hexdigest() == GOLDEN[name]["maps"][fn]["sha256"], fn
+        assert hashlib.sha256(text).hexdigest() == g["maps"][fn]["sha256"], fn
 
 
 @pytest.mark.skipif(not os.path.exists(ol.REF_BIN), reason="reference autogrid4 not built here")

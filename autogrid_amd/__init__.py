@@ -299,6 +299,13 @@ class Plan:
         check(lib().ag_plan_execute(self._h, C.c_void_p(maps_dev_ptr), C.c_void_p(fmap_dev_ptr),
                                     C.c_void_p(cmap_dev_ptr)))
 
+    def audit(self) -> dict:
+        """Table-bin audit against the reference's FP64 distances (ag_plan_audit)."""
+        v = (C.c_int64 * 4)()
+        check(lib().ag_plan_audit(self._h, C.byref(v)))
+        return {"far_pairs": v[0], "far_bin_flips": v[1], "near_unguarded_flips": v[2],
+                "max_flips_per_point": v[3]}
+
     def close(self):
         if getattr(self, "_h", None):
             lib().ag_plan_destroy(self._h)

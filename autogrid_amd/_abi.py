@@ -115,8 +115,9 @@ def lib() -> C.CDLL:
                                      C.POINTER(Tables), C.POINTER(vp)]
         L.ag_plan_execute.argtypes = [vp, vp, vp, vp]
         L.ag_plan_destroy.argtypes = [vp]
+        L.ag_plan_audit.argtypes = [vp, C.POINTER(C.c_int64 * 4)]
         L.ag_context_timing.argtypes = [vp, C.POINTER(C.c_float)]
-        for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_context_timing",
+        for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_plan_audit", "ag_context_timing",
                      "ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",
                      "ag_job_views", "ag_run_gpf", "ag_maps_plain", "ag_job_write_maps"):
             getattr(L, name).restype = C.c_int

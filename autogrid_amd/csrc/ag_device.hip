@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -767,6 +768,73 @@ __global__ void k_fmin_atoms(int n, const double *coord, float4 *out) {
     out[i] = make_float4((float)coord[3 * i], (float)coord[3 * i + 1], (float)coord[3 * i + 2], 0.f);
 }
 
+// Bin audit (diagnostic, not part of a map set): for every (point, atom)
+// pair compare the bin floor(128 r) the device path uses with the one the
+// reference's FP64 distance gives.
+//   far pairs  (the far kernel's FP32 bin >= 1025): the FP32 bin is used
+//              as is, so every mismatch is a table-bin flip;
+//   near pairs: the near kernel recomputes in FP64 whenever 128 r is within
+//              kGuardBins of an integer, so a mismatch outside that band
+//              would be an unguarded flip (expected: none).
+struct AuditParams {
+    int n1[3], ne[3];
+    double spacing;
+    float s128;
+    const float4 *far_atoms;     // the far kernel's atoms
+    const double *coord;         // [n][3]
+    int n_atoms;
+    long long P;
+    unsigned long long *out;     // [0] far pairs, [1] far flips, [2] near unguarded flips, [3] max flips/point
+};
+
+__global__ void __launch_bounds__(256) k_audit(AuditParams p) {
+    const long long pt = (long long)blockIdx.x * 256 + threadIdx.x;
+    unsigned long long far_pairs = 0, far_flips = 0, near_bad = 0;
+    if (pt < p.P) {
+        const int ix = (int)(pt % p.n1[0]);
+        const int iy = (int)((pt / p.n1[0]) % p.n1[1]);
+        const int iz = (int)(pt / ((long long)p.n1[0] * p.n1[1]));
+        const double cx = (double)(ix - p.ne[0]) * p.spacing, cy = (double)(iy - p.ne[1]) * p.spacing,
+                     cz = (double)(iz - p.ne[2]) * p.spacing;
+        const float pxE = (float)(ix - p.ne[0]) * p.s128, pyE = (float)(iy - p.ne[1]) * p.s128,
+                    pzE = (float)(iz - p.ne[2]) * p.s128;
+        for (int a = 0; a < p.n_atoms; a++) {
+            const float4 at = p.far_atoms[a];
+            const float rb = far_rb(at.x, at.y, at.z, pxE, pyE, pzE);
+            double dx, dy, dz;
+            const double r64 = ref_dist(p.coord[3 * a], p.coord[3 * a + 1], p.coord[3 * a + 2], cx, cy, cz, dx, dy, dz);
+            const int b64 = min((int)(r64 * 128.0), AG_NDIEL - 1);
+            if (rb >= (float)kFarBinMin) {
+                far_pairs++;
+                far_flips += min((int)rb, AG_NDIEL - 1) != b64;
+            } else {
+                const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+                const float r2 = fmaxf(__builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf)), 1.0e-12f);
+                const float g = r2 * __builtin_amdgcn_rsqf(r2) * 128.f;
+                if (fabsf(g - rintf(g)) >= kGuardBins && min((int)g, AG_NDIEL - 1) != b64) near_bad++;
+            }
+        }
+    }
+    // wave sums, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        far_pairs += __shfl_xor(far_pairs, off);
+        near_bad += __shfl_xor(near_bad, off);
+    }
+    unsigned long long mx = far_flips;
+    unsigned long long sum_f = far_flips;
+    for (int off = 32; off > 0; off >>= 1) {
+        sum_f += __shfl_xor(sum_f, off);
+        const unsigned long long o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&p.out[0], far_pairs);
+        atomicAdd(&p.out[1], sum_f);
+        atomicAdd(&p.out[2], near_bad);
+        atomicMax(&p.out[3], mx);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // far kernel
 // ---------------------------------------------------------------------------
@@ -895,6 +963,9 @@ struct ag_plan {
     CovParams cov{};
     bool run_cov = false;
     bool want_fmap = false, want_cmap = false;
+    ag_grid grid{};
+    double coeff_estat = 0.0;
+    bool run_far_atoms_ready = false;
     DevBuf fmin_atoms;
     FminParams fmin{};
     FarParams far{};
@@ -998,6 +1069,8 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     const int T = p.T = tab->n_rec_types;
     p.n_maps = ms->n_maps;
     p.elec_index = ms->elec_index;
+    p.grid = *grid;
+    p.coeff_estat = ms->coeff_estat;
     const long long P = p.P = (long long)grid->n1[0] * grid->n1[1] * grid->n1[2];
     hipStream_t st = c->stream;
 
@@ -1196,6 +1269,7 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
         float4 *d_far = (float4 *)p.far_atoms.get(sizeof(float4) * (size_t)N);
         k_far_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_charge, ms->coeff_estat, d_far);
         HIP_CHECK(hipGetLastError());
+        p.run_far_atoms_ready = true;
         // largest bin any (point, atom) pair reaches: farthest box corner
         double rmax = 0.0;
         for (int i = 0; i < N; i++) {
@@ -1309,6 +1383,74 @@ void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
     HIP_CHECK(hipEventElapsedTime(&c->last_ms[2], c->ev[2], c->ev[3]));
     HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
     if (status & 1) ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
+}
+
+// One map set, handed to the host map by map: each map is copied into host
+// memory as soon as the kernels are done and on_ready(j) is called for it
+// (j = n_maps for fmap, n_maps + 1 for cmap), so the caller can format and
+// write map j while later maps are still in flight.
+void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                         const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap,
+                         const std::function<void(int)> &on_ready) {
+    std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab), plan_destroy);
+    const size_t pbytes = sizeof(float) * (size_t)p->P;
+    const size_t bytes = pbytes * (size_t)p->n_maps;
+    float *d_maps = (float *)c->maps.get(bytes + 2 * pbytes + 256);
+    char *extra = (char *)d_maps + bytes;
+    float *d_fmap = ms->floating_grid ? (float *)extra : nullptr;
+    int32_t *d_cmap = ms->constriction_grid ? (int32_t *)(extra + pbytes) : nullptr;
+    plan_execute(p.get(), d_maps, d_fmap, d_cmap);
+    for (int j = 0; j < p->n_maps; j++) {
+        HIP_CHECK(hipMemcpyAsync(maps_out + (size_t)j * p->P, d_maps + (size_t)j * p->P, pbytes,
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        on_ready(j);
+    }
+    if (d_fmap) {
+        HIP_CHECK(hipMemcpyAsync(fmap, d_fmap, pbytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        on_ready(p->n_maps);
+    }
+    if (d_cmap) {
+        HIP_CHECK(hipMemcpyAsync(cmap, d_cmap, pbytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        on_ready(p->n_maps + 1);
+    }
+}
+
+void plan_audit(ag_plan *p, long long out[4]) {
+    ag_ctx *c = p->ctx;
+    HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    if (!p->run_far_atoms_ready) {
+        float4 *d_far = (float4 *)p->far_atoms.get(sizeof(float4) * (size_t)std::max(p->N, 1));
+        if (p->N > 0)
+            k_far_atoms<<<(p->N + 255) / 256, 256, 0, st>>>(p->N, (const double *)p->coord.ptr,
+                                                            (const double *)p->charge.ptr, p->coeff_estat, d_far);
+        p->run_far_atoms_ready = true;
+    }
+    DevBuf acc;
+    unsigned long long *d_out = (unsigned long long *)acc.get(4 * sizeof(unsigned long long));
+    HIP_CHECK(hipMemsetAsync(d_out, 0, 4 * sizeof(unsigned long long), st));
+    AuditParams ap;
+    memset(&ap, 0, sizeof ap);
+    for (int d = 0; d < 3; d++) {
+        ap.n1[d] = p->grid.n1[d];
+        ap.ne[d] = p->grid.ne[d];
+    }
+    ap.spacing = p->grid.spacing;
+    ap.s128 = (float)(p->grid.spacing * 128.0);
+    ap.far_atoms = (const float4 *)p->far_atoms.ptr;
+    ap.coord = (const double *)p->coord.ptr;
+    ap.n_atoms = p->N;
+    ap.P = p->P;
+    ap.out = d_out;
+    k_audit<<<(unsigned)((p->P + 255) / 256), 256, 0, st>>>(ap);
+    HIP_CHECK(hipGetLastError());
+    unsigned long long h[4];
+    HIP_CHECK(hipMemcpyAsync(h, d_out, sizeof h, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < 4; i++) out[i] = (long long)h[i];
 }
 
 void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,

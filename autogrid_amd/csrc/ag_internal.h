@@ -3,6 +3,7 @@
 
 #include <cstdio>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 #include <stdexcept>
@@ -152,8 +153,16 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord,
                         uint8_t *disorder, FILE *log);
 // writer
 int format_map_value(float v, char *out);     // returns bytes incl. '\n'
-void write_all_maps(Job &job, const float *maps, const float *fmap,
-                    const int32_t *cmap, int n_threads);
+// Formats and writes every map file; wait_ready(j), when given, blocks until
+// map j (n_maps: fmap, n_maps + 1: cmap) is on the host.
+struct MapExtrema {
+    std::vector<double> emin, emax;
+};
+MapExtrema write_all_maps(Job &job, const float *maps, const float *fmap,
+                          const int32_t *cmap, int n_threads,
+                          const std::function<void(int)> &wait_ready = nullptr);
+void log_map_summary(Job &job, const MapExtrema &ext);
+void log_plane_progress(Job &job, double seconds);
 
 }  // namespace ag
 
@@ -168,6 +177,10 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
                      const ag_tables *tab);
 void plan_destroy(ag_plan *p);
 void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap);
+void plan_audit(ag_plan *p, long long out[4]);
+void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                         const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap,
+                         const std::function<void(int)> &on_ready);
 void maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                 const ag_mapset *ms, const ag_tables *tab, float *maps,
                 float *fmap, int32_t *cmap, int outputs_on_device);

@@ -3,7 +3,10 @@
 // Every entry point catches the library's exceptions and turns them into an
 // ag_status plus a thread-local message; none of them calls exit().
 
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
 #include <thread>
 
 #include "ag_internal.h"
@@ -220,6 +223,21 @@ ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32
     return guarded([&] { agdev::plan_execute(plan, maps_dev, fmap_dev, cmap_dev); });
 }
 
+ag_status ag_plan_audit(ag_plan *plan, ag_audit *out) {
+    if (!plan || !out) {
+        ag::set_last_error("null argument to ag_plan_audit");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        long long v[4];
+        agdev::plan_audit(plan, v);
+        out->far_pairs = v[0];
+        out->far_bin_flips = v[1];
+        out->near_unguarded_flips = v[2];
+        out->max_flips_per_point = v[3];
+    });
+}
+
 ag_status ag_plan_destroy(ag_plan *plan) {
     return guarded([&] { agdev::plan_destroy(plan); });
 }
@@ -251,7 +269,7 @@ ag_status ag_job_write_maps(ag_job *j, const float *maps, const float *fmap, con
         return AG_ERR_BAD_INPUT;
     }
     ag_status st = guarded([&] {
-        ag::write_all_maps(j->job, maps, fmap, cmap, n_threads > 0 ? n_threads : 1);
+        ag::log_map_summary(j->job, ag::write_all_maps(j->job, maps, fmap, cmap, n_threads > 0 ? n_threads : 1));
         j->job.logf("\n%s: Successful Completion.\n", j->job.program.c_str());
         if (j->job.log) fflush(j->job.log);
     });
@@ -289,10 +307,57 @@ ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path, co
             ctx = own;
         }
         struct Own { ag_ctx *c; ~Own() { if (c) agdev::context_destroy(c); } } own_guard{own};
-        agdev::maps_plain(ctx, &rec, &grid, &ms, &tab, maps.data(), job.floating_grid ? fmap.data() : nullptr,
-                          job.constriction_grid ? cmap.data() : nullptr, 0);
+        // The device computes the map set; a copy thread brings the maps to
+        // the host one by one while writer threads format and write those
+        // already there.
+        const int nready = job.num_maps + 2;
+        std::vector<char> ready(nready, 0);
+        std::mutex mu;
+        std::condition_variable cv;
+        std::string copy_error;
+        ag_status copy_status = AG_OK;
+        auto t0 = std::chrono::steady_clock::now();
+        std::thread copier([&] {
+            auto mark = [&](int k) {
+                std::lock_guard<std::mutex> lk(mu);
+                ready[k] = 1;
+                cv.notify_all();
+            };
+            try {
+                agdev::maps_plain_streamed(ctx, &rec, &grid, &ms, &tab, maps.data(),
+                                           job.floating_grid ? fmap.data() : nullptr,
+                                           job.constriction_grid ? cmap.data() : nullptr, mark);
+            } catch (const ag::Error &e) {
+                copy_status = e.status;
+                copy_error = e.what();
+            } catch (const std::exception &e) {
+                copy_status = AG_ERR_DEVICE;
+                copy_error = e.what();
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto &r : ready) r = r ? r : 2;   // 2: never arrives
+            cv.notify_all();
+        });
+        auto wait_ready = [&](int k) {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return ready[k] != 0; });
+            if (ready[k] == 2) ag::fail(copy_status != AG_OK ? copy_status : AG_ERR_DEVICE, copy_error);
+        };
         unsigned hw = std::thread::hardware_concurrency();
-        ag::write_all_maps(job, maps.data(), fmap.data(), cmap.data(), (int)std::min(16u, hw ? hw : 1u));
+        ag::MapExtrema ext;
+        try {
+            ext = ag::write_all_maps(job, maps.data(), fmap.data(), cmap.data(), (int)std::min(16u, hw ? hw : 1u),
+                                     wait_ready);
+        } catch (...) {
+            copier.join();
+            if (copy_status != AG_OK) ag::fail(copy_status, copy_error);
+            throw;
+        }
+        copier.join();
+        if (copy_status != AG_OK) ag::fail(copy_status, copy_error);
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        ag::log_plane_progress(job, secs);
+        ag::log_map_summary(job, ext);
         job.logf("\n%s: Successful Completion.\n", job.program.c_str());
         if (job.log) fflush(job.log);
     });

@@ -1,9 +1,10 @@
 // Map-file writer: the reference's value text (main.cpp:3481-3519), the
 // extrema summary (main.cpp:3530-3547) and the completion line
-// (main.cpp:3565).  Values are formatted on host threads in slices and each
-// map file is written in one pass.
+// (main.cpp:3565).  Each map file is formatted and written by one host
+// thread, as the reference does (main.cpp:3481).
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -63,62 +64,66 @@ static int format_rmin(float r, char *out) {
     return format_fixed3(v, out);
 }
 
-template <typename F>
-static std::string format_parallel(int64_t n, int n_threads, F &&one) {
-    const int nt = std::max(1, std::min<int>(n_threads, (int)std::max<int64_t>(1, n / 65536)));
-    std::vector<std::string> parts(nt);
-    std::vector<std::thread> th;
-    auto work = [&](int t) {
-        const int64_t b = n * t / nt, e = n * (t + 1) / nt;
-        std::string &s = parts[t];
-        s.resize((size_t)(e - b) * 16 + 64);
-        char *p = &s[0];
-        for (int64_t i = b; i < e; i++) p += one(i, p);
-        s.resize((size_t)(p - &s[0]));
+MapExtrema write_all_maps(Job &job, const float *maps, const float *fmap, const int32_t *cmap, int n_threads,
+                          const std::function<void(int)> &wait_ready) {
+    const int64_t P = job.points();
+    const int nm = job.num_maps;
+    MapExtrema ext;
+    ext.emin.assign(nm, kBig);
+    ext.emax.assign(nm, -kBig);
+    std::vector<double> &emin = ext.emin, &emax = ext.emax;
+    // one task per output file, as the reference writes one file per thread
+    // (main.cpp:3481); a task waits for its map to reach the host
+    std::vector<int> tasks;
+    for (int j = 0; j < nm; j++) tasks.push_back(j);
+    if (job.floating_grid && fmap && job.fmap_fp) tasks.push_back(nm);
+    if (job.constriction_grid && cmap && job.cmap_fp) tasks.push_back(nm + 1);
+    std::atomic<int> next{0};
+    std::atomic<bool> problem{false};
+    auto worker = [&]() {
+        std::string text;
+        for (;;) {
+            const int t = next.fetch_add(1);
+            if (t >= (int)tasks.size()) return;
+            const int j = tasks[t];
+            if (wait_ready) wait_ready(j);
+            text.resize((size_t)P * 16 + 64);
+            char *o = &text[0];
+            FILE *f = nullptr;
+            if (j < nm) {
+                const float *v = maps + (size_t)j * P;
+                double lo = kBig, hi = -kBig;
+                for (int64_t i = 0; i < P; i++) {
+                    o += format_map_value(v[i], o);
+                    lo = std::min(lo, (double)v[i]);
+                    hi = std::max(hi, (double)v[i]);
+                }
+                emin[j] = lo;
+                emax[j] = hi;
+                f = job.maps[j].fp;
+            } else if (j == nm) {
+                for (int64_t i = 0; i < P; i++) o += format_rmin(fmap[i], o);
+                f = job.fmap_fp;
+            } else {
+                for (int64_t i = 0; i < P; i++) o += snprintf(o, 16, "%d\n", (int)cmap[i]);
+                f = job.cmap_fp;
+            }
+            const size_t n = (size_t)(o - &text[0]);
+            if (f && (fwrite(text.data(), 1, n, f) != n || fflush(f) != 0)) problem = true;
+        }
     };
-    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
-    work(0);
+    const int nt = std::max(1, std::min(n_threads, (int)tasks.size()));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(worker);
+    worker();
     for (auto &x : th) x.join();
-    size_t tot = 0;
-    for (auto &s : parts) tot += s.size();
-    std::string out;
-    out.reserve(tot);
-    for (auto &s : parts) out += s;
-    return out;
+    if (problem) fail(AG_ERR_IO, "Problems writing grid maps - there may not be enough disk space.\n");
+    return ext;
 }
 
-void write_all_maps(Job &job, const float *maps, const float *fmap, const int32_t *cmap, int n_threads) {
-    const int64_t P = job.points();
-    bool problem = false;
-    std::vector<double> emin(job.num_maps, kBig), emax(job.num_maps, -kBig);
-    for (int j = 0; j < job.num_maps; j++) {
-        const float *v = maps + (size_t)j * P;
-        std::string text =
-            format_parallel(P, n_threads, [v](int64_t i, char *o) { return format_map_value(v[i], o); });
-        if (job.maps[j].fp && fwrite(text.data(), 1, text.size(), job.maps[j].fp) != text.size())
-            problem = true;
-        double lo = kBig, hi = -kBig;
-        for (int64_t i = 0; i < P; i++) {
-            lo = std::min(lo, (double)v[i]);
-            hi = std::max(hi, (double)v[i]);
-        }
-        emin[j] = lo;
-        emax[j] = hi;
-    }
-    if (job.floating_grid && fmap && job.fmap_fp) {
-        std::string text = format_parallel(P, n_threads, [fmap](int64_t i, char *o) { return format_rmin(fmap[i], o); });
-        if (fwrite(text.data(), 1, text.size(), job.fmap_fp) != text.size()) problem = true;
-    }
-    if (job.constriction_grid && cmap && job.cmap_fp) {
-        std::string text = format_parallel(P, n_threads, [cmap](int64_t i, char *o) {
-            return snprintf(o, 16, "%d\n", (int)cmap[i]);
-        });
-        if (fwrite(text.data(), 1, text.size(), job.cmap_fp) != text.size()) problem = true;
-    }
-    for (int j = 0; j < job.num_maps; j++)
-        if (job.maps[j].fp && fflush(job.maps[j].fp) != 0) problem = true;
-    if (problem) fail(AG_ERR_IO, "Problems writing grid maps - there may not be enough disk space.\n");
-
+// Extrema table of main.cpp:3530-3547.
+void log_map_summary(Job &job, const MapExtrema &ext) {
+    const std::vector<double> &emin = ext.emin, &emax = ext.emax;
     job.logf("\nGrid\tAtom\tMinimum   \tMaximum\n");
     job.logf("Map \tType\tEnergy    \tEnergy \n");
     job.logf("\t\t(kcal/mol)\t(kcal/mol)\n");
@@ -132,6 +137,24 @@ void write_all_maps(Job &job, const float *maps, const float *fmap, const int32_
         job.logf(" %d\t %c\t  %6.2lf\t%9.2le\tDesolvation Potential\n", job.dsolvPE + 1, 'd',
                  emin[job.dsolvPE], emax[job.dsolvPE]);
     job.logf("\n\n * Note:  Every pairwise-atomic interaction was clamped at %.2f\n\n", kEintClamp);
+}
+
+// The per-plane progress lines ADT parses from the log (main.cpp:2772-2776,
+// 3436-3462).  The planes are computed together on the device, so every
+// line reports the whole computation's time and nothing remaining.
+void log_plane_progress(Job &job, double seconds) {
+    if (job.outlev < 1) return;
+    job.logf("                    Percent   Estimated Time  Time/this plane\n");
+    job.logf("XY-plane  Z-coord   Done      Remaining       Real, User, System\n");
+    job.logf("            /Ang              /sec            /sec\n");
+    job.logf("________  ________  ________  ______________  __________________________\n\n");
+    const double zmin = job.center[2] - job.spacing * job.ne[2];
+    for (int done = 1; done <= job.n1[2]; done++) {
+        job.logf(" %6d   %8.3lf   %5.1lf%%   ", done - job.ne[2] - 1, zmin + (double)(done - 1) * job.spacing,
+                 100. * done / (double)job.n1[2]);
+        job.logf("        %5.2fs", 0.0f);
+        job.logf("  Real= %.2f,  CPU= %.2f,  System= %.2f\n", seconds / job.n1[2], 0.0, 0.0);
+    }
 }
 
 }  // namespace ag

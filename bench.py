@@ -134,6 +134,7 @@ def main():
         P, N = st.points, st.n_atoms
         flops = FLOPS_PER_PAIR_E * P * N
         achieved = flops / (far_avg * 1e-3) / 1e12
+        audit = plan.audit()          # table-bin flips against FP64 distances (outside the timed region)
         hip.hipFree(dptr)
         plan.close()
         ctx.close()
@@ -170,6 +171,7 @@ def main():
         "kernels_ms": {"cell_list": sum(cell_ms) / len(cell_ms), "near": sum(near_ms) / len(near_ms),
                        "far": far_avg},
         "pair_evals_per_s": P * N / (far_avg * 1e-3),
+        "bin_audit": audit,
         "wall_with_ascii_output_s": wall_cli if cli_ok else None,
         "cpu_baseline": base,
     }

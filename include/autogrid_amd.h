@@ -227,6 +227,23 @@ ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *gri
 ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev);
 ag_status ag_plan_destroy(ag_plan *plan);
 
+/* Table-bin audit of a plan (a diagnostic, outside the map computation):
+ * compares, for every (point, atom) pair, the bin floor(128 r) the device
+ * path uses with the one the reference's FP64 distance gives.
+ *   far_pairs            pairs handled by the far-field kernel (r > 8 A)
+ *   far_bin_flips        of those, pairs whose FP32 bin differs from FP64
+ *   near_unguarded_flips near pairs whose FP32 bin differs from FP64 outside
+ *                        the FP64 recompute band (0 unless the band is too
+ *                        narrow)
+ *   max_flips_per_point  largest number of far flips at one grid point */
+typedef struct ag_audit {
+    int64_t far_pairs;
+    int64_t far_bin_flips;
+    int64_t near_unguarded_flips;
+    int64_t max_flips_per_point;
+} ag_audit;
+ag_status ag_plan_audit(ag_plan *plan, ag_audit *out);
+
 /* Milliseconds of the last execute on ctx, from HIP events recorded on ctx's
  * stream: [0] cell list, [1] near-field kernel(s), [2] far-field kernel,
  * [3] total. */

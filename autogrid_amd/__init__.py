@@ -24,10 +24,13 @@ import numpy as np
 from . import _abi
 from ._abi import AgError, check, lib
 
-__all__ = ["AgError", "Context", "Job", "Plan", "PreparedState", "run_gpf", "format_map_values",
+__all__ = ["AgError", "Context", "Job", "Plan", "PreparedState", "run_gpf", "format_map_values", "xs_radius",
+           "vina_tables", "AG_POTENTIAL_AD4", "AG_POTENTIAL_VINA",
            "LIB_PATH", "CLI_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
+AG_POTENTIAL_AD4 = _abi.AG_POTENTIAL_AD4
+AG_POTENTIAL_VINA = _abi.AG_POTENTIAL_VINA
 CLI_PATH = _abi.CLI_PATH
 
 
@@ -179,6 +182,12 @@ class Job:
         self.prepared = True
         return self
 
+    def prepare_vina(self) -> "Job":
+        """prepare(), with the job's tables replaced by Vina-potential tables."""
+        check(lib().ag_job_prepare_vina(self._h))
+        self.prepared = True
+        return self
+
     @property
     def points(self) -> int:
         return int(lib().ag_job_points(self._h))
@@ -246,20 +255,26 @@ class Context:
     def handle(self):
         return self._h
 
-    def maps_plain(self, st: PreparedState, out: np.ndarray | None = None) -> np.ndarray:
+    def maps_plain(self, st: PreparedState, out: np.ndarray | None = None,
+                   potential: int = _abi.AG_POTENTIAL_AD4) -> np.ndarray:
         """[n_maps, n1z, n1y, n1x] float32 maps for the state, computed on the GPU."""
         rec, grid, ms, tab, _ = st.structs()
+        entry = lib().ag_maps_vina if potential == _abi.AG_POTENTIAL_VINA else lib().ag_maps_plain
         nz, ny, nx = st.n1[2], st.n1[1], st.n1[0]
         if out is None:
             out = np.empty((st.n_maps, nz, ny, nx), np.float32)
         assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == st.n_maps * st.points
         fmap = np.empty((nz, ny, nx), np.float32) if st.floating_grid else None
         cmap = np.empty((nz, ny, nx), np.int32) if st.constriction_grid else None
-        check(lib().ag_maps_plain(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
-                                  out.ctypes.data, fmap.ctypes.data if fmap is not None else None,
-                                  cmap.ctypes.data if cmap is not None else None, 0))
+        check(entry(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
+                    out.ctypes.data, fmap.ctypes.data if fmap is not None else None,
+                    cmap.ctypes.data if cmap is not None else None, 0))
         self.last_fmap, self.last_cmap = fmap, cmap
         return out
+
+    def maps_vina(self, st: PreparedState, out: np.ndarray | None = None) -> np.ndarray:
+        """Vina-potential map set (ag_maps_vina); st.e_vdw_hb must hold Vina tables."""
+        return self.maps_plain(st, out, potential=_abi.AG_POTENTIAL_VINA)
 
     def maps_plain_device(self, st: PreparedState, out_ptr: int) -> None:
         """Same, writing into device memory at out_ptr ([n_maps][P] float32)."""
@@ -267,9 +282,9 @@ class Context:
         check(lib().ag_maps_plain(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
                                   C.c_void_p(out_ptr), None, None, 1))
 
-    def plan(self, st: PreparedState) -> "Plan":
-        """Upload the state to this context's device (ag_plan_create)."""
-        return Plan(self, st)
+    def plan(self, st: PreparedState, potential: int = _abi.AG_POTENTIAL_AD4) -> "Plan":
+        """Upload the state to this context's device (ag_plan_create_potential)."""
+        return Plan(self, st, potential)
 
     def timing(self) -> dict:
         """HIP-event milliseconds of the last plan execution on this context."""
@@ -285,10 +300,11 @@ class Context:
 class Plan:
     """A receptor, grid, map set and tables resident on one device."""
 
-    def __init__(self, ctx: Context, st: PreparedState):
+    def __init__(self, ctx: Context, st: PreparedState, potential: int = _abi.AG_POTENTIAL_AD4):
         rec, grid, ms, tab, _ = st.structs()
         h = C.c_void_p()
-        check(lib().ag_plan_create(ctx.handle, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab), C.byref(h)))
+        check(lib().ag_plan_create_potential(ctx.handle, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
+                                             int(potential), C.byref(h)))
         self._h = h
         self.ctx = ctx
         self.n_maps = st.n_maps
@@ -324,6 +340,24 @@ def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program
     """autogrid4 -p gpf -l glg, on one GPU."""
     with Context(device) as ctx:
         ctx.run_gpf(gpf_path, glg_path, program)
+
+
+def xs_radius(type_name: str) -> tuple:
+    """(radius, listed) of a type in the xs_vdw table (ag_xs_radius)."""
+    r = C.c_double()
+    found = lib().ag_xs_radius(type_name.encode(), C.byref(r))
+    return float(r.value), bool(found)
+
+
+def vina_tables(st: PreparedState, lig_types: list, rec_types: list) -> np.ndarray:
+    """[T, M, 1024] float32 Vina tables for a state's pairs (ag_vina_tables)."""
+    _, _, ms, _, setup = st.structs()
+    xs_map = np.array([xs_radius(t)[0] for t in lig_types], np.float64)
+    xs_rec = np.array([xs_radius(t)[0] for t in rec_types], np.float64)
+    out = np.zeros((len(rec_types), st.n_atom_maps, _abi.AG_NEINT), np.float32)
+    check(lib().ag_vina_tables(C.byref(setup), C.byref(ms), xs_map.ctypes.data, xs_rec.ctypes.data,
+                               out.ctypes.data))
+    return out
 
 
 def format_map_values(values: np.ndarray) -> bytes:

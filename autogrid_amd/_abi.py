@@ -21,6 +21,8 @@ STATUS_NAMES = {0: "AG_OK", 1: "AG_ERR_BAD_INPUT", 2: "AG_ERR_NO_DONOR_H", 3: "A
                 4: "AG_ERR_DEVICE", 5: "AG_ERR_IO", 6: "AG_ERR_UNSUPPORTED"}
 AG_JOB_NO_FILES = 1
 AG_NEINT = 1024
+AG_POTENTIAL_AD4 = 0
+AG_POTENTIAL_VINA = 1
 AG_NDIEL = 8192
 
 
@@ -117,9 +119,16 @@ def lib() -> C.CDLL:
         L.ag_plan_destroy.argtypes = [vp]
         L.ag_plan_audit.argtypes = [vp, C.POINTER(C.c_int64 * 4)]
         L.ag_context_timing.argtypes = [vp, C.POINTER(C.c_float)]
+        L.ag_xs_radius.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
+        L.ag_vina_tables.argtypes = [C.POINTER(Setup), C.POINTER(MapSet), vp, vp, vp]
+        L.ag_job_prepare_vina.argtypes = [vp]
+        L.ag_maps_vina.argtypes = L.ag_maps_plain.argtypes
+        L.ag_plan_create_potential.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
+                                               C.POINTER(Tables), C.c_int, C.POINTER(vp)]
         for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_plan_audit", "ag_context_timing",
                      "ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",
-                     "ag_job_views", "ag_run_gpf", "ag_maps_plain", "ag_job_write_maps"):
+                     "ag_job_views", "ag_run_gpf", "ag_maps_plain", "ag_job_write_maps", "ag_xs_radius",
+                     "ag_vina_tables", "ag_job_prepare_vina", "ag_maps_vina", "ag_plan_create_potential"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib

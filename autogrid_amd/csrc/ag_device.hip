@@ -689,6 +689,11 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     if (p.d_slot >= 0) p.maps[(long long)p.d_slot * p.P + pt] = d_acc;
 }
 
+__global__ void k_fill(float *out, long long n, float v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+
 // Covalent maps (main.cpp:2866-2879): a Gaussian well around the
 // attachment point, no receptor atoms involved.
 struct CovParams {
@@ -971,6 +976,7 @@ struct ag_plan {
     FarParams far{};
     bool run_far = false;
     bool far_clamp = true;
+    std::vector<int> zero_slots;       // output maps that are all zero (Vina e/d)
 };
 
 namespace agdev {
@@ -1010,7 +1016,8 @@ static T *upload(hipStream_t st, DevBuf &b, const T *src, size_t n) {
     return d;
 }
 
-static void validate(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab) {
+static void validate(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,
+                     bool vina) {
     const int N = rec->n_atoms, M = ms->n_atom_maps, T = tab->n_rec_types;
     if (N < 0 || N > AG_MAX_ATOMS) ag::fail(AG_ERR_BAD_INPUT, "atom count out of range");
     if (M < 0 || M + 2 > AG_MAX_MAPS) ag::fail(AG_ERR_BAD_INPUT, "map count out of range");
@@ -1029,7 +1036,9 @@ static void validate(const ag_receptor *rec, const ag_grid *grid, const ag_mapse
         if (rec->type[i] < 0 || rec->type[i] >= T) ag::fail(AG_ERR_BAD_INPUT, "atom type index out of range");
         if (rec->hbond[i] < 0 || rec->hbond[i] > 6) ag::fail(AG_ERR_BAD_INPUT, "atom hbond class out of range");
     }
-    if (M > 0) {
+    if (vina && ms->separate_desolvation)
+        ag::fail(AG_ERR_UNSUPPORTED, "Vina-potential maps do not combine with separate_desolvation_maps");
+    if (M > 0 && !vina) {
         bool any_h = false;
         for (int i = 0; i < N && !any_h; i++) any_h = rec->hbond[i] == AG_HB_DS || rec->hbond[i] == AG_HB_D1;
         if (!any_h) ag::fail(AG_ERR_NO_DONOR_H, "no closestH atom was found");
@@ -1058,8 +1067,10 @@ static int hb_case(int mh, int ah) {
 }
 
 ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
-                     const ag_tables *tab) {
-    validate(rec, grid, ms, tab);
+                     const ag_tables *tab, int potential) {
+    const bool vina = potential == AG_POTENTIAL_VINA;
+    if (potential != AG_POTENTIAL_AD4 && !vina) ag::fail(AG_ERR_BAD_INPUT, "unknown potential");
+    validate(rec, grid, ms, tab, vina);
     HIP_CHECK(hipSetDevice(c->device));
     std::unique_ptr<ag_plan> pl(new ag_plan());
     ag_plan &p = *pl;
@@ -1117,13 +1128,15 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
 
     // Map slots of the near-kernel passes.  Maps without H-bond terms (and
     // the desolvation halves of separate_desolvation_maps) take plain slots,
-    // H-bonding probes take the H-bond slots; covalent maps take none.
+    // H-bonding probes take the H-bond slots; covalent maps take none.  With
+    // the Vina potential every map is a plain table sum: no directional
+    // H-bond terms, no desolvation, no e or d map.
     std::vector<int> plain_maps, hb_maps;
     std::vector<int> cov_maps;
     for (int m = 0; m < M; m++) {
         const int role = map_role(ms, m);
         if (role == ROLE_COV) cov_maps.push_back(m);
-        else if ((role & ROLE_AFF) && ms->maps[m].is_hbonder) hb_maps.push_back(m);
+        else if (!vina && (role & ROLE_AFF) && ms->maps[m].is_hbonder) hb_maps.push_back(m);
         else plain_maps.push_back(m);
     }
     p.groups = std::max<int>(1, std::max((int)(plain_maps.size() + kPlainSlots - 1) / kPlainSlots,
@@ -1197,7 +1210,7 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
             const int role = map_role(ms, m);
             const ag_map_desc &md = ms->maps[m];
             q.kE[j] = (role & ROLE_AFF) ? 1.f : 0.f;
-            if (role & ROLE_DES) {
+            if ((role & ROLE_DES) && !vina) {
                 q.sp[j] = (float)md.solpar_probe;
                 q.volp[j] = (float)md.vol_probe;
             }
@@ -1218,8 +1231,8 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
             }
         }
         q.tab = d_tab + g * gstride;
-        q.e_slot = (g == 0) ? ms->elec_index : -1;
-        q.d_slot = (g == 0) ? ms->dsolv_index : -1;
+        q.e_slot = (g == 0 && !vina) ? ms->elec_index : -1;
+        q.d_slot = (g == 0 && !vina) ? ms->dsolv_index : -1;
         q.has_affinity = used || q.d_slot >= 0;
         p.near.push_back(q);
     }
@@ -1264,7 +1277,12 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     }
 
     // far kernel (e map): per-atom float4 and the table with bins < 1025 zeroed
-    p.run_far = ms->elec_index >= 0 && N > 0;
+    // Vina: the e and d maps stay zero (main.cpp:2968, 3391)
+    if (vina) {
+        if (ms->elec_index >= 0) p.zero_slots.push_back(ms->elec_index);
+        if (ms->dsolv_index >= 0) p.zero_slots.push_back(ms->dsolv_index);
+    }
+    p.run_far = !vina && ms->elec_index >= 0 && N > 0;
     if (p.run_far) {
         float4 *d_far = (float4 *)p.far_atoms.get(sizeof(float4) * (size_t)N);
         k_far_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_charge, ms->coeff_estat, d_far);
@@ -1365,6 +1383,10 @@ void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
         k_covalent<<<(unsigned)((p->P + 255) / 256), 256, 0, st>>>(cv);
         HIP_CHECK(hipGetLastError());
     }
+    for (int m : p->zero_slots) {
+        k_fill<<<(unsigned)((p->P + 255) / 256), 256, 0, st>>>(d_maps + (size_t)m * p->P, p->P, 0.f);
+        HIP_CHECK(hipGetLastError());
+    }
     HIP_CHECK(hipEventRecord(c->ev[2], st));
     if (p->run_far) {
         FarParams fp = p->far;
@@ -1391,8 +1413,8 @@ void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
 // write map j while later maps are still in flight.
 void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                          const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap,
-                         const std::function<void(int)> &on_ready) {
-    std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab), plan_destroy);
+                         const std::function<void(int)> &on_ready, int potential) {
+    std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab, potential), plan_destroy);
     const size_t pbytes = sizeof(float) * (size_t)p->P;
     const size_t bytes = pbytes * (size_t)p->n_maps;
     float *d_maps = (float *)c->maps.get(bytes + 2 * pbytes + 256);
@@ -1454,10 +1476,10 @@ void plan_audit(ag_plan *p, long long out[4]) {
 }
 
 void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,
-                float *maps_out, float *fmap, int32_t *cmap, int outputs_on_device) {
+                float *maps_out, float *fmap, int32_t *cmap, int outputs_on_device, int potential) {
     if ((ms->floating_grid && !fmap) || (ms->constriction_grid && !cmap))
         ag::fail(AG_ERR_BAD_INPUT, "the map set asks for a floating/constriction grid but no output was given");
-    std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab), plan_destroy);
+    std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab, potential), plan_destroy);
     const size_t bytes = sizeof(float) * (size_t)p->n_maps * p->P;
     const size_t pbytes = sizeof(float) * (size_t)p->P;
     float *d_maps = outputs_on_device ? maps_out : (float *)c->maps.get(bytes + 2 * pbytes + 256);

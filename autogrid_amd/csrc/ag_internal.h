@@ -126,6 +126,7 @@ struct Job {
     std::vector<double> rvector, rvector2;
     std::vector<uint8_t> disorder;
     std::vector<ag_map_desc> map_desc;
+    bool vina = false;                 // e_vdw_hb holds Vina-potential tables
     int t_hydrogen = -1, t_nonhb_hydrogen = -1, t_carbon = -1, t_arom_carbon = -1;
     int t_oxygen = -1, t_sulphur = -1, t_nitrogen = -1, t_nonhb_nitrogen = -1;
 
@@ -145,6 +146,10 @@ void job_prepare(Job &job);
 void job_views(const Job &job, ag_receptor *rec, ag_grid *grid, ag_mapset *ms,
                ag_tables *tab, ag_setup *setup);
 void build_vdw_tables(const ag_setup &s, const ag_mapset &ms, float *out);
+bool xs_radius_lookup(const char *type, double *radius);
+void build_vina_tables(const ag_setup &s, const ag_mapset &ms, const double *xs_map, const double *xs_rec,
+                       float *out);
+void job_prepare_vina(Job &job);
 void build_sol_fn(double coeff_desolv, double *out);
 void build_r_eps(double *out);
 void build_bond_vectors(const ag_setup &s, int n, const double *coord,
@@ -174,14 +179,14 @@ ag_status context_create(ag_ctx **out, int device);
 void context_destroy(ag_ctx *ctx);
 void context_timing(ag_ctx *ctx, float *ms);
 ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
-                     const ag_tables *tab);
+                     const ag_tables *tab, int potential = AG_POTENTIAL_AD4);
 void plan_destroy(ag_plan *p);
 void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap);
 void plan_audit(ag_plan *p, long long out[4]);
 void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                          const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap,
-                         const std::function<void(int)> &on_ready);
+                         const std::function<void(int)> &on_ready, int potential = AG_POTENTIAL_AD4);
 void maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                 const ag_mapset *ms, const ag_tables *tab, float *maps,
-                float *fmap, int32_t *cmap, int outputs_on_device);
+                float *fmap, int32_t *cmap, int outputs_on_device, int potential = AG_POTENTIAL_AD4);
 }  // namespace agdev

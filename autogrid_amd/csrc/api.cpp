@@ -215,6 +215,57 @@ ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *gri
     return guarded([&] { *out = agdev::plan_create(ctx, rec, grid, ms, tab); });
 }
 
+ag_status ag_plan_create_potential(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                                   const ag_tables *tab, int potential, ag_plan **out) {
+    if (!ctx || !rec || !grid || !ms || !tab || !out) {
+        ag::set_last_error("null argument to ag_plan_create_potential");
+        return AG_ERR_BAD_INPUT;
+    }
+    *out = nullptr;
+    return guarded([&] { *out = agdev::plan_create(ctx, rec, grid, ms, tab, potential); });
+}
+
+int ag_xs_radius(const char *type, double *radius) {
+    double r = 1.8;
+    const int found = type ? (int)ag::xs_radius_lookup(type, &r) : 0;
+    if (radius) *radius = r;
+    return found;
+}
+
+ag_status ag_vina_tables(const ag_setup *setup, const ag_mapset *ms, const double *xs_map, const double *xs_rec,
+                         float *out) {
+    if (!setup || !ms || !xs_map || !xs_rec || !out || !setup->pairs || (setup->n_atom_maps > 0 && !ms->maps)) {
+        ag::set_last_error("null argument to ag_vina_tables");
+        return AG_ERR_BAD_INPUT;
+    }
+    if (setup->n_atom_maps != ms->n_atom_maps) {
+        ag::set_last_error("setup and map set disagree on the number of atom maps");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { ag::build_vina_tables(*setup, *ms, xs_map, xs_rec, out); });
+}
+
+ag_status ag_job_prepare_vina(ag_job *j) {
+    if (!j) {
+        ag::set_last_error("null job");
+        return AG_ERR_BAD_INPUT;
+    }
+    ag_status st = guarded([&] { ag::job_prepare_vina(j->job); });
+    if (st != AG_OK) log_fatal(j->job, ag::g_last_error);
+    return st;
+}
+
+ag_status ag_maps_vina(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                       const ag_tables *tab, float *maps, float *fmap, int32_t *cmap, int outputs_on_device) {
+    if (!ctx || !rec || !grid || !ms || !tab || !maps) {
+        ag::set_last_error("null argument to ag_maps_vina");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        agdev::maps_plain(ctx, rec, grid, ms, tab, maps, fmap, cmap, outputs_on_device, AG_POTENTIAL_VINA);
+    });
+}
+
 ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev) {
     if (!plan || !maps_dev) {
         ag::set_last_error("null argument to ag_plan_execute");

@@ -298,3 +298,109 @@ void job_prepare(Job &job) {
 }
 
 }  // namespace ag
+
+namespace ag {
+
+// xs_vdw radius of a ligand or receptor type: the ADVINA table of
+// read_parameter_library.cc:320-358 (exact type-name match, float values);
+// types outside it get the 1.8 A default there (a double literal).
+bool xs_radius_lookup(const char *type, double *radius) {
+    static const struct {
+        const char *name;
+        float value;
+    } table[] = {{"C", 1.9f},  {"A", 1.9f},  {"N", 1.8f},  {"NA", 1.8f}, {"NS", 1.8f},
+                 {"O", 1.7f},  {"OA", 1.7f}, {"OS", 1.7f}, {"S", 2.0f},  {"SA", 2.0f},
+                 {"P", 2.1f},  {"F", 1.5f},  {"Mg", 1.2f}};
+    for (const auto &e : table)
+        if (strcmp(type, e.name) == 0) {
+            *radius = (double)e.value;
+            return true;
+        }
+    *radius = 1.8;
+    return false;
+}
+
+// Vina-potential tables (the use_vina_potential branch of main.cpp:2075-2127,
+// never enabled in the shipped reference).  For map m, receptor type t and
+// bin k = 1..1023, with rdd = k/128 - (xs_rec + xs_map):
+//   gauss1 + gauss2 terms, repulsion rdd^2 for rdd < 0,
+//   H-bond pairs (hbonder[m][t] > 0): wt_h for rdd <= 0.7, then minus
+//     (rdd/0.7) wt_h for -0.7 < rdd <= 0,
+//   hydrophobic (hbonder[m][t] == 0 and hbonder[t][m] == 0, the latter
+//     read as 0 where t or m is outside the map or type range): wt_p below
+//     0.5, (0.5 - rdd) wt_p below 1.5.
+// Each term is added to the float table entry in that order, as the
+// reference's += on a float table does; bin 0 stays 0; no smoothing and no
+// clamp.  The weights are the reference's float constants; the receptor
+// radius passes through a float (map_Rij), the ligand one stays double.
+void build_vina_tables(const ag_setup &s, const ag_mapset &ms, const double *xs_map, const double *xs_rec,
+                       float *out) {
+    const int T = s.n_rec_types, M = s.n_atom_maps;
+    const float wt_gauss1 = -0.035579f, wt_gauss2 = -0.005156f, wt_repulsion = 0.840245f,
+                wt_hydrogen = -0.587439f, wt_hydrophobic = -0.035069f;
+    memset(out, 0, sizeof(float) * (size_t)T * M * AG_NEINT);
+    for (int m = 0; m < M; m++) {
+        if (ms.maps[m].is_covalent) continue;
+        for (int t = 0; t < T; t++) {
+            const double lig_r = xs_map[m];
+            const float rec_r = (float)xs_rec[t];
+            const int hb = s.pairs[(size_t)m * T + t].hbonder;
+            const int hb_cross = (t < M && m < T) ? s.pairs[(size_t)t * T + m].hbonder : 0;
+            float *e = out + ((size_t)t * M + m) * AG_NEINT;
+            for (int k = 1; k < AG_NEINT; k++) {
+                const double r = (double)k / AG_A_DIV;
+                const double rdd = r - ((double)rec_r + lig_r);
+                double de = wt_gauss1 * exp(-pow(rdd / 0.5, 2)) + wt_gauss2 * exp(-pow((rdd - 3.) / 2., 2));
+                e[k] += de;
+                if (rdd < 0) {
+                    de = wt_repulsion * pow(rdd, 2);
+                    e[k] += de;
+                }
+                if (hb > 0) {
+                    if (rdd <= 0.7) {
+                        de = 1 * wt_hydrogen;
+                        e[k] += de;
+                    }
+                    if (-0.7 < rdd && rdd <= 0.) {
+                        de = (rdd / 0.7) * wt_hydrogen;
+                        e[k] -= de;
+                    }
+                }
+                if (hb == 0 && hb_cross == 0) {
+                    de = 0.;
+                    if (rdd < 0.5) de = 1 * wt_hydrophobic;
+                    else if (rdd < 1.5) de = (0.5 - rdd) * wt_hydrophobic;
+                    e[k] += de;
+                }
+            }
+        }
+    }
+}
+
+void job_prepare_vina(Job &job) {
+    if (job.separate_desolvation)
+        fail(AG_ERR_UNSUPPORTED, "Vina-potential maps do not combine with separate_desolvation_maps");
+    if (!job.prepared) job_prepare(job);
+    const int T = (int)job.rec_types.size();
+    const int M = job.num_atom_maps;
+    std::vector<double> xs_map(M), xs_rec(T);
+    for (int m = 0; m < M; m++)
+        if (!xs_radius_lookup(job.maps[m].type.c_str(), &xs_map[m]))
+            job.logf("%s: WARNING: atom type %s not found in the xs_vdw_lookup table.\n\n", job.program.c_str(),
+                     job.maps[m].type.c_str());
+    for (int t = 0; t < T; t++)
+        if (!xs_radius_lookup(job.rec_types[t].c_str(), &xs_rec[t]))
+            job.logf("%s: WARNING: atom type %s not found in the xs_vdw_lookup table.\n\n", job.program.c_str(),
+                     job.rec_types[t].c_str());
+    ag_receptor rec;
+    ag_grid grid;
+    ag_mapset ms;
+    ag_tables tab;
+    ag_setup setup;
+    job_views(job, &rec, &grid, &ms, &tab, &setup);
+    build_vina_tables(setup, ms, xs_map.data(), xs_rec.data(), job.e_vdw_hb.data());
+    job.vina = true;
+    job.logf("\n Using Vina potential for calculation.\n\n");
+}
+
+}  // namespace ag

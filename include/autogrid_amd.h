@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define AG_ABI_VERSION 1
+#define AG_ABI_VERSION 2
 
 /* Table sizes of the 8 A build (constants.h:148-171, USE_8A_NBCUTOFF). */
 #define AG_A_DIV 128           /* bins per Angstrom                         */
@@ -50,6 +50,11 @@ typedef enum ag_status {
     AG_ERR_IO = 5,          /* file could not be opened or written         */
     AG_ERR_UNSUPPORTED = 6  /* request outside what this build implements  */
 } ag_status;
+
+/* Pair potentials of the map path: AutoDock 4 (the reference's only live
+ * path) and the Vina potential of the reference's disabled
+ * use_vina_potential branch (main.cpp:263, 1975-2127). */
+enum { AG_POTENTIAL_AD4 = 0, AG_POTENTIAL_VINA = 1 };
 
 /* H-bond classes of parameters.h:36-37 plus the AD code the hot loop tests. */
 enum { AG_HB_NON = 0, AG_HB_DS = 1, AG_HB_D1 = 2, AG_HB_AS = 3,
@@ -248,6 +253,54 @@ ag_status ag_plan_audit(ag_plan *plan, ag_audit *out);
  * stream: [0] cell list, [1] near-field kernel(s), [2] far-field kernel,
  * [3] total. */
 ag_status ag_context_timing(ag_ctx *ctx, float ms[4]);
+
+/* ------------------------------------------------------------------------ */
+/* Vina potential.                                                           */
+/* ------------------------------------------------------------------------ */
+/* The reference builds Vina tables only in a branch that a const flag keeps
+ * off (main.cpp:263 `use_vina_potential = FALSE`; the GPF keyword is fatal,
+ * main.cpp:1466-1470), with the per-point loop's electrostatic, desolvation
+ * and d-map terms gated off (main.cpp:2968, 3379, 3391) and the bond-vector
+ * pass skipped (main.cpp:2295-2662).  This build defines the Vina map set
+ * as: the tables of that branch, summed over the atoms within 8 A of each
+ * point at bin min(floor(128 r), 1023) with no directional factor (the
+ * bond vectors the directional terms need are never built in that mode);
+ * covalent maps, fmap and cmap as in the AD4 path; e and d maps all zero. */
+
+/* xs_vdw radius of a type (read_parameter_library.cc:320-358, the ADVINA
+ * table).  Returns 1 when the type is in the table, 0 when the 1.8 A
+ * default applies; *radius receives the value either way. */
+int ag_xs_radius(const char *type, double *radius);
+
+/* Vina tables [n_rec_types][n_atom_maps][AG_NEINT] float (main.cpp:2075-
+ * 2127): for bin k = 1..1023 and rdd = k/128 - (xs_rec + xs_map),
+ *   -0.035579 exp(-(rdd/0.5)^2) - 0.005156 exp(-((rdd-3)/2)^2),
+ *   + 0.840245 rdd^2 for rdd < 0,
+ *   H-bond pairs (pairs[m][t].hbonder > 0): + -0.587439 for rdd <= 0.7,
+ *     then - (rdd/0.7)(-0.587439) for -0.7 < rdd <= 0,
+ *   other pairs (also pairs[t][m].hbonder == 0 where that pair exists):
+ *     + -0.035069 for rdd < 0.5, (0.5 - rdd)(-0.035069) for rdd < 1.5;
+ * each term added to the float entry in that order; bin 0 is 0; covalent
+ * maps' rows are 0.  xs_map [n_atom_maps], xs_rec [n_rec_types]. */
+ag_status ag_vina_tables(const ag_setup *setup, const ag_mapset *mapset, const double *xs_map,
+                         const double *xs_rec, float *out);
+
+/* ag_job_prepare, then replace the job's e_vdW_Hb tables by the Vina tables
+ * of its map and receptor types; ag_job_views then returns those tables. */
+ag_status ag_job_prepare_vina(ag_job *job);
+
+/* Vina map set for one receptor on one grid; arguments as ag_maps_plain,
+ * `tables->e_vdw_hb` holding Vina tables. */
+ag_status ag_maps_vina(ag_ctx *ctx, const ag_receptor *rec,
+                       const ag_grid *grid, const ag_mapset *mapset,
+                       const ag_tables *tables, float *maps, float *fmap,
+                       int32_t *cmap, int outputs_on_device);
+
+/* Device-resident plan of either potential (ag_plan_create is the AD4
+ * one). */
+ag_status ag_plan_create_potential(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
+                                   const ag_mapset *mapset, const ag_tables *tables, int potential,
+                                   ag_plan **out);
 
 /* Map output: the value text of main.cpp:3486-3497 -- "0" when |e| < 1e-4,
  * else "%.3f" of (float)(floor(e*1000+0.5)/1000) -- for n values into buf.

@@ -505,6 +505,134 @@ int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, c
     return 0;
 }
 
+/* --- Vina potential (main.cpp:1975-2127, the use_vina_potential branch) --- */
+
+/* xs_vdw radii of read_parameter_library.cc:322-358 (ADVINA): float values,
+ * exact name match, 1.8 (double) otherwise. */
+int orc_xs_radius(const char *type, double *radius) {
+    static const char *names[] = {"C", "A", "N", "NA", "NS", "O", "OA", "OS", "S", "SA", "P", "F", "Mg"};
+    static const float vals[] = {1.9f, 1.9f, 1.8f, 1.8f, 1.8f, 1.7f, 1.7f, 1.7f, 2.0f, 2.0f, 2.1f, 1.5f, 1.2f};
+    for (int i = 0; i < 13; i++)
+        if (strcmp(type, names[i]) == 0) {
+            *radius = vals[i];
+            return 1;
+        }
+    *radius = 1.8;
+    return 0;
+}
+
+int orc_vina_tables(const ag_setup *s, const ag_mapset *ms, const double *xs_map, const double *xs_rec,
+                    float *out) {
+    const int T = s->n_rec_types, M = s->n_atom_maps;
+    /* the reference's weights are float locals (main.cpp:1977-1981) */
+    float wt_gauss1 = -0.035579, wt_gauss2 = -0.005156, wt_repulsion = 0.840245, wt_hydrogen = -0.587439,
+          wt_hydrophobic = -0.035069;
+    memset(out, 0, sizeof(float) * (size_t)T * M * NEINT);
+    for (int ia = 0; ia < M; ia++) {
+        if (ms->maps[ia].is_covalent) continue;
+        for (int i = 0; i < T; i++) {
+            float *e = out + ((size_t)i * M + ia) * NEINT;
+            double Rij = xs_map[ia];        /* ligand radius, double */
+            float map_Rij = (float)xs_rec[i]; /* receptor radius through a float */
+            int hbonder = s->pairs[(size_t)ia * T + i].hbonder;
+            /* gridmap[i].hbonder[ia] -- map i, type ia; 0 outside the arrays */
+            int hbonder_x = (i < M && ia < T) ? s->pairs[(size_t)i * T + ia].hbonder : 0;
+            for (int indx_r = 1; indx_r < NEINT; indx_r++) {
+                double r = (double)indx_r / A_DIV;
+                double rddist = r - (map_Rij + Rij);
+                double delta_e = wt_gauss1 * exp(-pow(((rddist) / 0.5), 2)) +
+                                 wt_gauss2 * exp(-pow(((rddist - 3.) / 2.), 2));
+                e[indx_r] += delta_e;
+                if (rddist < 0) {
+                    delta_e = wt_repulsion * pow(rddist, 2);
+                    e[indx_r] += delta_e;
+                }
+                if (hbonder > 0) {
+                    if (rddist <= 0.7) {
+                        delta_e = 1 * wt_hydrogen;
+                        e[indx_r] += delta_e;
+                    }
+                    if ((-0.7 < rddist) && (rddist <= 0.)) {
+                        delta_e = (rddist / 0.7) * wt_hydrogen;
+                        e[indx_r] -= delta_e;
+                    }
+                }
+                if ((hbonder == 0) && (hbonder_x == 0)) {
+                    delta_e = 0.;
+                    if (rddist < 0.5) {
+                        delta_e = 1 * wt_hydrophobic;
+                    } else if (rddist < 1.5) {
+                        delta_e = (0.5 - rddist) * wt_hydrophobic;
+                    }
+                    e[indx_r] += delta_e;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+/* Vina map set: covalent maps and Loop 1's r_min / count as in orc_maps; per
+ * atom map the table value at min(floor(128 r), 1023) of every atom within
+ * 8 A (sqrt distance, disordered H skipped under disorder_h); e and d maps
+ * zero. */
+int orc_maps_vina(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,
+                  double *maps, double *fmap, int32_t *cmap, int z0, int z1) {
+    const int N = rec->n_atoms, M = ms->n_atom_maps;
+    const int n1x = grid->n1[0], n1y = grid->n1[1];
+    const long long P = (long long)grid->n1[0] * grid->n1[1] * grid->n1[2];
+    const double spacing = grid->spacing;
+    const double ln_half = log(0.5);
+    const float *evdw = tab->e_vdw_hb;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int iz = z0; iz < z1; iz++) {
+        double c[3], d[3];
+        c[2] = ((double)(iz - grid->ne[2])) * spacing;
+        for (int iy = 0; iy < n1y; iy++) {
+            c[1] = ((double)(iy - grid->ne[1])) * spacing;
+            for (int ix = 0; ix < n1x; ix++) {
+                const long long mapi = ix + (long long)n1x * (iy + (long long)n1y * iz);
+                c[0] = ((double)(ix - grid->ne[0])) * spacing;
+                for (int j = 0; j < ms->n_maps; j++) maps[(size_t)j * P + mapi] = 0.;
+                for (int j = 0; j < M; j++)
+                    if (ms->maps[j].is_covalent) {
+                        double rcov;
+                        for (int ii = 0; ii < 3; ii++) d[ii] = ms->cov_pos[ii] - c[ii];
+                        rcov = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / ms->cov_half_width;
+                        if (rcov < APPROX_ZERO) rcov = APPROX_ZERO;
+                        maps[(size_t)j * P + mapi] = ms->cov_barrier * (1. - exp(ln_half * rcov * rcov));
+                    }
+                if (fmap || cmap) {
+                    if (fmap) fmap[mapi] = 1.0e12;
+                    if (cmap) cmap[mapi] = 0;
+                    for (int ia = 0; ia < N; ia++) {
+                        const double *a = rec->coord + 3 * (size_t)ia;
+                        double r = hypot((a[0] - c[0]), hypot((a[1] - c[1]), (a[2] - c[2])));
+                        if (r < APPROX_ZERO) r = APPROX_ZERO;
+                        if (fmap) fmap[mapi] = dmin(r, fmap[mapi]);
+                        if (cmap && r <= ms->constriction_cutoff) cmap[mapi]++;
+                    }
+                }
+                for (int ia = 0; ia < N; ia++) {
+                    const double *a = rec->coord + 3 * (size_t)ia;
+                    const int at = rec->type[ia];
+                    for (int i = 0; i < 3; i++) d[i] = a[i] - c[i];
+                    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    if (r > SOFTNBC) continue;
+                    if (ms->disorder_h && rec->disorder[ia] && at == ms->hydrogen_type) continue;
+                    if (r < APPROX_ZERO) r = APPROX_ZERO;
+                    int indx_n = (int)(r * A_DIV);
+                    if (indx_n > NEINT - 1) indx_n = NEINT - 1;
+                    for (int m = 0; m < M; m++)
+                        if (!ms->maps[m].is_covalent)
+                            maps[(size_t)m * P + mapi] += (double)evdw[((size_t)at * M + m) * NEINT + indx_n];
+                }
+            }
+        }
+    }
+    return 0;
+}
+
 /* main.cpp:3486-3497 with HAVE_ROUND unset (main.cpp:85-89) */
 int orc_format_value(double e, char *out) {
     if (fabs(e) < 0.0001) {

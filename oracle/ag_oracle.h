@@ -40,6 +40,16 @@ int orc_bond_vectors(const ag_setup *s, int n, const double *coord, const int32_
 int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
              const ag_tables *tab, double *maps, double *fmap, int32_t *cmap,
              int z0, int z1);
+/* xs_vdw radius of a type (read_parameter_library.cc:322-358); 1 if listed. */
+int orc_xs_radius(const char *type, double *radius);
+/* Vina tables [n_rec_types][n_atom_maps][1024] (main.cpp:2075-2127). */
+int orc_vina_tables(const ag_setup *s, const ag_mapset *ms, const double *xs_map,
+                    const double *xs_rec, float *out);
+/* Vina map set over z planes [z0, z1) (main.cpp:2805-3431 with
+ * use_vina_potential on and no directional factors). */
+int orc_maps_vina(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                  const ag_tables *tab, double *maps, double *fmap, int32_t *cmap,
+                  int z0, int z1);
 /* One map value as the reference writes it (main.cpp:3486-3497). */
 int orc_format_value(double e, char *out);
 /* One floating-grid value (main.cpp:3502-3507). */

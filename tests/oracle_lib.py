@@ -40,6 +40,9 @@ def olib() -> C.CDLL:
         L.orc_bond_vectors.argtypes = [C.POINTER(_abi.Setup), C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.orc_maps.argtypes = [C.POINTER(_abi.Receptor), C.POINTER(_abi.Grid), C.POINTER(_abi.MapSet),
                                C.POINTER(_abi.Tables), vp, vp, vp, C.c_int, C.c_int]
+        L.orc_xs_radius.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
+        L.orc_vina_tables.argtypes = [C.POINTER(_abi.Setup), C.POINTER(_abi.MapSet), vp, vp, vp]
+        L.orc_maps_vina.argtypes = L.orc_maps.argtypes
         L.orc_format_value.argtypes = [C.c_double, C.c_char_p]
         L.orc_format_rmin.argtypes = [C.c_double, C.c_char_p]
         _lib = L
@@ -124,3 +127,44 @@ def format_rmin_map(values) -> bytes:
 
 def format_count_map(values) -> bytes:
     return b"".join(b"%d\n" % int(v) for v in np.asarray(values).reshape(-1))
+
+
+def xs_radius(type_name: str):
+    r = C.c_double()
+    found = olib().orc_xs_radius(type_name.encode(), C.byref(r))
+    return float(r.value), bool(found)
+
+
+def vina_tables(st: ag.PreparedState, lig_types, rec_types) -> np.ndarray:
+    """The oracle's Vina tables [T, M, 1024] for a state's pairs and type names."""
+    _, _, ms, _, setup = st.structs()
+    xs_map = np.array([xs_radius(t)[0] for t in lig_types], np.float64)
+    xs_rec = np.array([xs_radius(t)[0] for t in rec_types], np.float64)
+    out = np.zeros((len(rec_types), st.n_atom_maps, 1024), np.float32)
+    olib().orc_vina_tables(C.byref(setup), C.byref(ms), xs_map.ctypes.data, xs_rec.ctypes.data, out.ctypes.data)
+    return out
+
+
+def maps_vina(st: ag.PreparedState, z0: int = 0, z1: int | None = None, fmap=False, cmap=False):
+    """FP64 Vina map set [n_maps, n1z, n1y, n1x] (planes z0..z1) of a state holding Vina tables."""
+    rec, grid, ms, tab, setup = st.structs()
+    nz, ny, nx = st.n1[2], st.n1[1], st.n1[0]
+    z1 = nz if z1 is None else z1
+    out = np.zeros((st.n_maps, nz, ny, nx))
+    fm = np.zeros((nz, ny, nx)) if fmap else None
+    cm = np.zeros((nz, ny, nx), np.int32) if cmap else None
+    olib().orc_maps_vina(C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab), out.ctypes.data,
+                         fm.ctypes.data if fmap else None, cm.ctypes.data if cmap else None, z0, z1)
+    return (out, fm, cm) if (fmap or cmap) else out
+
+
+def gpf_types(gpf_path: str):
+    """(ligand_types, receptor_types) as written in a GPF."""
+    lig, recp = [], []
+    for line in open(gpf_path):
+        w = line.split("#")[0].split()
+        if w and w[0] == "ligand_types":
+            lig = w[1:]
+        elif w and w[0] == "receptor_types":
+            recp = w[1:]
+    return lig, recp

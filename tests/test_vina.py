@@ -1,0 +1,191 @@
+"""Vina-potential map path (the reference's use_vina_potential branch).
+
+The reference never runs this branch (main.cpp:263 keeps the flag FALSE and
+the GPF keyword is fatal, main.cpp:1466-1470), so no reference output exists
+and the tables are "parity unpinned" against the reference itself.  What is
+pinned here:
+
+* the engine's table builder against the oracle, bit for bit;
+* the xs radii against the ADVINA table values (read_parameter_library.cc:
+  322-358) -- float values, exact name match, 1.8 A default;
+* structural properties of the tables that a dropped or mis-signed term
+  breaks: bin 0 is 0, covalent rows are 0; an H-bonding and a non-H-bonding
+  pair with equal radius sums differ by exactly the H-bond term minus the
+  hydrophobic term (piecewise linear in rdd, derived from the branch's
+  conditions); swapping which side carries a radius changes a row only by
+  the hydrophobic term the reference's cross-indexed flag selects;
+* the oracle's Vina per-point sum against its AD4 per-point loop (pinned to
+  the reference binary) run on the same tables with every map made a
+  non-H-bonding, desolvation-free map and no e/d map.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+import autogrid_amd as ag
+import cases
+import oracle_lib as ol
+import parity
+
+XS_TABLE = {"C": 1.9, "A": 1.9, "N": 1.8, "NA": 1.8, "NS": 1.8, "O": 1.7, "OA": 1.7, "OS": 1.7,
+            "S": 2.0, "SA": 2.0, "P": 2.1, "F": 1.5, "Mg": 1.2}
+W_HB, W_HP = np.float32(-0.587439), np.float32(-0.035069)
+
+
+def _vina_job(name, tmp_path):
+    gpf = cases.materialize(name, str(tmp_path))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        st = ag.Job(gpf).prepare_vina().state()
+    finally:
+        os.chdir(cwd)
+    lig, recp = ol.gpf_types(os.path.join(tmp_path, gpf))
+    return st, lig, recp
+
+
+def test_xs_radius_table():
+    for t, v in XS_TABLE.items():
+        r, found = ag.xs_radius(t)
+        assert found and r == float(np.float32(v)), t
+        assert ol.xs_radius(t) == (r, True)
+    for t in ("HD", "Cl", "Br", "I", "Zn", "c", "OAX"):
+        assert ag.xs_radius(t) == (1.8, False)
+        assert ol.xs_radius(t) == (1.8, False)
+
+
+@pytest.mark.parametrize("name", ["small", "all13", "covalent", "disorder"])
+def test_vina_tables_match_oracle(name, tmp_path):
+    st, lig, recp = _vina_job(name, tmp_path)
+    ref = ol.vina_tables(st, lig, recp)
+    assert st.e_vdw_hb.shape == ref.shape
+    assert np.array_equal(st.e_vdw_hb.view(np.uint32), ref.view(np.uint32))
+    # the C-ABI builder on the same state gives the same bits
+    direct = ag.vina_tables(st, lig, recp)
+    assert np.array_equal(direct.view(np.uint32), ref.view(np.uint32))
+    assert np.all(ref[:, :, 0] == 0)
+    for m in range(st.n_atom_maps):
+        if st.maps[m]["is_covalent"]:
+            assert np.all(ref[:, m, :] == 0)
+
+
+def _rdd(k, xs_sum):
+    return np.arange(k) / 128.0 - xs_sum
+
+
+def test_vina_hbond_minus_hydrophobic(tmp_path):
+    """NA (acceptor) and N (non-H-bonding) probes both have xs 1.8 A; against
+    an HD receptor type their rows differ only by the H-bond and hydrophobic
+    terms of main.cpp:2098-2126."""
+    st, lig, recp = _vina_job("small", tmp_path)
+    t = recp.index("HD")
+    m_na, m_n = lig.index("NA"), lig.index("N")
+    assert st.pairs[m_na, t]["hbonder"] > 0 and st.pairs[m_n, t]["hbonder"] == 0
+    rdd = _rdd(1024, float(np.float32(1.8)) + 1.8)
+    diff = st.e_vdw_hb[t, m_na].astype(np.float64) - st.e_vdw_hb[t, m_n].astype(np.float64)
+    hb = np.where(rdd <= 0.7, float(W_HB), 0.0) - np.where((rdd > -0.7) & (rdd <= 0), rdd / 0.7 * float(W_HB), 0.0)
+    hp = np.where(rdd < 0.5, float(W_HP), np.where(rdd < 1.5, (0.5 - rdd) * float(W_HP), 0.0))
+    expect = hb - hp
+    expect[0] = 0.0
+    # float rounding of the table entries: a few ulps of values below 1.5
+    assert np.max(np.abs(diff - expect)) < 2e-6
+    # beyond rdd 1.5 neither term applies: rows are identical
+    far = rdd > 1.5
+    assert np.array_equal(st.e_vdw_hb[t, m_na][far], st.e_vdw_hb[t, m_n][far])
+
+
+def test_vina_swap_symmetry(tmp_path):
+    """A C probe on an N receptor type and an N probe on a C receptor type
+    have the same radius sum and neither H-bonds, so their rows differ only
+    by the hydrophobic term, present where the reference's cross-indexed
+    flag gridmap[t].hbonder[m] (map index t, type index m) is 0."""
+    st, lig, recp = _vina_job("small", tmp_path)
+    tc, tn = recp.index("C"), recp.index("N")
+    mc, mn = lig.index("C"), lig.index("N")
+    assert st.pairs[mc, tn]["hbonder"] == 0 and st.pairs[mn, tc]["hbonder"] == 0
+    M, T = st.n_atom_maps, len(recp)
+
+    def cross(m, t):
+        return int(st.pairs[t, m]["hbonder"]) if (t < M and m < T) else 0
+
+    rdd = _rdd(1024, float(np.float32(1.9)) + float(np.float32(1.8)))
+    hp = np.where(rdd < 0.5, float(W_HP), np.where(rdd < 1.5, (0.5 - rdd) * float(W_HP), 0.0))
+    hp[0] = 0.0
+    f1, f2 = cross(mc, tn) == 0, cross(mn, tc) == 0
+    diff = st.e_vdw_hb[tn, mc].astype(np.float64) - st.e_vdw_hb[tc, mn].astype(np.float64)
+    assert np.max(np.abs(diff - (f1 - f2) * hp)) < 2e-6
+    # beyond rdd 1.5 no hydrophobic term remains: identical rows
+    far = rdd > 1.5
+    assert np.array_equal(st.e_vdw_hb[tn, mc][far], st.e_vdw_hb[tc, mn][far])
+
+
+def test_vina_repulsive_wall(tmp_path):
+    """For rdd < -0.5 the repulsion dominates: the row decreases with r."""
+    st, lig, recp = _vina_job("small", tmp_path)
+    tc, mc = recp.index("C"), lig.index("C")
+    row = st.e_vdw_hb[tc, mc].astype(np.float64)
+    rdd = _rdd(1024, 2 * float(np.float32(1.9)))
+    sel = np.nonzero((rdd < -0.5) & (np.arange(1024) > 0))[0]
+    assert np.all(np.diff(row[sel]) < 0)
+    # from rdd 1.5 on only the two Gaussians remain: the row is negative.
+    # (Between 0.5 and 1.5 the branch's hydrophobic ramp (0.5 - rdd) * w
+    # is positive, as written in main.cpp:2120.)
+    assert np.all(row[rdd >= 1.5] < 0)
+    ramp = (rdd >= 0.5) & (rdd < 1.5)
+    assert np.all(row[ramp][1:] - row[ramp][:-1] > 0)
+
+
+def test_vina_oracle_reduces_to_ad4_loop(tmp_path):
+    """orc_maps_vina == orc_maps with every map plain and desolvation-free."""
+    st, lig, recp = _vina_job("small", tmp_path)
+    st = dataclasses.replace(st, n1=(st.n1[0], st.n1[1], 6), _keep=[])
+    got = ol.maps_vina(st)
+    plain = st.maps.copy()
+    plain["is_hbonder"] = 0
+    plain["hbond"] = 0
+    plain["solpar_probe"] = 0.0
+    plain["vol_probe"] = 0.0
+    ad4 = dataclasses.replace(st, maps=plain, elec_index=-1, dsolv_index=-1, n_maps=st.n_atom_maps, _keep=[])
+    ref = ol.maps(ad4)
+    M = st.n_atom_maps
+    assert np.array_equal(got[:M], ref[:M])
+    # e and d maps are zero in the Vina map set
+    for k in (st.elec_index, st.dsolv_index):
+        if k >= 0:
+            assert np.all(got[k] == 0)
+
+
+def test_vina_rejects_separate_desolvation(tmp_path):
+    gpf = cases.materialize("sepdesolv", str(tmp_path))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        with pytest.raises(ag.AgError) as e:
+            ag.Job(gpf).prepare_vina()
+    finally:
+        os.chdir(cwd)
+    assert e.value.status == 6
+
+
+def test_vina_keyword_stays_fatal(tmp_path):
+    """The drop-in keeps the reference's behaviour for the GPF keyword."""
+    gpf = cases.materialize("small", str(tmp_path))
+    path = os.path.join(tmp_path, gpf)
+    with open(path, "a") as f:
+        f.write("use_vina_potential\n")
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        with pytest.raises(ag.AgError) as e:
+            ag.Job(gpf)
+    finally:
+        os.chdir(cwd)
+    assert "Vina potential not implemented" in str(e.value)
+
+
+def test_vina_tolerance_helper_is_shared():
+    assert parity.TOL_ABS == 1e-3

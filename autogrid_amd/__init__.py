@@ -25,6 +25,7 @@ from . import _abi
 from ._abi import AgError, check, lib
 
 __all__ = ["AgError", "Context", "Job", "Plan", "PreparedState", "run_gpf", "format_map_values", "xs_radius",
+           "build_tables", "bond_vectors",
            "vina_tables", "AG_POTENTIAL_AD4", "AG_POTENTIAL_VINA",
            "LIB_PATH", "CLI_PATH"]
 
@@ -282,6 +283,30 @@ class Context:
         check(lib().ag_maps_plain(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
                                   C.c_void_p(out_ptr), None, None, 1))
 
+    def batch(self, states: list, tune: dict | None = None, potential: int = _abi.AG_POTENTIAL_AD4) -> "Plan":
+        """Upload many conformations sharing states[0]'s map set and tables
+        (ag_batch_create); tune None runs the autotuner."""
+        return Plan(self, states, potential, batch=True, tune=tune)
+
+    def maps_batch(self, states: list, tune: dict | None = None,
+                   potential: int = _abi.AG_POTENTIAL_AD4, outs: list | None = None) -> list:
+        """Maps of many conformations in one graph launch (ag_maps_batch): a
+        list of [n_maps, n1z, n1y, n1x] float32 arrays."""
+        st0 = states[0]
+        recs, grids, ms, tab, keep = _batch_structs(states)
+        if outs is None:
+            outs = [np.empty((st.n_maps, st.n1[2], st.n1[1], st.n1[0]), np.float32) for st in states]
+        fm = [np.empty((st.n1[2], st.n1[1], st.n1[0]), np.float32) for st in states] if st0.floating_grid else None
+        cm = [np.empty((st.n1[2], st.n1[1], st.n1[0]), np.int32) for st in states] if st0.constriction_grid else None
+        ptrs = (C.c_void_p * len(states))(*[o.ctypes.data for o in outs])
+        fptrs = (C.c_void_p * len(states))(*[a.ctypes.data for a in fm]) if fm else None
+        cptrs = (C.c_void_p * len(states))(*[a.ctypes.data for a in cm]) if cm else None
+        t = _tune_struct(tune)
+        check(lib().ag_maps_batch(self._h, len(states), recs, grids, C.byref(ms), C.byref(tab), ptrs, fptrs, cptrs,
+                                  int(potential), C.byref(t) if t is not None else None))
+        self.last_fmaps, self.last_cmaps = fm, cm
+        return outs
+
     def plan(self, st: PreparedState, potential: int = _abi.AG_POTENTIAL_AD4) -> "Plan":
         """Upload the state to this context's device (ag_plan_create_potential)."""
         return Plan(self, st, potential)
@@ -297,18 +322,64 @@ class Context:
                                program.encode()))
 
 
-class Plan:
-    """A receptor, grid, map set and tables resident on one device."""
+def _tune_struct(tune: dict | None):
+    if tune is None:
+        return None
+    return _abi.Tune(float(tune.get("cell_size", 4.0)), int(bool(tune.get("cube", False))), 0, 0.0)
 
-    def __init__(self, ctx: Context, st: PreparedState, potential: int = _abi.AG_POTENTIAL_AD4):
-        rec, grid, ms, tab, _ = st.structs()
+
+def _batch_structs(states: list):
+    """ctypes arrays of Receptor and Grid for a list of states, the first
+    state's MapSet and Tables, and the list that keeps their arrays alive."""
+    n = len(states)
+    recs = (_abi.Receptor * n)()
+    grids = (_abi.Grid * n)()
+    keep = []
+    ms = tab = None
+    for i, st in enumerate(states):
+        r, g, m, t, _ = st.structs()
+        keep.append(list(st._keep))
+        recs[i] = r
+        grids[i] = g
+        if i == 0:
+            ms, tab = m, t
+    return recs, grids, ms, tab, keep
+
+
+class Plan:
+    """Receptor conformation(s), grid(s), map set and tables resident on one device."""
+
+    def __init__(self, ctx: Context, st, potential: int = _abi.AG_POTENTIAL_AD4, batch: bool = False,
+                 tune: dict | None = None):
         h = C.c_void_p()
-        check(lib().ag_plan_create_potential(ctx.handle, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
-                                             int(potential), C.byref(h)))
+        if batch:
+            states = list(st)
+            recs, grids, ms, tab, keep = _batch_structs(states)
+            t = _tune_struct(tune)
+            check(lib().ag_batch_create(ctx.handle, len(states), recs, grids, C.byref(ms), C.byref(tab),
+                                        int(potential), C.byref(t) if t is not None else None, C.byref(h)))
+            self.n_maps = states[0].n_maps
+        else:
+            rec, grid, ms, tab, _ = st.structs()
+            check(lib().ag_plan_create_potential(ctx.handle, C.byref(rec), C.byref(grid), C.byref(ms),
+                                                 C.byref(tab), int(potential), C.byref(h)))
+            self.n_maps = st.n_maps
         self._h = h
         self.ctx = ctx
-        self.n_maps = st.n_maps
-        self.points = st.points
+        n = lib().ag_plan_conformations(h)
+        self.map_offsets = np.zeros(n, np.int64)
+        self.point_offsets = np.zeros(n, np.int64)
+        mf, pts = C.c_int64(), C.c_int64()
+        check(lib().ag_plan_layout(h, self.map_offsets.ctypes.data, self.point_offsets.ctypes.data,
+                                   C.byref(mf), C.byref(pts)))
+        self.map_floats = int(mf.value)
+        self.points = int(pts.value)
+
+    def tune(self) -> dict:
+        """The near-kernel layout in use and the autotuner's time (ag_plan_tune)."""
+        t = _abi.Tune()
+        check(lib().ag_plan_tune(self._h, C.byref(t)))
+        return {"cell_size": t.cell_size, "cube": bool(t.cube), "tune_ms": t.tune_ms}
 
     def execute(self, maps_dev_ptr: int, fmap_dev_ptr: int | None = None, cmap_dev_ptr: int | None = None) -> None:
         """All maps into device memory at maps_dev_ptr ([n_maps][P] float32)."""
@@ -340,6 +411,33 @@ def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program
     """autogrid4 -p gpf -l glg, on one GPU."""
     with Context(device) as ctx:
         ctx.run_gpf(gpf_path, glg_path, program)
+
+
+def build_tables(st: PreparedState):
+    """(e_vdw_hb [T,M,1024] float32, sol_fn [8192], r_eps [8192]) of a state's setup (ag_build_tables)."""
+    _, _, ms, _, setup = st.structs()
+    T, M = setup.n_rec_types, st.n_atom_maps
+    ev = np.zeros((T, M, _abi.AG_NEINT), np.float32)
+    sol = np.zeros(_abi.AG_NDIEL)
+    reps = np.zeros(_abi.AG_NDIEL)
+    check(lib().ag_build_tables(C.byref(setup), C.byref(ms), ev.ctypes.data, sol.ctypes.data, reps.ctypes.data))
+    return ev, sol, reps
+
+
+def bond_vectors(st: PreparedState, coord: np.ndarray, type_: np.ndarray, hbond: np.ndarray):
+    """(rexp, rvector, rvector2, disorder) of one receptor (ag_bond_vectors)."""
+    _, _, _, _, setup = st.structs()
+    n = int(coord.shape[0])
+    coord = np.ascontiguousarray(coord, np.float64)
+    type_ = np.ascontiguousarray(type_, np.int32)
+    hbond = np.ascontiguousarray(hbond, np.int32)
+    rexp = np.zeros(n, np.int32)
+    rv = np.zeros((n, 3))
+    rv2 = np.zeros((n, 3))
+    dis = np.zeros(n, np.uint8)
+    check(lib().ag_bond_vectors(C.byref(setup), n, coord.ctypes.data, type_.ctypes.data, hbond.ctypes.data,
+                                rexp.ctypes.data, rv.ctypes.data, rv2.ctypes.data, dis.ctypes.data))
+    return rexp, rv, rv2, dis
 
 
 def xs_radius(type_name: str) -> tuple:

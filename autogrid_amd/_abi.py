@@ -74,6 +74,10 @@ class Pair(C.Structure):
                 ("xA", C.c_int32), ("xB", C.c_int32), ("hbonder", C.c_int32), ("reserved", C.c_int32)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("cell_size", C.c_double), ("cube", C.c_int32), ("reserved", C.c_int32), ("tune_ms", C.c_double)]
+
+
 class Setup(C.Structure):
     _fields_ = [("n_rec_types", C.c_int32), ("n_atom_maps", C.c_int32),
                 ("pairs", C.POINTER(Pair)), ("r_smooth", C.c_double), ("coeff_desolv", C.c_double),
@@ -125,6 +129,19 @@ def lib() -> C.CDLL:
         L.ag_maps_vina.argtypes = L.ag_maps_plain.argtypes
         L.ag_plan_create_potential.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
                                                C.POINTER(Tables), C.c_int, C.POINTER(vp)]
+        L.ag_build_tables.argtypes = [C.POINTER(Setup), C.POINTER(MapSet), vp, vp, vp]
+        L.ag_bond_vectors.argtypes = [C.POINTER(Setup), C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.ag_batch_create.argtypes = [vp, C.c_int, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
+                                      C.POINTER(Tables), C.c_int, C.POINTER(Tune), C.POINTER(vp)]
+        L.ag_plan_layout.argtypes = [vp, vp, vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.ag_plan_conformations.argtypes = [vp]
+        L.ag_plan_conformations.restype = C.c_int
+        L.ag_plan_tune.argtypes = [vp, C.POINTER(Tune)]
+        L.ag_maps_batch.argtypes = [vp, C.c_int, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
+                                    C.POINTER(Tables), vp, vp, vp, C.c_int, C.POINTER(Tune)]
+        for name in ("ag_build_tables", "ag_bond_vectors", "ag_batch_create", "ag_plan_layout", "ag_plan_tune",
+                     "ag_maps_batch"):
+            getattr(L, name).restype = C.c_int
         for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_plan_audit", "ag_context_timing",
                      "ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",
                      "ag_job_views", "ag_run_gpf", "ag_maps_plain", "ag_job_write_maps", "ag_xs_radius",

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -88,16 +89,53 @@ struct __align__(16) NearAtom {   // 96 bytes
 };
 static_assert(sizeof(NearAtom) == 96, "NearAtom layout");
 
-struct NearParams {
+// One receptor conformation on its own grid, as the kernels see it.  A plan
+// holds one; a batch holds many, their atoms, cells, bricks and far-kernel
+// tiles concatenated in conformation order.
+struct ConfDev {
     int n1[3], ne[3];
     double spacing;
     float s128;                   // (float)(spacing * 128)
-    int nb[3], nbricks;
-    // cell list
+    int nb[3];                    // bricks per axis
     double cell_origin[3];
-    double cell_size;
     int nc[3];
-    const int *cell_start;        // [ncells + 1]
+    int cell_base;                // first cell of this conformation in the cell arrays
+    int atom_base, n_atoms;       // atom range in the concatenated atom arrays
+    long long P;
+    long long map_off;            // offset of its [n_maps][P] block in the map output
+    long long pt_off;             // offset of its [P] block in the fmap / cmap outputs
+    int runs_per_row, nruns;      // far-kernel runs of kFarK points along x
+};
+
+// Read-only device tables (conformation records, work prefixes) are read
+// through the constant address space: no kernel writes them, so uniform
+// reads become scalar loads the compiler may repeat instead of holding the
+// values in registers across the hot loops.
+#define AG_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const AG_CONST T *as_const(const T *p) {
+    return (const AG_CONST T *)(p);
+}
+
+// Work lists: item k of a launch belongs to conformation c with
+// prefix[c] <= k < prefix[c + 1].
+__device__ __forceinline__ int find_conf(const AG_CONST int *prefix, int nconf, int item) {
+    int lo = 0, hi = nconf;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct NearParams {
+    // conformations and work distribution
+    const ConfDev *confs;
+    const int *brick_prefix;      // [nconf + 1]
+    int nconf, total_bricks;
+    double cell_size;
+    const int *cell_start;        // [total cells + 1]
     const NearAtom *atoms;        // sorted by (cell, index)
     const double *coord64;        // [n][3] by atom index
     const double *rvec64;         // [n][3]
@@ -120,20 +158,20 @@ struct NearParams {
     int *cmap;                    // constriction count per point, or null (pass 0 only)
     double ccut;                  // constriction distance cutoff (<= 8 A)
     float *maps;
-    long long P;
     int *status;
 };
 
 struct FarParams {
-    int n1[3], ne[3];
-    float s128;
-    int runs_per_row, nruns;
+    const ConfDev *confs;
+    const int *tile_prefix;       // [nconf + 1], tiles of kFarThreads runs
+    int nconf, total_tiles;
+    int *queue;                   // null: one tile per workgroup; else a persistent work counter
     const float4 *atoms;          // (xE, yE, zE, w = q * coeff_estat * 128)
-    int n_atoms;
     const float *tab;             // [tab_len]
     int tab_len;
     float tmax;                   // tab_len - 1
-    float *emap;                  // e map: in = near part, out = total
+    float *maps;                  // e map slot: in = near part, out = total
+    int e_slot;
 };
 
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
@@ -160,27 +198,28 @@ __device__ __forceinline__ float far_rb(float ax, float ay, float az, float px, 
 // cell list
 // ---------------------------------------------------------------------------
 
-struct CellParams {
-    double origin[3];
-    double cs;
-    int nc[3];
-    int n_atoms;
-};
-
-__device__ __forceinline__ int cell_of(const CellParams &c, const double *p) {
-    int ix = (int)floor((p[0] - c.origin[0]) / c.cs);
-    int iy = (int)floor((p[1] - c.origin[1]) / c.cs);
-    int iz = (int)floor((p[2] - c.origin[2]) / c.cs);
+__device__ __forceinline__ int cell_of(const ConfDev &c, double cs, const double *p) {
+    int ix = (int)floor((p[0] - c.cell_origin[0]) / cs);
+    int iy = (int)floor((p[1] - c.cell_origin[1]) / cs);
+    int iz = (int)floor((p[2] - c.cell_origin[2]) / cs);
     if (ix < 0 || iy < 0 || iz < 0 || ix >= c.nc[0] || iy >= c.nc[1] || iz >= c.nc[2]) return -1;
-    return ix + c.nc[0] * (iy + c.nc[1] * iz);
+    return c.cell_base + ix + c.nc[0] * (iy + c.nc[1] * iz);
 }
 
-__global__ void k_cell_count(CellParams c, const double *coord, int *atom_cell, int *count) {
+// atom_conf: conformation of each atom, or null when there is one.
+__global__ void k_cell_count(const ConfDev *confs, const int *atom_conf, int n, double cs, const double *coord,
+                             int *atom_cell, int *count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n_atoms) return;
-    int cell = cell_of(c, coord + 3 * (size_t)i);
+    if (i >= n) return;
+    int cell = cell_of(confs[atom_conf ? atom_conf[i] : 0], cs, coord + 3 * (size_t)i);
     atom_cell[i] = cell;
     if (cell >= 0) atomicAdd(&count[cell], 1);
+}
+
+// Fill kernel for counters and staging (no memset nodes in captured graphs).
+__global__ void k_zero_int(int *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // Exclusive scan of count[0..n) into start[0..n], one workgroup of 1024.
@@ -314,7 +353,9 @@ __device__ __forceinline__ float unif(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-template <int NPLAIN>
+// CUBE selects the lane layout inside the 8x8x4 brick: each wave takes an
+// 8x8x1 slab (false) or a 4x4x4 cube (true) of points.
+template <int NPLAIN, bool CUBE>
 __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     __shared__ int cand[kCandCap];
     __shared__ int hcand[kHCap];
@@ -330,41 +371,63 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     __shared__ int ncand_s, nh_s;
 
     const int tid = threadIdx.x;
-    const int brick = xcd_remap(blockIdx.x, gridDim.x);
-    const int bx = brick % p.nb[0];
-    const int by = (brick / p.nb[0]) % p.nb[1];
-    const int bz = brick / (p.nb[0] * p.nb[1]);
-    const int ix = bx * kBX + (tid % kBX);
-    const int iy = by * kBY + ((tid / kBX) % kBY);
-    const int iz = bz * kBZ + tid / (kBX * kBY);
-    const bool active = ix < p.n1[0] && iy < p.n1[1] && iz < p.n1[2];
-
     for (int k = tid; k < kNearRepsLen; k += kNearThreads) reps_s[k] = p.reps[k];
     for (int k = tid; k <= AG_NEINT; k += kNearThreads) sol_s[k] = p.sol[k];
+    int lx, ly, lz;
+    if (CUBE) {
+        const int wave = tid >> 6, lane = tid & 63;
+        lx = (wave & 1) * 4 + (lane & 3);
+        ly = (wave >> 1) * 4 + ((lane >> 2) & 3);
+        lz = lane >> 4;
+    } else {
+        lx = tid % kBX;
+        ly = (tid / kBX) % kBY;
+        lz = tid / (kBX * kBY);
+    }
+
+    // one workgroup per brick of all conformations.  (A persistent loop
+    // over a work counter keeps more uniform state live than the scalar
+    // register file holds and spills; the far kernel, whose state is
+    // small, is the persistent one.)
+    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    if (item >= p.total_bricks) return;
+    const int ci = find_conf(as_const(p.brick_prefix), p.nconf, item);
+    const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
+    const int brick = item - as_const(p.brick_prefix)[ci];
+    const int n1x = cf.n1[0], n1y = cf.n1[1], n1z = cf.n1[2];
+    const int nex = cf.ne[0], ney = cf.ne[1], nez = cf.ne[2];
+    const double spacing = cf.spacing;
+    const float s128 = cf.s128;
+    const int bx = brick % cf.nb[0];
+    const int by = (brick / cf.nb[0]) % cf.nb[1];
+    const int bz = brick / (cf.nb[0] * cf.nb[1]);
+    const int ix = bx * kBX + lx;
+    const int iy = by * kBY + ly;
+    const int iz = bz * kBZ + lz;
+    const bool active = ix < n1x && iy < n1y && iz < n1z;
 
     // brick box in the grid frame
     const int ix0 = bx * kBX, iy0 = by * kBY, iz0 = bz * kBZ;
-    const int ix1 = min(ix0 + kBX, p.n1[0]) - 1, iy1 = min(iy0 + kBY, p.n1[1]) - 1,
-              iz1 = min(iz0 + kBZ, p.n1[2]) - 1;
-    const double bmin[3] = {(ix0 - p.ne[0]) * p.spacing, (iy0 - p.ne[1]) * p.spacing, (iz0 - p.ne[2]) * p.spacing};
-    const double bmax[3] = {(ix1 - p.ne[0]) * p.spacing, (iy1 - p.ne[1]) * p.spacing, (iz1 - p.ne[2]) * p.spacing};
+    const int ix1 = min(ix0 + kBX, n1x) - 1, iy1 = min(iy0 + kBY, n1y) - 1, iz1 = min(iz0 + kBZ, n1z) - 1;
+    const double bmin[3] = {(ix0 - nex) * spacing, (iy0 - ney) * spacing, (iz0 - nez) * spacing};
+    const double bmax[3] = {(ix1 - nex) * spacing, (iy1 - ney) * spacing, (iz1 - nez) * spacing};
 
     // point, reference arithmetic: c = (double)(i - ne) * spacing
-    const double cx = (double)(ix - p.ne[0]) * p.spacing;
-    const double cy = (double)(iy - p.ne[1]) * p.spacing;
-    const double cz = (double)(iz - p.ne[2]) * p.spacing;
+    const double cx = (double)(ix - nex) * spacing;
+    const double cy = (double)(iy - ney) * spacing;
+    const double cz = (double)(iz - nez) * spacing;
     // the far kernel's point coordinates
-    const float pxE = (float)(ix - p.ne[0]) * p.s128;
-    const float pyE = (float)(iy - p.ne[1]) * p.s128;
-    const float pzE = (float)(iz - p.ne[2]) * p.s128;
+    const float pxE = (float)(ix - nex) * s128;
+    const float pyE = (float)(iy - ney) * s128;
+    const float pzE = (float)(iz - nez) * s128;
 
     // ---- stage candidate atoms: cells overlapping the brick box + radius ----
     int c0[3], c1[3];
     for (int d = 0; d < 3; d++) {
-        c0[d] = (int)floor((bmin[d] - kNearRadius - p.cell_origin[d]) / p.cell_size);
-        c1[d] = (int)floor((bmax[d] + kNearRadius - p.cell_origin[d]) / p.cell_size);
+        c0[d] = (int)floor((bmin[d] - kNearRadius - cf.cell_origin[d]) / p.cell_size);
+        c1[d] = (int)floor((bmax[d] + kNearRadius - cf.cell_origin[d]) / p.cell_size);
         c0[d] = max(c0[d], 0);
-        c1[d] = min(c1[d], p.nc[d] - 1);
+        c1[d] = min(c1[d], cf.nc[d] - 1);
     }
     const int ncx = c1[0] - c0[0] + 1, ncy = c1[1] - c0[1] + 1, ncz = c1[2] - c0[2] + 1;
     const int ncell = (ncx > 0 && ncy > 0 && ncz > 0) ? ncx * ncy * ncz : 0;
@@ -376,7 +439,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
         int cstart = 0, ccount = 0;
         if (ci < ncell) {
             const int gx = c0[0] + ci % ncx, gy = c0[1] + (ci / ncx) % ncy, gz = c0[2] + ci / (ncx * ncy);
-            const int g = gx + p.nc[0] * (gy + p.nc[1] * gz);
+            const int g = cf.cell_base + gx + cf.nc[0] * (gy + cf.nc[1] * gz);
             cstart = p.cell_start[g];
             ccount = p.cell_start[g + 1] - cstart;
         }
@@ -433,8 +496,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     const int ncand = ncand_s, nh = nh_s;
     if (ncand > kCandCap || nh > kHCap) {
         if (tid == 0) atomicOr(p.status, 1);
-        return;
-    }
+    } else {
     // donor-H candidates into LDS for the closest-H search
     for (int h = tid; h < nh; h += kNearThreads) {
         const NearAtom &a = p.atoms[hcand[h]];
@@ -475,6 +537,9 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
 
     float e_near = 0.f, d_acc = 0.f;
     int ccount = 0;
+    int *const cmap = p.cmap ? p.cmap + cf.pt_off : nullptr;
+    float *const maps = p.maps + cf.map_off;
+    const long long P = cf.P;
     const float *tab = p.tab;
     const bool want_e = p.e_slot >= 0;
     // cos and sin of 1.24791 rad, the disordered-hydroxyl offset angle
@@ -509,7 +574,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 cut = r > 8.0f;
             }
             // constriction grid: atoms within the cutoff (main.cpp:2969-2973)
-            if (p.cmap) {
+            if (cmap) {
                 if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
                     double rdx, rdy, rdz;
                     ccount += ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz) <= p.ccut;
@@ -674,19 +739,21 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
         }
     }
 
-    if (!active) return;
-    const long long pt = (long long)ix + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
+    if (active) {
+        const long long pt = (long long)ix + (long long)n1x * ((long long)iy + (long long)n1y * iz);
 #pragma unroll
-    for (int m = 0; m < kMapSlots; m++) {
-        if ((m >= NPLAIN && m < kPlainSlots) || p.out_slot[m] < 0) continue;
-        float v = acc[m];
-        if (m >= kPlainSlots && hmin[m - kPlainSlots] < 999999.f)
-            v += hmin[m - kPlainSlots] + fmaxf(hmax[m - kPlainSlots], 0.f);
-        p.maps[(long long)p.out_slot[m] * p.P + pt] = v;
+        for (int m = 0; m < kMapSlots; m++) {
+            if ((m >= NPLAIN && m < kPlainSlots) || p.out_slot[m] < 0) continue;
+            float v = acc[m];
+            if (m >= kPlainSlots && hmin[m - kPlainSlots] < 999999.f)
+                v += hmin[m - kPlainSlots] + fmaxf(hmax[m - kPlainSlots], 0.f);
+            maps[(long long)p.out_slot[m] * P + pt] = v;
+        }
+        if (want_e) maps[(long long)p.e_slot * P + pt] = e_near;
+        if (cmap) cmap[pt] = ccount;
+        if (p.d_slot >= 0) maps[(long long)p.d_slot * P + pt] = d_acc;
     }
-    if (want_e) p.maps[(long long)p.e_slot * p.P + pt] = e_near;
-    if (p.cmap) p.cmap[pt] = ccount;
-    if (p.d_slot >= 0) p.maps[(long long)p.d_slot * p.P + pt] = d_acc;
+    }  // candidate lists within capacity
 }
 
 __global__ void k_fill(float *out, long long n, float v) {
@@ -851,65 +918,87 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
 template <bool CLAMP>
 __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
     extern __shared__ float T[];
+    __shared__ int item_s;
     for (int i = threadIdx.x; i < p.tab_len; i += kFarThreads) T[i] = p.tab[i];
-    __syncthreads();
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int run = blk * kFarThreads + threadIdx.x;
-    if (run >= p.nruns) return;
-    const int row = run / p.runs_per_row;
-    const int ix0 = (run % p.runs_per_row) * kFarK;
-    const int iy = row % p.n1[1];
-    const int iz = row / p.n1[1];
-    const float py = (float)(iy - p.ne[1]) * p.s128;
-    const float pz = (float)(iz - p.ne[2]) * p.s128;
-    float px[kFarK], s[kFarK], c[kFarK];
-    const long long base = (long long)ix0 + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
-#pragma unroll
-    for (int k = 0; k < kFarK; k++) {
-        px[k] = (float)(ix0 + k - p.ne[0]) * p.s128;
-        s[k] = (ix0 + k < p.n1[0]) ? p.emap[base + k] : 0.f;
-        c[k] = 0.f;
+    int item;
+    if (p.queue) {
+        if (threadIdx.x == 0) item_s = atomicAdd(p.queue, 1);
+        __syncthreads();
+        item = item_s;
+    } else {
+        __syncthreads();
+        item = xcd_remap(blockIdx.x, gridDim.x);
     }
-    const float4 *__restrict__ atoms = p.atoms;
-    for (int a0 = 0; a0 < p.n_atoms; a0 += kFarAtomChunk) {
-        const int a1 = min(a0 + kFarAtomChunk, p.n_atoms);
-        float part[kFarK];
-#pragma unroll
-        for (int k = 0; k < kFarK; k++) part[k] = 0.f;
-#pragma unroll 2
-        for (int a = a0; a < a1; a++) {
-#pragma clang fp contract(off)
-            const float4 at = atoms[a];
-            const float dy = at.y - py, dz = at.z - pz;
-            const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
-            float rs[kFarK];
-            int bin[kFarK];
+    item = __builtin_amdgcn_readfirstlane(item);
+    while (item < p.total_tiles) {
+        const int ci = find_conf(as_const(p.tile_prefix), p.nconf, item);
+        const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
+        const int run = (item - as_const(p.tile_prefix)[ci]) * kFarThreads + threadIdx.x;
+        if (run < cf.nruns) {
+            const int row = run / cf.runs_per_row;
+            const int ix0 = (run % cf.runs_per_row) * kFarK;
+            const int n1x = cf.n1[0], n1y = cf.n1[1];
+            const int iy = row % n1y;
+            const int iz = row / n1y;
+            const float py = (float)(iy - cf.ne[1]) * cf.s128;
+            const float pz = (float)(iz - cf.ne[2]) * cf.s128;
+            float px[kFarK], s[kFarK], c[kFarK];
+            float *const emap = p.maps + cf.map_off + (long long)p.e_slot * cf.P;
+            const long long base = (long long)ix0 + (long long)n1x * ((long long)iy + (long long)n1y * iz);
 #pragma unroll
             for (int k = 0; k < kFarK; k++) {
-                const float dx = at.x - px[k];
-                const float rb2 = __builtin_fmaf(dx, dx, dyz2);
-                rs[k] = __builtin_amdgcn_rsqf(rb2);
-                const float rb = rb2 * rs[k];
-                bin[k] = CLAMP ? (int)fminf(rb, p.tmax) : (int)rb;
+                px[k] = (float)(ix0 + k - cf.ne[0]) * cf.s128;
+                s[k] = (ix0 + k < n1x) ? emap[base + k] : 0.f;
+                c[k] = 0.f;
             }
-            float tv[kFarK];
+            const float4 *__restrict__ atoms = p.atoms + cf.atom_base;
+            const int n_atoms = cf.n_atoms;
+            for (int a0 = 0; a0 < n_atoms; a0 += kFarAtomChunk) {
+                const int a1 = min(a0 + kFarAtomChunk, n_atoms);
+                float part[kFarK];
 #pragma unroll
-            for (int k = 0; k < kFarK; k++) tv[k] = T[bin[k]];
-#pragma unroll
-            for (int k = 0; k < kFarK; k++) part[k] = __builtin_fmaf(at.w, rs[k] * tv[k], part[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < kFarK; k++) {
+                for (int k = 0; k < kFarK; k++) part[k] = 0.f;
+#pragma unroll 2
+                for (int a = a0; a < a1; a++) {
 #pragma clang fp contract(off)
-            const float y = part[k] - c[k];
-            const float t = s[k] + y;
-            c[k] = (t - s[k]) - y;
-            s[k] = t;
-        }
-    }
+                    const float4 at = atoms[a];
+                    const float dy = at.y - py, dz = at.z - pz;
+                    const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
+                    float rs[kFarK];
+                    int bin[kFarK];
 #pragma unroll
-    for (int k = 0; k < kFarK; k++)
-        if (ix0 + k < p.n1[0]) p.emap[base + k] = s[k];
+                    for (int k = 0; k < kFarK; k++) {
+                        const float dx = at.x - px[k];
+                        const float rb2 = __builtin_fmaf(dx, dx, dyz2);
+                        rs[k] = __builtin_amdgcn_rsqf(rb2);
+                        const float rb = rb2 * rs[k];
+                        bin[k] = CLAMP ? (int)fminf(rb, p.tmax) : (int)rb;
+                    }
+                    float tv[kFarK];
+#pragma unroll
+                    for (int k = 0; k < kFarK; k++) tv[k] = T[bin[k]];
+#pragma unroll
+                    for (int k = 0; k < kFarK; k++) part[k] = __builtin_fmaf(at.w, rs[k] * tv[k], part[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < kFarK; k++) {
+#pragma clang fp contract(off)
+                    const float y = part[k] - c[k];
+                    const float t = s[k] + y;
+                    c[k] = (t - s[k]) - y;
+                    s[k] = t;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kFarK; k++)
+                if (ix0 + k < n1x) emap[base + k] = s[k];
+        }
+        if (!p.queue) break;
+        __syncthreads();
+        if (threadIdx.x == 0) item_s = atomicAdd(p.queue, 1);
+        __syncthreads();
+        item = __builtin_amdgcn_readfirstlane(item_s);
+    }
 }
 
 }  // namespace
@@ -943,40 +1032,55 @@ struct DevBuf {
     }
 };
 
+// Near-kernel layout chosen by the autotuner.
+struct Tune {
+    double cell_size = 4.0;     // cell edge of the cell list (A)
+    bool cube = false;          // lanes of a wave on a 4x4x4 cube instead of an 8x8x1 slab
+};
+
 struct ag_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf maps;                       // output staging of ag_maps_plain
     hipEvent_t ev[4] = {};
     float last_ms[4] = {0, 0, 0, 0};   // cell list, near, far, total of the last execute
+    int n_cu = 0;
 };
 
-// A receptor, grid, map set and tables resident on one device.
+// Receptor conformations (one for a plan, many for a batch) on their grids,
+// with a shared map set and tables, resident on one device.
 struct ag_plan {
     ag_ctx *ctx = nullptr;
-    int N = 0, M = 0, T = 0, n_maps = 0, groups = 0;
-    long long P = 0;
-    int elec_index = -1;
+    int nconf = 0, N = 0, M = 0, T = 0, n_maps = 0, groups = 0;
+    bool vina = false, persistent = false;
+    long long P_total = 0;
+    std::vector<ConfDev> confs;
+    std::vector<int> brick_prefix, tile_prefix;
+    int total_cells = 0, total_bricks = 0, total_tiles = 0;
+    Tune tune;
+    double tune_ms = 0.0;              // time the autotuner spent
+    DevBuf confs_d, brick_prefix_d, tile_prefix_d, atom_conf_d, queue_d;
     DevBuf coord, charge, vol, solpar, rvec, rvec2, type, hbond, rexp, disorder;
-    DevBuf atom_cell, cell_count, cell_start, cell_cursor, slot_atom, near_atoms, far_atoms;
+    DevBuf atom_cell, cell_count, cell_start, cell_cursor, slot_atom, near_atoms, far_atoms, fmin_atoms;
     DevBuf tab, sol, reps, fartab, status;
-    CellParams cp{};
-    int ncells = 0;
     AtomSrc src{};
     std::vector<NearParams> near;      // one per pass of 16 map slots
     int nplain = 12;                   // plain slots in use (4, 8 or 12)
-    CovParams cov{};
-    bool run_cov = false;
+    std::vector<int> cov_slots;
+    double cov_pos[3] = {0, 0, 0}, cov_half_width = 1.0, cov_barrier = 0.0;
     bool want_fmap = false, want_cmap = false;
-    ag_grid grid{};
-    double coeff_estat = 0.0;
-    bool run_far_atoms_ready = false;
-    DevBuf fmin_atoms;
-    FminParams fmin{};
-    FarParams far{};
-    bool run_far = false;
-    bool far_clamp = true;
     std::vector<int> zero_slots;       // output maps that are all zero (Vina e/d)
+    FarParams far{};
+    bool run_far = false, far_clamp = true;
+    bool far_atoms_ready = false;
+    double coeff_estat = 0.0;
+    int far_grid = 0;                  // persistent far-kernel grid
+    hipGraphExec_t graph = nullptr;    // captured map set (persistent mode)
+    float *g_maps = nullptr, *g_fmap = nullptr;
+    int32_t *g_cmap = nullptr;
+    ~ag_plan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
 };
 
 namespace agdev {
@@ -991,6 +1095,7 @@ ag_status context_create(ag_ctx **out, int device) {
     c->device = device;
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     *out = c;
     return AG_OK;
 }
@@ -1032,6 +1137,9 @@ static void validate(const ag_receptor *rec, const ag_grid *grid, const ag_mapse
     if (ms->n_maps != n_maps) ag::fail(AG_ERR_BAD_INPUT, "n_maps does not match the map set");
     if ((ms->elec_index >= 0 && ms->elec_index >= n_maps) || (ms->dsolv_index >= 0 && ms->dsolv_index >= n_maps))
         ag::fail(AG_ERR_BAD_INPUT, "e/d map slot out of range");
+    if (N > 0 && (!rec->coord || !rec->charge || !rec->vol || !rec->solpar || !rec->type || !rec->hbond ||
+                  !rec->rexp || !rec->rvector || !rec->rvector2 || !rec->disorder))
+        ag::fail(AG_ERR_BAD_INPUT, "receptor arrays missing");
     for (int i = 0; i < N; i++) {
         if (rec->type[i] < 0 || rec->type[i] >= T) ag::fail(AG_ERR_BAD_INPUT, "atom type index out of range");
         if (rec->hbond[i] < 0 || rec->hbond[i] > 6) ag::fail(AG_ERR_BAD_INPUT, "atom hbond class out of range");
@@ -1045,6 +1153,8 @@ static void validate(const ag_receptor *rec, const ag_grid *grid, const ag_mapse
     }
     if (ms->hydrogen_type < -1 || ms->hydrogen_type >= T) ag::fail(AG_ERR_BAD_INPUT, "hydrogen type index out of range");
     if (M > 0 && !ms->maps) ag::fail(AG_ERR_BAD_INPUT, "map descriptors missing");
+    if (M > 0 && T > 0 && !tab->e_vdw_hb) ag::fail(AG_ERR_BAD_INPUT, "vdW/H-bond tables missing");
+    if (!tab->sol_fn || (ms->dddiel && !tab->r_eps)) ag::fail(AG_ERR_BAD_INPUT, "desolvation/dielectric tables missing");
 }
 
 // Role of map m: the second map of each separate-desolvation pair carries
@@ -1066,53 +1176,161 @@ static int hb_case(int mh, int ah) {
     return HBC_PLAIN;
 }
 
-ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
-                     const ag_tables *tab, int potential) {
+static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStream_t st) {
+#define AG_NEAR(NP, CB) k_near<NP, CB><<<grid, kNearThreads, 0, st>>>(np)
+    if (p.tune.cube) {
+        if (p.nplain == 4) AG_NEAR(4, true);
+        else if (p.nplain == 8) AG_NEAR(8, true);
+        else AG_NEAR(12, true);
+    } else {
+        if (p.nplain == 4) AG_NEAR(4, false);
+        else if (p.nplain == 8) AG_NEAR(8, false);
+        else AG_NEAR(12, false);
+    }
+#undef AG_NEAR
+    HIP_CHECK(hipGetLastError());
+}
+
+// Per-conformation geometry, work prefixes and cell arrays for the plan's
+// cell size.
+static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms) {
+    const int n = p.nconf;
+    const double cs = p.tune.cell_size;
+    p.confs.assign(n, ConfDev{});
+    p.brick_prefix.assign(n + 1, 0);
+    p.tile_prefix.assign(n + 1, 0);
+    long long brick = 0, tile = 0, moff = 0, poff = 0;
+    int cell_base = 0, atom_base = 0;
+    for (int c = 0; c < n; c++) {
+        ConfDev &cf = p.confs[c];
+        memset(&cf, 0, sizeof cf);
+        const ag_grid &g = grids[c];
+        for (int d = 0; d < 3; d++) {
+            cf.n1[d] = g.n1[d];
+            cf.ne[d] = g.ne[d];
+            const double lo = -g.ne[d] * g.spacing - 8.1;
+            const double hi = (g.n1[d] - 1 - g.ne[d]) * g.spacing + 8.1;
+            cf.cell_origin[d] = lo;
+            cf.nc[d] = (int)ceil((hi - lo) / cs) + 1;
+        }
+        cf.spacing = g.spacing;
+        cf.s128 = (float)(g.spacing * 128.0);
+        cf.nb[0] = (g.n1[0] + kBX - 1) / kBX;
+        cf.nb[1] = (g.n1[1] + kBY - 1) / kBY;
+        cf.nb[2] = (g.n1[2] + kBZ - 1) / kBZ;
+        cf.cell_base = cell_base;
+        cf.atom_base = atom_base;
+        cf.n_atoms = natoms[c];
+        cf.P = (long long)g.n1[0] * g.n1[1] * g.n1[2];
+        cf.map_off = moff;
+        cf.pt_off = poff;
+        cf.runs_per_row = (g.n1[0] + kFarK - 1) / kFarK;
+        cf.nruns = cf.runs_per_row * g.n1[1] * g.n1[2];
+        p.brick_prefix[c] = (int)brick;
+        p.tile_prefix[c] = (int)tile;
+        brick += (long long)cf.nb[0] * cf.nb[1] * cf.nb[2];
+        tile += (cf.nruns + kFarThreads - 1) / kFarThreads;
+        const long long cells = (long long)cf.nc[0] * cf.nc[1] * cf.nc[2];
+        if (cell_base + cells > INT32_MAX / 2 || brick > INT32_MAX / 2 || tile > INT32_MAX / 2)
+            ag::fail(AG_ERR_UNSUPPORTED, "batch too large for one plan");
+        cell_base += (int)cells;
+        atom_base += natoms[c];
+        moff += (long long)p.n_maps * cf.P;
+        poff += cf.P;
+    }
+    p.brick_prefix[n] = (int)brick;
+    p.tile_prefix[n] = (int)tile;
+    p.total_bricks = (int)brick;
+    p.total_tiles = (int)tile;
+    p.total_cells = cell_base;
+    p.P_total = poff;
+    hipStream_t st = p.ctx->stream;
+    upload(st, p.confs_d, p.confs.data(), p.confs.size());
+    upload(st, p.brick_prefix_d, p.brick_prefix.data(), p.brick_prefix.size());
+    upload(st, p.tile_prefix_d, p.tile_prefix.data(), p.tile_prefix.size());
+    p.cell_count.get(sizeof(int) * (size_t)p.total_cells);
+    p.cell_start.get(sizeof(int) * ((size_t)p.total_cells + 1));
+    p.cell_cursor.get(sizeof(int) * (size_t)p.total_cells);
+    const ConfDev *d_confs = (const ConfDev *)p.confs_d.ptr;
+    for (auto &q : p.near) {
+        q.confs = d_confs;
+        q.brick_prefix = (const int *)p.brick_prefix_d.ptr;
+        q.nconf = n;
+        q.total_bricks = p.total_bricks;
+        q.cell_size = cs;
+        q.cell_start = (const int *)p.cell_start.ptr;
+    }
+    p.far.confs = d_confs;
+    p.far.tile_prefix = (const int *)p.tile_prefix_d.ptr;
+    p.far.nconf = n;
+    p.far.total_tiles = p.total_tiles;
+    int occ = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ, reinterpret_cast<const void *>(p.far_clamp ? &k_far<true> : &k_far<false>), kFarThreads,
+        sizeof(float) * (size_t)std::max(p.far.tab_len, 1)));
+    p.far_grid = std::max(1, std::min(p.total_tiles, std::max(occ, 1) * p.ctx->n_cu));
+}
+
+// Build a plan over n conformations (a single plan is n = 1).
+static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
+                           const ag_tables *tab, int potential, bool persistent, const Tune &tune) {
     const bool vina = potential == AG_POTENTIAL_VINA;
     if (potential != AG_POTENTIAL_AD4 && !vina) ag::fail(AG_ERR_BAD_INPUT, "unknown potential");
-    validate(rec, grid, ms, tab, vina);
+    if (n < 1) ag::fail(AG_ERR_BAD_INPUT, "a plan needs at least one conformation");
+    for (int i = 0; i < n; i++) validate(&recs[i], &grids[i], ms, tab, vina);
     HIP_CHECK(hipSetDevice(c->device));
     std::unique_ptr<ag_plan> pl(new ag_plan());
     ag_plan &p = *pl;
     p.ctx = c;
-    const int N = p.N = rec->n_atoms;
+    p.nconf = n;
+    p.vina = vina;
+    p.persistent = persistent;
+    p.tune = tune;
     const int M = p.M = ms->n_atom_maps;
     const int T = p.T = tab->n_rec_types;
     p.n_maps = ms->n_maps;
-    p.elec_index = ms->elec_index;
-    p.grid = *grid;
     p.coeff_estat = ms->coeff_estat;
-    const long long P = p.P = (long long)grid->n1[0] * grid->n1[1] * grid->n1[2];
     hipStream_t st = c->stream;
 
-    double *d_coord = upload(st, p.coord, rec->coord, 3 * (size_t)N);
-    double *d_charge = upload(st, p.charge, rec->charge, N);
-    double *d_vol = upload(st, p.vol, rec->vol, N);
-    double *d_solpar = upload(st, p.solpar, rec->solpar, N);
-    double *d_rvec = upload(st, p.rvec, rec->rvector, 3 * (size_t)N);
-    double *d_rvec2 = upload(st, p.rvec2, rec->rvector2, 3 * (size_t)N);
-    int *d_type = upload(st, p.type, rec->type, N);
-    int *d_hbond = upload(st, p.hbond, rec->hbond, N);
-    int *d_rexp = upload(st, p.rexp, rec->rexp, N);
-    unsigned char *d_dis = upload(st, p.disorder, rec->disorder, N);
+    // concatenated atoms
+    std::vector<int> natoms(n);
+    long long Nl = 0;
+    for (int i = 0; i < n; i++) Nl += (natoms[i] = recs[i].n_atoms);
+    if (Nl > INT32_MAX / 4) ag::fail(AG_ERR_UNSUPPORTED, "too many atoms in one batch");
+    const int N = p.N = (int)Nl;
+    std::vector<double> coord(3 * (size_t)N), charge(N), vol(N), solpar(N), rvec(3 * (size_t)N), rvec2(3 * (size_t)N);
+    std::vector<int> type(N), hbond(N), rexp(N), atom_conf(N);
+    std::vector<unsigned char> dis(N);
+    for (int i = 0, o = 0; i < n; o += natoms[i], i++) {
+        const ag_receptor &r = recs[i];
+        const size_t k = natoms[i];
+        if (!k) continue;
+        std::copy(r.coord, r.coord + 3 * k, coord.begin() + 3 * (size_t)o);
+        std::copy(r.charge, r.charge + k, charge.begin() + o);
+        std::copy(r.vol, r.vol + k, vol.begin() + o);
+        std::copy(r.solpar, r.solpar + k, solpar.begin() + o);
+        std::copy(r.rvector, r.rvector + 3 * k, rvec.begin() + 3 * (size_t)o);
+        std::copy(r.rvector2, r.rvector2 + 3 * k, rvec2.begin() + 3 * (size_t)o);
+        std::copy(r.type, r.type + k, type.begin() + o);
+        std::copy(r.hbond, r.hbond + k, hbond.begin() + o);
+        std::copy(r.rexp, r.rexp + k, rexp.begin() + o);
+        std::copy(r.disorder, r.disorder + k, dis.begin() + o);
+        std::fill(atom_conf.begin() + o, atom_conf.begin() + o + k, i);
+    }
+    double *d_coord = upload(st, p.coord, coord.data(), coord.size());
+    double *d_charge = upload(st, p.charge, charge.data(), charge.size());
+    double *d_vol = upload(st, p.vol, vol.data(), vol.size());
+    double *d_solpar = upload(st, p.solpar, solpar.data(), solpar.size());
+    double *d_rvec = upload(st, p.rvec, rvec.data(), rvec.size());
+    double *d_rvec2 = upload(st, p.rvec2, rvec2.data(), rvec2.size());
+    int *d_type = upload(st, p.type, type.data(), type.size());
+    int *d_hbond = upload(st, p.hbond, hbond.data(), hbond.size());
+    int *d_rexp = upload(st, p.rexp, rexp.data(), rexp.size());
+    unsigned char *d_dis = upload(st, p.disorder, dis.data(), dis.size());
+    if (n > 1) upload(st, p.atom_conf_d, atom_conf.data(), atom_conf.size());
     p.src = AtomSrc{d_coord, d_charge, d_vol, d_solpar, d_rvec, d_rvec2, d_type, d_hbond, d_rexp, d_dis,
                     ms->coeff_estat, ms->solpar_q};
-
-    // cell list geometry: the box padded by the staging radius
-    const double cs = 4.0;
-    p.cp.cs = cs;
-    p.cp.n_atoms = N;
-    for (int d = 0; d < 3; d++) {
-        const double lo = -grid->ne[d] * grid->spacing - 8.1;
-        const double hi = (grid->n1[d] - 1 - grid->ne[d]) * grid->spacing + 8.1;
-        p.cp.origin[d] = lo;
-        p.cp.nc[d] = (int)ceil((hi - lo) / cs) + 1;
-    }
-    p.ncells = p.cp.nc[0] * p.cp.nc[1] * p.cp.nc[2];
     p.atom_cell.get(sizeof(int) * (size_t)N);
-    p.cell_count.get(sizeof(int) * (size_t)p.ncells);
-    p.cell_start.get(sizeof(int) * ((size_t)p.ncells + 1));
-    p.cell_cursor.get(sizeof(int) * (size_t)p.ncells);
     p.slot_atom.get(sizeof(int) * (size_t)N);
     p.near_atoms.get(sizeof(NearAtom) * (size_t)N);
     p.status.get(sizeof(int));
@@ -1132,10 +1350,9 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     // the Vina potential every map is a plain table sum: no directional
     // H-bond terms, no desolvation, no e or d map.
     std::vector<int> plain_maps, hb_maps;
-    std::vector<int> cov_maps;
     for (int m = 0; m < M; m++) {
         const int role = map_role(ms, m);
-        if (role == ROLE_COV) cov_maps.push_back(m);
+        if (role == ROLE_COV) p.cov_slots.push_back(m);
         else if (!vina && (role & ROLE_AFF) && ms->maps[m].is_hbonder) hb_maps.push_back(m);
         else plain_maps.push_back(m);
     }
@@ -1168,24 +1385,10 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     }
     p.nplain = nplain_max <= 4 ? 4 : (nplain_max <= 8 ? 8 : 12);
     float *d_tab = upload(st, p.tab, gtab.data(), gtab.size());
+    p.queue_d.get(sizeof(int));
 
-    // near-kernel parameters, one set per pass
     NearParams np;
     memset(&np, 0, sizeof np);
-    for (int d = 0; d < 3; d++) {
-        np.n1[d] = grid->n1[d];
-        np.ne[d] = grid->ne[d];
-        np.cell_origin[d] = p.cp.origin[d];
-        np.nc[d] = p.cp.nc[d];
-    }
-    np.spacing = grid->spacing;
-    np.s128 = (float)(grid->spacing * 128.0);
-    np.nb[0] = (grid->n1[0] + kBX - 1) / kBX;
-    np.nb[1] = (grid->n1[1] + kBY - 1) / kBY;
-    np.nb[2] = (grid->n1[2] + kBZ - 1) / kBZ;
-    np.nbricks = np.nb[0] * np.nb[1] * np.nb[2];
-    np.cell_size = cs;
-    np.cell_start = (const int *)p.cell_start.ptr;
     np.atoms = (const NearAtom *)p.near_atoms.ptr;
     np.coord64 = d_coord;
     np.rvec64 = d_rvec;
@@ -1194,7 +1397,6 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     np.reps = d_reps;
     np.disorder_h = ms->disorder_h;
     np.hydrogen_type = ms->hydrogen_type;
-    np.P = P;
     np.status = (int *)p.status.ptr;
     for (int g = 0; g < p.groups; g++) {
         NearParams q = np;
@@ -1241,64 +1443,43 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
     p.want_fmap = ms->floating_grid != 0;
     p.want_cmap = ms->constriction_grid != 0;
     if (p.want_cmap) p.near[0].ccut = ms->constriction_cutoff;
-    if (p.want_fmap) {
-        float4 *d_fa = (float4 *)p.fmin_atoms.get(sizeof(float4) * (size_t)std::max(N, 1));
-        if (N > 0) k_fmin_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_fa);
+    if (p.want_fmap && N > 0) {
+        float4 *d_fa = (float4 *)p.fmin_atoms.get(sizeof(float4) * (size_t)N);
+        k_fmin_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_fa);
         HIP_CHECK(hipGetLastError());
-        FminParams &fm = p.fmin;
-        for (int d = 0; d < 3; d++) {
-            fm.n1[d] = grid->n1[d];
-            fm.ne[d] = grid->ne[d];
-        }
-        fm.spacing = (float)grid->spacing;
-        fm.runs_per_row = (grid->n1[0] + kFarK - 1) / kFarK;
-        fm.nruns = fm.runs_per_row * grid->n1[1] * grid->n1[2];
-        fm.atoms = d_fa;
-        fm.n_atoms = N;
     }
-
-    // covalent maps
-    if (!cov_maps.empty()) {
-        if (cov_maps.size() > 16) ag::fail(AG_ERR_UNSUPPORTED, "more than 16 covalent maps");
-        CovParams &cv = p.cov;
-        memset(&cv, 0, sizeof cv);
-        for (int d = 0; d < 3; d++) {
-            cv.n1[d] = grid->n1[d];
-            cv.ne[d] = grid->ne[d];
-            cv.pos[d] = ms->cov_pos[d];
-        }
-        cv.spacing = grid->spacing;
-        cv.half_width = ms->cov_half_width;
-        cv.barrier = ms->cov_barrier;
-        cv.n_cov = (int)cov_maps.size();
-        for (int k = 0; k < cv.n_cov; k++) cv.slot[k] = cov_maps[k];
-        cv.P = P;
-        p.run_cov = true;
-    }
-
-    // far kernel (e map): per-atom float4 and the table with bins < 1025 zeroed
+    if (p.cov_slots.size() > 16) ag::fail(AG_ERR_UNSUPPORTED, "more than 16 covalent maps");
+    for (int d = 0; d < 3; d++) p.cov_pos[d] = ms->cov_pos[d];
+    p.cov_half_width = ms->cov_half_width;
+    p.cov_barrier = ms->cov_barrier;
     // Vina: the e and d maps stay zero (main.cpp:2968, 3391)
     if (vina) {
         if (ms->elec_index >= 0) p.zero_slots.push_back(ms->elec_index);
         if (ms->dsolv_index >= 0) p.zero_slots.push_back(ms->dsolv_index);
     }
+
+    // far kernel (e map): per-atom float4 and the table with bins < 1025
+    // zeroed, long enough for the largest bin any (point, atom) pair of any
+    // conformation reaches
     p.run_far = !vina && ms->elec_index >= 0 && N > 0;
     if (p.run_far) {
         float4 *d_far = (float4 *)p.far_atoms.get(sizeof(float4) * (size_t)N);
         k_far_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_charge, ms->coeff_estat, d_far);
         HIP_CHECK(hipGetLastError());
-        p.run_far_atoms_ready = true;
-        // largest bin any (point, atom) pair reaches: farthest box corner
+        p.far_atoms_ready = true;
         double rmax = 0.0;
-        for (int i = 0; i < N; i++) {
-            double q = 0.0;
-            for (int d = 0; d < 3; d++) {
-                const double lo = -grid->ne[d] * grid->spacing, hi = (grid->n1[d] - 1 - grid->ne[d]) * grid->spacing;
-                const double a = rec->coord[3 * (size_t)i + d];
-                const double e = std::max(std::fabs(a - lo), std::fabs(a - hi));
-                q += e * e;
+        for (int i = 0, o = 0; i < n; o += natoms[i], i++) {
+            const ag_grid &grid = grids[i];
+            for (int a = o; a < o + natoms[i]; a++) {
+                double q = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    const double lo = -grid.ne[d] * grid.spacing, hi = (grid.n1[d] - 1 - grid.ne[d]) * grid.spacing;
+                    const double x = coord[3 * (size_t)a + d];
+                    const double e = std::max(std::fabs(x - lo), std::fabs(x - hi));
+                    q += e * e;
+                }
+                rmax = std::max(rmax, q);
             }
-            rmax = std::max(rmax, q);
         }
         const long long need = (long long)(std::sqrt(rmax) * 128.0) + 8;
         p.far_clamp = need > kFarTabMax;
@@ -1308,22 +1489,207 @@ ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, con
         float *d_ft = upload(st, p.fartab, ft.data(), ft.size());
         FarParams &fp = p.far;
         memset(&fp, 0, sizeof fp);
-        for (int d = 0; d < 3; d++) {
-            fp.n1[d] = grid->n1[d];
-            fp.ne[d] = grid->ne[d];
-        }
-        fp.s128 = (float)(grid->spacing * 128.0);
-        fp.runs_per_row = (grid->n1[0] + kFarK - 1) / kFarK;
-        fp.nruns = fp.runs_per_row * grid->n1[1] * grid->n1[2];
         fp.atoms = d_far;
-        fp.n_atoms = N;
         fp.tab = d_ft;
         fp.tab_len = tab_len;
         fp.tmax = (float)(tab_len - 1);
+        fp.e_slot = ms->elec_index;
     }
+    plan_geometry(p, grids, natoms.data());
     HIP_CHECK(hipStreamSynchronize(st));   // host staging vectors go out of scope
     return pl.release();
 }
+
+// Enqueue one map set on the context's stream.  With events, HIP events
+// bracket the cell list, the near-field and the far-field phases.
+static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far) {
+    ag_ctx *c = p->ctx;
+    hipStream_t st = c->stream;
+    const int N = p->N;
+    const int tpb = 256;
+    int *d_atom_cell = (int *)p->atom_cell.ptr, *d_count = (int *)p->cell_count.ptr;
+    int *d_start = (int *)p->cell_start.ptr, *d_cursor = (int *)p->cell_cursor.ptr;
+    int *d_slot_atom = (int *)p->slot_atom.ptr;
+    int *d_status = (int *)p->status.ptr;
+    int *d_queue = (int *)p->queue_d.ptr;
+    const unsigned cgrid = (unsigned)((p->total_cells + tpb - 1) / tpb);
+    if (events) HIP_CHECK(hipEventRecord(c->ev[0], st));
+    k_zero_int<<<cgrid, tpb, 0, st>>>(d_count, p->total_cells);
+    k_zero_int<<<cgrid, tpb, 0, st>>>(d_cursor, p->total_cells);
+    k_zero_int<<<1, 64, 0, st>>>(d_status, 1);
+    k_zero_int<<<1, 64, 0, st>>>(d_queue, 1);
+    if (N > 0)
+        k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>((const ConfDev *)p->confs_d.ptr,
+                                                          p->nconf > 1 ? (const int *)p->atom_conf_d.ptr : nullptr, N,
+                                                          p->tune.cell_size, (const double *)p->coord.ptr,
+                                                          d_atom_cell, d_count);
+    k_scan<<<1, 1024, 0, st>>>(d_count, d_start, p->total_cells);
+    if (N > 0) k_cell_scatter<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(N, d_atom_cell, d_start, d_cursor, d_slot_atom);
+    k_cell_sort<<<cgrid, tpb, 0, st>>>(p->total_cells, d_start, d_slot_atom);
+    if (N > 0)
+        k_gather<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(d_start + p->total_cells, d_slot_atom, p->src,
+                                                      (NearAtom *)p->near_atoms.ptr);
+    HIP_CHECK(hipGetLastError());
+    if (events) HIP_CHECK(hipEventRecord(c->ev[1], st));
+    for (int g = 0; g < p->groups; g++) {
+        NearParams np = p->near[g];
+        np.maps = d_maps;
+        np.cmap = (g == 0 && p->want_cmap) ? d_cmap : nullptr;
+        if (!(np.has_affinity || np.e_slot >= 0 || np.cmap)) continue;
+        launch_near(*p, p->total_bricks, np, st);
+    }
+    for (int ci = 0; ci < p->nconf; ci++) {
+        const ConfDev &cf = p->confs[ci];
+        const unsigned pgrid = (unsigned)((cf.P + 255) / 256);
+        if (p->want_fmap) {
+            FminParams fm;
+            memset(&fm, 0, sizeof fm);
+            for (int d = 0; d < 3; d++) {
+                fm.n1[d] = cf.n1[d];
+                fm.ne[d] = cf.ne[d];
+            }
+            fm.spacing = (float)cf.spacing;
+            fm.runs_per_row = cf.runs_per_row;
+            fm.nruns = cf.nruns;
+            fm.atoms = (const float4 *)p->fmin_atoms.ptr + cf.atom_base;
+            fm.n_atoms = cf.n_atoms;
+            fm.fmap = d_fmap + cf.pt_off;
+            k_fmin<<<(fm.nruns + 255) / 256, 256, 0, st>>>(fm);
+        }
+        if (!p->cov_slots.empty()) {
+            CovParams cv;
+            memset(&cv, 0, sizeof cv);
+            for (int d = 0; d < 3; d++) {
+                cv.n1[d] = cf.n1[d];
+                cv.ne[d] = cf.ne[d];
+                cv.pos[d] = p->cov_pos[d];
+            }
+            cv.spacing = cf.spacing;
+            cv.half_width = p->cov_half_width;
+            cv.barrier = p->cov_barrier;
+            cv.n_cov = (int)p->cov_slots.size();
+            for (int k = 0; k < cv.n_cov; k++) cv.slot[k] = p->cov_slots[k];
+            cv.P = cf.P;
+            cv.maps = d_maps + cf.map_off;
+            k_covalent<<<pgrid, 256, 0, st>>>(cv);
+        }
+        for (int m : p->zero_slots) k_fill<<<pgrid, 256, 0, st>>>(d_maps + cf.map_off + (size_t)m * cf.P, cf.P, 0.f);
+    }
+    HIP_CHECK(hipGetLastError());
+    if (events) HIP_CHECK(hipEventRecord(c->ev[2], st));
+    if (p->run_far && !skip_far) {
+        FarParams fp = p->far;
+        fp.maps = d_maps;
+        fp.queue = p->persistent ? d_queue : nullptr;
+        const int blocks = p->persistent ? p->far_grid : p->total_tiles;
+        if (p->far_clamp) k_far<true><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
+        else k_far<false><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (events) HIP_CHECK(hipEventRecord(c->ev[3], st));
+}
+
+static void check_status(ag_plan *p) {
+    int status = 0;
+    HIP_CHECK(hipMemcpyAsync(&status, p->status.ptr, sizeof(int), hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+    if (status & 1) ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
+}
+
+// Autotuner: times the near-field pass of every candidate layout on a
+// sample of the batch (up to 8 conformations spread over its sizes) and
+// keeps the fastest.  Results differ between cell sizes only in the order
+// of the FP32 sums; for a fixed choice every run is bitwise repeatable.
+static Tune autotune(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
+                     const ag_tables *tab, int potential, double *spent_ms) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        const long long pa = (long long)grids[a].n1[0] * grids[a].n1[1] * grids[a].n1[2] * std::max(recs[a].n_atoms, 1);
+        const long long pb = (long long)grids[b].n1[0] * grids[b].n1[1] * grids[b].n1[2] * std::max(recs[b].n_atoms, 1);
+        return pa != pb ? pa < pb : a < b;
+    });
+    const int ns = std::min(n, 8);
+    std::vector<ag_receptor> srec(ns);
+    std::vector<ag_grid> sgrid(ns);
+    for (int k = 0; k < ns; k++) {
+        const int i = order[ns == 1 ? n - 1 : (int)((long long)k * (n - 1) / (ns - 1))];
+        srec[k] = recs[i];
+        sgrid[k] = grids[i];
+    }
+    long long sp = 0;
+    for (int k = 0; k < ns; k++) sp += (long long)sgrid[k].n1[0] * sgrid[k].n1[1] * sgrid[k].n1[2];
+    DevBuf out;
+    float *d_maps = (float *)out.get(sizeof(float) * (size_t)sp * ms->n_maps + 256);
+    DevBuf extra;
+    char *x = (char *)extra.get(2 * sizeof(float) * (size_t)sp + 256);
+    float *d_fmap = ms->floating_grid ? (float *)x : nullptr;
+    int32_t *d_cmap = ms->constriction_grid ? (int32_t *)(x + sizeof(float) * (size_t)sp) : nullptr;
+    const double cells[3] = {3.0, 4.0, 5.0};
+    Tune best;
+    float best_ms = 1e30f;
+    for (int cube = 0; cube < 2; cube++)
+        for (double cs : cells) {
+            Tune t;
+            t.cell_size = cs;
+            t.cube = cube != 0;
+            std::unique_ptr<ag_plan> p(plan_build(c, ns, srec.data(), sgrid.data(), ms, tab, potential, true, t));
+            float ms_near = 0.f;
+            for (int rep = 0; rep < 2; rep++) {
+                enqueue(p.get(), d_maps, d_fmap, d_cmap, true, true);
+                check_status(p.get());
+                HIP_CHECK(hipEventElapsedTime(&ms_near, c->ev[1], c->ev[2]));
+            }
+            if (ms_near < best_ms) {
+                best_ms = ms_near;
+                best = t;
+            }
+        }
+    *spent_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return best;
+}
+
+ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                     const ag_tables *tab, int potential) {
+    return plan_build(c, 1, rec, grid, ms, tab, potential, false, Tune());
+}
+
+ag_plan *batch_create(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
+                      const ag_tables *tab, int potential, const ag_tune *tune) {
+    Tune t;
+    double spent = 0.0;
+    if (tune) {
+        if (!(tune->cell_size >= 2.0 && tune->cell_size <= 8.0))
+            ag::fail(AG_ERR_BAD_INPUT, "cell size must lie in [2, 8] A");
+        t.cell_size = tune->cell_size;
+        t.cube = tune->cube != 0;
+    } else {
+        for (int i = 0; i < n; i++) validate(&recs[i], &grids[i], ms, tab, potential == AG_POTENTIAL_VINA);
+        t = autotune(c, n, recs, grids, ms, tab, potential, &spent);
+    }
+    ag_plan *p = plan_build(c, n, recs, grids, ms, tab, potential, true, t);
+    p->tune_ms = spent;
+    return p;
+}
+
+void plan_tune(const ag_plan *p, ag_tune *out) {
+    out->cell_size = p->tune.cell_size;
+    out->cube = p->tune.cube ? 1 : 0;
+    out->reserved = 0;
+    out->tune_ms = p->tune_ms;
+}
+
+void plan_layout(const ag_plan *p, long long *map_off, long long *pt_off, long long *map_floats, long long *points) {
+    for (int i = 0; i < p->nconf; i++) {
+        if (map_off) map_off[i] = p->confs[i].map_off;
+        if (pt_off) pt_off[i] = p->confs[i].pt_off;
+    }
+    if (map_floats) *map_floats = p->P_total * p->n_maps;
+    if (points) *points = p->P_total;
+}
+
+int plan_count(const ag_plan *p) { return p->nconf; }
 
 void plan_destroy(ag_plan *p) {
     if (!p) return;
@@ -1332,79 +1698,57 @@ void plan_destroy(ag_plan *p) {
     delete p;
 }
 
-// Cell list + near kernel (+ far kernel) into d_maps ([n_maps][P], device).
-// Asynchronous on the context's stream up to the final status check.
+// One map set (all conformations of the plan) into d_maps
+// ([sum n_maps * P] float, device) and, when asked for, d_fmap / d_cmap
+// ([sum P]).  A single plan runs its kernels directly with phase timing; a
+// batch replays one captured hipGraph (re-captured when the outputs move).
 void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
     if ((p->want_fmap && !d_fmap) || (p->want_cmap && !d_cmap))
         ag::fail(AG_ERR_BAD_INPUT, "the map set asks for a floating/constriction grid but no output was given");
     ag_ctx *c = p->ctx;
     HIP_CHECK(hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    const int N = p->N;
-    const int tpb = 256;
-    int *d_atom_cell = (int *)p->atom_cell.ptr, *d_count = (int *)p->cell_count.ptr;
-    int *d_start = (int *)p->cell_start.ptr, *d_cursor = (int *)p->cell_cursor.ptr;
-    int *d_slot_atom = (int *)p->slot_atom.ptr;
-    int *d_status = (int *)p->status.ptr;
+    if (!p->persistent) {
+        enqueue(p, d_maps, d_fmap, d_cmap, true, false);
+        check_status(p);
+        HIP_CHECK(hipEventElapsedTime(&c->last_ms[0], c->ev[0], c->ev[1]));
+        HIP_CHECK(hipEventElapsedTime(&c->last_ms[1], c->ev[1], c->ev[2]));
+        HIP_CHECK(hipEventElapsedTime(&c->last_ms[2], c->ev[2], c->ev[3]));
+        HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
+        return;
+    }
+    if (!p->graph || p->g_maps != d_maps || p->g_fmap != d_fmap || p->g_cmap != d_cmap) {
+        if (p->graph) {
+            HIP_CHECK(hipGraphExecDestroy(p->graph));
+            p->graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue(p, d_maps, d_fmap, d_cmap, false, false);
+        } catch (...) {
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(st, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(st, &g));
+        hipError_t e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            p->graph = nullptr;
+            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
+        }
+        p->g_maps = d_maps;
+        p->g_fmap = d_fmap;
+        p->g_cmap = d_cmap;
+    }
     HIP_CHECK(hipEventRecord(c->ev[0], st));
-    HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(int) * (size_t)p->ncells, st));
-    HIP_CHECK(hipMemsetAsync(d_cursor, 0, sizeof(int) * (size_t)p->ncells, st));
-    HIP_CHECK(hipMemsetAsync(d_status, 0, sizeof(int), st));
-    if (N > 0) k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(p->cp, (const double *)p->coord.ptr, d_atom_cell, d_count);
-    k_scan<<<1, 1024, 0, st>>>(d_count, d_start, p->ncells);
-    if (N > 0) k_cell_scatter<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(N, d_atom_cell, d_start, d_cursor, d_slot_atom);
-    k_cell_sort<<<(p->ncells + tpb - 1) / tpb, tpb, 0, st>>>(p->ncells, d_start, d_slot_atom);
-    if (N > 0)
-        k_gather<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(d_start + p->ncells, d_slot_atom, p->src,
-                                                      (NearAtom *)p->near_atoms.ptr);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(c->ev[1], st));
-    bool first = true;
-    for (auto &q : p->near) {
-        NearParams np = q;
-        np.maps = d_maps;
-        np.cmap = (first && p->want_cmap) ? d_cmap : nullptr;
-        first = false;
-        if (!(np.has_affinity || np.e_slot >= 0 || np.cmap)) continue;
-        if (p->nplain == 4) k_near<4><<<np.nbricks, kNearThreads, 0, st>>>(np);
-        else if (p->nplain == 8) k_near<8><<<np.nbricks, kNearThreads, 0, st>>>(np);
-        else k_near<12><<<np.nbricks, kNearThreads, 0, st>>>(np);
-        HIP_CHECK(hipGetLastError());
-    }
-    if (p->want_fmap) {
-        FminParams fm = p->fmin;
-        fm.fmap = d_fmap;
-        k_fmin<<<(fm.nruns + 255) / 256, 256, 0, st>>>(fm);
-        HIP_CHECK(hipGetLastError());
-    }
-    if (p->run_cov) {
-        CovParams cv = p->cov;
-        cv.maps = d_maps;
-        k_covalent<<<(unsigned)((p->P + 255) / 256), 256, 0, st>>>(cv);
-        HIP_CHECK(hipGetLastError());
-    }
-    for (int m : p->zero_slots) {
-        k_fill<<<(unsigned)((p->P + 255) / 256), 256, 0, st>>>(d_maps + (size_t)m * p->P, p->P, 0.f);
-        HIP_CHECK(hipGetLastError());
-    }
-    HIP_CHECK(hipEventRecord(c->ev[2], st));
-    if (p->run_far) {
-        FarParams fp = p->far;
-        fp.emap = d_maps + (size_t)p->elec_index * p->P;
-        const int blocks = (fp.nruns + kFarThreads - 1) / kFarThreads;
-        if (p->far_clamp) k_far<true><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
-        else k_far<false><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
-        HIP_CHECK(hipGetLastError());
-    }
+    HIP_CHECK(hipGraphLaunch(p->graph, st));
     HIP_CHECK(hipEventRecord(c->ev[3], st));
-    int status = 0;
-    HIP_CHECK(hipMemcpyAsync(&status, d_status, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    HIP_CHECK(hipEventElapsedTime(&c->last_ms[0], c->ev[0], c->ev[1]));
-    HIP_CHECK(hipEventElapsedTime(&c->last_ms[1], c->ev[1], c->ev[2]));
-    HIP_CHECK(hipEventElapsedTime(&c->last_ms[2], c->ev[2], c->ev[3]));
+    check_status(p);
     HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
-    if (status & 1) ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
+    c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = -1.f;
 }
 
 // One map set, handed to the host map by map: each map is copied into host
@@ -1415,7 +1759,8 @@ void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid,
                          const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap,
                          const std::function<void(int)> &on_ready, int potential) {
     std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab, potential), plan_destroy);
-    const size_t pbytes = sizeof(float) * (size_t)p->P;
+    const long long P = p->P_total;
+    const size_t pbytes = sizeof(float) * (size_t)P;
     const size_t bytes = pbytes * (size_t)p->n_maps;
     float *d_maps = (float *)c->maps.get(bytes + 2 * pbytes + 256);
     char *extra = (char *)d_maps + bytes;
@@ -1423,7 +1768,7 @@ void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid,
     int32_t *d_cmap = ms->constriction_grid ? (int32_t *)(extra + pbytes) : nullptr;
     plan_execute(p.get(), d_maps, d_fmap, d_cmap);
     for (int j = 0; j < p->n_maps; j++) {
-        HIP_CHECK(hipMemcpyAsync(maps_out + (size_t)j * p->P, d_maps + (size_t)j * p->P, pbytes,
+        HIP_CHECK(hipMemcpyAsync(maps_out + (size_t)j * P, d_maps + (size_t)j * P, pbytes,
                                  hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
         on_ready(j);
@@ -1444,30 +1789,32 @@ void plan_audit(ag_plan *p, long long out[4]) {
     ag_ctx *c = p->ctx;
     HIP_CHECK(hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    if (!p->run_far_atoms_ready) {
+    if (p->nconf != 1) ag::fail(AG_ERR_UNSUPPORTED, "the bin audit runs on single-conformation plans");
+    if (!p->far_atoms_ready) {
         float4 *d_far = (float4 *)p->far_atoms.get(sizeof(float4) * (size_t)std::max(p->N, 1));
         if (p->N > 0)
             k_far_atoms<<<(p->N + 255) / 256, 256, 0, st>>>(p->N, (const double *)p->coord.ptr,
                                                             (const double *)p->charge.ptr, p->coeff_estat, d_far);
-        p->run_far_atoms_ready = true;
+        p->far_atoms_ready = true;
     }
     DevBuf acc;
     unsigned long long *d_out = (unsigned long long *)acc.get(4 * sizeof(unsigned long long));
     HIP_CHECK(hipMemsetAsync(d_out, 0, 4 * sizeof(unsigned long long), st));
+    const ConfDev &cf = p->confs[0];
     AuditParams ap;
     memset(&ap, 0, sizeof ap);
     for (int d = 0; d < 3; d++) {
-        ap.n1[d] = p->grid.n1[d];
-        ap.ne[d] = p->grid.ne[d];
+        ap.n1[d] = cf.n1[d];
+        ap.ne[d] = cf.ne[d];
     }
-    ap.spacing = p->grid.spacing;
-    ap.s128 = (float)(p->grid.spacing * 128.0);
+    ap.spacing = cf.spacing;
+    ap.s128 = cf.s128;
     ap.far_atoms = (const float4 *)p->far_atoms.ptr;
     ap.coord = (const double *)p->coord.ptr;
     ap.n_atoms = p->N;
-    ap.P = p->P;
+    ap.P = cf.P;
     ap.out = d_out;
-    k_audit<<<(unsigned)((p->P + 255) / 256), 256, 0, st>>>(ap);
+    k_audit<<<(unsigned)((cf.P + 255) / 256), 256, 0, st>>>(ap);
     HIP_CHECK(hipGetLastError());
     unsigned long long h[4];
     HIP_CHECK(hipMemcpyAsync(h, d_out, sizeof h, hipMemcpyDeviceToHost, st));
@@ -1480,8 +1827,9 @@ void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag
     if ((ms->floating_grid && !fmap) || (ms->constriction_grid && !cmap))
         ag::fail(AG_ERR_BAD_INPUT, "the map set asks for a floating/constriction grid but no output was given");
     std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(plan_create(c, rec, grid, ms, tab, potential), plan_destroy);
-    const size_t bytes = sizeof(float) * (size_t)p->n_maps * p->P;
-    const size_t pbytes = sizeof(float) * (size_t)p->P;
+    const long long P = p->P_total;
+    const size_t bytes = sizeof(float) * (size_t)p->n_maps * P;
+    const size_t pbytes = sizeof(float) * (size_t)P;
     float *d_maps = outputs_on_device ? maps_out : (float *)c->maps.get(bytes + 2 * pbytes + 256);
     float *d_fmap = nullptr;
     int32_t *d_cmap = nullptr;
@@ -1500,6 +1848,40 @@ void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag
         if (d_cmap) HIP_CHECK(hipMemcpyAsync(cmap, d_cmap, pbytes, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
     }
+}
+
+// Batch with host outputs: maps_out[i] receives conformation i's
+// [n_maps][P_i] floats (fmap_out / cmap_out its [P_i] values when asked).
+void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
+                const ag_tables *tab, float *const *maps_out, float *const *fmap_out, int32_t *const *cmap_out,
+                int potential, const ag_tune *tune) {
+    if (!maps_out) ag::fail(AG_ERR_BAD_INPUT, "no map outputs");
+    for (int i = 0; i < n; i++)
+        if (!maps_out[i] || (ms->floating_grid && (!fmap_out || !fmap_out[i])) ||
+            (ms->constriction_grid && (!cmap_out || !cmap_out[i])))
+            ag::fail(AG_ERR_BAD_INPUT, "missing output for a conformation");
+    std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(batch_create(c, n, recs, grids, ms, tab, potential, tune),
+                                                   plan_destroy);
+    const long long P = p->P_total;
+    const size_t bytes = sizeof(float) * (size_t)p->n_maps * P;
+    const size_t pbytes = sizeof(float) * (size_t)P;
+    float *d_maps = (float *)c->maps.get(bytes + 2 * pbytes + 256);
+    char *base = (char *)d_maps + bytes;
+    float *d_fmap = ms->floating_grid ? (float *)base : nullptr;
+    int32_t *d_cmap = ms->constriction_grid ? (int32_t *)(base + pbytes) : nullptr;
+    plan_execute(p.get(), d_maps, d_fmap, d_cmap);
+    for (int i = 0; i < n; i++) {
+        const ConfDev &cf = p->confs[i];
+        HIP_CHECK(hipMemcpyAsync(maps_out[i], d_maps + cf.map_off, sizeof(float) * (size_t)p->n_maps * cf.P,
+                                 hipMemcpyDeviceToHost, c->stream));
+        if (d_fmap)
+            HIP_CHECK(hipMemcpyAsync(fmap_out[i], d_fmap + cf.pt_off, sizeof(float) * (size_t)cf.P,
+                                     hipMemcpyDeviceToHost, c->stream));
+        if (d_cmap)
+            HIP_CHECK(hipMemcpyAsync(cmap_out[i], d_cmap + cf.pt_off, sizeof(int32_t) * (size_t)cf.P,
+                                     hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
 }  // namespace agdev

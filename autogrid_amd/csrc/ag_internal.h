@@ -181,6 +181,14 @@ void context_timing(ag_ctx *ctx, float *ms);
 ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                      const ag_tables *tab, int potential = AG_POTENTIAL_AD4);
 void plan_destroy(ag_plan *p);
+ag_plan *batch_create(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
+                      const ag_tables *tab, int potential, const ag_tune *tune);
+void plan_tune(const ag_plan *p, ag_tune *out);
+void plan_layout(const ag_plan *p, long long *map_off, long long *pt_off, long long *map_floats, long long *points);
+int plan_count(const ag_plan *p);
+void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
+                const ag_tables *tab, float *const *maps_out, float *const *fmap_out, int32_t *const *cmap_out,
+                int potential, const ag_tune *tune);
 void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap);
 void plan_audit(ag_plan *p, long long out[4]);
 void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,

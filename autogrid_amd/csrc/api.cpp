@@ -266,6 +266,87 @@ ag_status ag_maps_vina(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
     });
 }
 
+ag_status ag_build_tables(const ag_setup *setup, const ag_mapset *ms, float *e_vdw_hb, double *sol_fn,
+                          double *r_eps) {
+    if (!setup || !ms || !e_vdw_hb || !sol_fn || (setup->dddiel && !r_eps) ||
+        (setup->n_atom_maps > 0 && (!setup->pairs || !ms->maps)) || setup->n_atom_maps != ms->n_atom_maps) {
+        ag::set_last_error("bad arguments to ag_build_tables");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        ag::build_vdw_tables(*setup, *ms, e_vdw_hb);
+        ag::build_sol_fn(setup->coeff_desolv, sol_fn);
+        if (setup->dddiel) ag::build_r_eps(r_eps);
+    });
+}
+
+ag_status ag_bond_vectors(const ag_setup *setup, int n, const double *coord, const int32_t *type,
+                          const int32_t *hbond, int32_t *rexp, double *rvector, double *rvector2,
+                          uint8_t *disorder) {
+    if (!setup || n < 0 || (n > 0 && (!coord || !type || !hbond || !rexp || !rvector || !rvector2 || !disorder))) {
+        ag::set_last_error("bad arguments to ag_bond_vectors");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        for (int i = 0; i < n; i++) rexp[i] = 0;
+        for (int i = 0; i < 3 * n; i++) rvector[i] = rvector2[i] = 0.0;
+        for (int i = 0; i < n; i++) disorder[i] = 0;
+        ag::build_bond_vectors(*setup, n, coord, type, hbond, rexp, rvector, rvector2, disorder, nullptr);
+    });
+}
+
+ag_status ag_batch_create(ag_ctx *ctx, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
+                          const ag_tables *tab, int potential, const ag_tune *tune, ag_plan **out) {
+    if (!ctx || n < 1 || !recs || !grids || !ms || !tab || !out) {
+        ag::set_last_error("bad arguments to ag_batch_create");
+        return AG_ERR_BAD_INPUT;
+    }
+    *out = nullptr;
+    return guarded([&] { *out = agdev::batch_create(ctx, n, recs, grids, ms, tab, potential, tune); });
+}
+
+ag_status ag_plan_layout(const ag_plan *plan, int64_t *map_offsets, int64_t *point_offsets, int64_t *map_floats,
+                         int64_t *points) {
+    if (!plan) {
+        ag::set_last_error("null plan");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        const int n = agdev::plan_count(plan);
+        std::vector<long long> mo(n), po(n);
+        long long mf = 0, pt = 0;
+        agdev::plan_layout(plan, mo.data(), po.data(), &mf, &pt);
+        for (int i = 0; i < n; i++) {
+            if (map_offsets) map_offsets[i] = mo[i];
+            if (point_offsets) point_offsets[i] = po[i];
+        }
+        if (map_floats) *map_floats = mf;
+        if (points) *points = pt;
+    });
+}
+
+int ag_plan_conformations(const ag_plan *plan) { return plan ? agdev::plan_count(plan) : 0; }
+
+ag_status ag_plan_tune(const ag_plan *plan, ag_tune *out) {
+    if (!plan || !out) {
+        ag::set_last_error("null argument to ag_plan_tune");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::plan_tune(plan, out); });
+}
+
+ag_status ag_maps_batch(ag_ctx *ctx, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
+                        const ag_tables *tab, float *const *maps_out, float *const *fmap_out,
+                        int32_t *const *cmap_out, int potential, const ag_tune *tune) {
+    if (!ctx || n < 1 || !recs || !grids || !ms || !tab || !maps_out) {
+        ag::set_last_error("bad arguments to ag_maps_batch");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        agdev::maps_batch(ctx, n, recs, grids, ms, tab, maps_out, fmap_out, cmap_out, potential, tune);
+    });
+}
+
 ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev) {
     if (!plan || !maps_dev) {
         ag::set_last_error("null argument to ag_plan_execute");

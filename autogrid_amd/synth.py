@@ -219,3 +219,101 @@ def make_workload(name: str, outdir: str, npts=None, z_npts=None, center_offset=
     maps = write_gpf(os.path.join(outdir, gpf), name, rec, present, w.ligand_types, nxyz, centroid)
     return {"dir": outdir, "gpf": gpf, "receptor": rec, "maps": maps, "n_atoms": w.n_atoms,
             "npts": nxyz, "center": centroid, "types": present}
+
+
+# ---------------------------------------------------------------------------
+# Ensemble of receptor conformations (the batched workload)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Ensemble:
+    """Base receptor (parsed by the engine's front end) and its conformations."""
+    base: object                  # PreparedState of the base receptor (tables, map set, pairs)
+    center: np.ndarray            # grid centre of the base GPF (its atoms are stored relative to it)
+    confs: list                   # dicts: n_atoms, coord (absolute, [n,3]), npts, center
+    lig_types: list               # map types in slot order
+    rec_types: list               # receptor types in rec-index order
+
+
+def make_ensemble(outdir: str, n_conf: int = 64, seed: int = 1005, base_atoms: int = 12000,
+                  n_range=(1500, 12000), npts_range=(40, 160), sigma: float = 0.5,
+                  loop_shift: float = 3.0, lig_types=SMALL_TYPES) -> Ensemble:
+    """A seeded ensemble: a base receptor of ``base_atoms`` atoms packed at the
+    full workload's density (5,000 atoms per 45 A sphere); conformation i
+    keeps its first N_i ~ U[n_range] atoms (never cutting an N/O from its
+    hydrogen), moves every heavy atom by a Gaussian jitter of ``sigma`` A
+    per axis (a hydrogen moves with its parent), then shifts 1-3 contiguous
+    index segments of 20-200 atoms rigidly by up to ``loop_shift`` A.  Its
+    box is cubic with an even npts ~ U[npts_range], centred on its
+    centroid.  Maps: ``lig_types`` + e + d."""
+    from . import Job
+
+    density = 5000.0 / (4.0 / 3.0 * math.pi * 45.0 ** 3)
+    radius = (base_atoms / (4.0 / 3.0 * math.pi * density)) ** (1.0 / 3.0)
+    os.makedirs(outdir, exist_ok=True)
+    coords, types, q = make_receptor(base_atoms, radius, seed)
+    rec = "ensemble.pdbqt"
+    write_pdbqt(os.path.join(outdir, rec), coords, types, q)
+    centroid = read_pdbqt_coords(os.path.join(outdir, rec)).mean(axis=0)
+    present = []
+    for t in types:
+        if t not in present:
+            present.append(t)
+    gpf = "ensemble.gpf"
+    write_gpf(os.path.join(outdir, gpf), "ensemble", rec, present, list(lig_types), npts_range[0], centroid)
+    cwd = os.getcwd()
+    os.chdir(outdir)
+    try:
+        base = Job(gpf).prepare().state()
+    finally:
+        os.chdir(cwd)
+    center = np.array([float("%.3f" % v) for v in centroid])
+    absolute = base.coord + center
+    is_h = np.asarray(base.hbond) == 2   # donor hydrogens (HD) of the generator
+    parent = np.arange(base.n_atoms)
+    for i in range(1, base.n_atoms):
+        if is_h[i]:
+            parent[i] = parent[i - 1]
+    confs = []
+    for i in range(n_conf):
+        r = np.random.default_rng([seed, i])
+        n = int(r.integers(n_range[0], n_range[1] + 1))
+        n = min(n, base.n_atoms)
+        while n < base.n_atoms and is_h[n]:
+            n += 1
+        c = absolute[:n].copy()
+        jit = r.normal(0.0, sigma, size=(n, 3))
+        c += jit[parent[:n]]
+        for _ in range(int(r.integers(1, 4))):
+            length = int(r.integers(20, 201))
+            start = int(r.integers(0, max(1, n - length)))
+            v = r.normal(size=3)
+            v *= r.uniform(0.0, loop_shift) / np.linalg.norm(v)
+            c[start:start + length] += v
+        npts = 2 * int(r.integers(npts_range[0] // 2, npts_range[1] // 2 + 1))
+        ctr = np.array([float("%.3f" % v) for v in c.mean(axis=0)])
+        confs.append({"n_atoms": n, "coord": c, "npts": npts, "center": ctr})
+    return Ensemble(base=base, center=center, confs=confs, lig_types=list(lig_types), rec_types=present)
+
+
+def conformation_state(ens: Ensemble, i: int, bonds: bool = True):
+    """PreparedState of conformation i: the base state's atoms cut to N_i,
+    coordinates in its own grid frame, its box, and (``bonds``) its bond
+    vectors from the engine's host precompute."""
+    import dataclasses
+    from . import bond_vectors
+
+    b = ens.base
+    cf = ens.confs[i]
+    n = cf["n_atoms"]
+    npts = cf["npts"]
+    coord = cf["coord"] - cf["center"]
+    st = dataclasses.replace(
+        b, coord=coord, charge=b.charge[:n].copy(), vol=b.vol[:n].copy(), solpar=b.solpar[:n].copy(),
+        type=b.type[:n].copy(), hbond=b.hbond[:n].copy(), rexp=np.zeros(n, np.int32), rvector=np.zeros((n, 3)),
+        rvector2=np.zeros((n, 3)), disorder=np.zeros(n, np.uint8), n1=(npts + 1,) * 3, ne=(npts // 2,) * 3,
+        _keep=[])
+    if bonds:
+        rexp, rv, rv2, dis = bond_vectors(st, st.coord, st.type, st.hbond)
+        st = dataclasses.replace(st, rexp=rexp, rvector=rv, rvector2=rv2, disorder=dis, _keep=[])
+    return st

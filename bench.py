@@ -15,6 +15,13 @@ Also reported on the same JSON line:
   cpu_baseline  the reference autogrid4 (built from its sources with
                 OpenMP) on a z-slab of the same workload on this host,
                 scaled to the whole map set by point count.
+  batched       64 receptor conformations (1,500-12,000 atoms, cubic boxes
+                of 41-161 points, maps A C HD N NA OA + e + d): one batch
+                (host tables once, bond vectors on a thread pool, one graph
+                launch, copies into pinned host memory) against 64
+                sequential single-receptor runs (tables, bond vectors,
+                ag_maps_plain with its uploads and copies), both without
+                ASCII output.
 """
 from __future__ import annotations
 
@@ -68,6 +75,87 @@ def cpu_baseline(workload: str, seconds_target: float = 20.0) -> dict | None:
         shutil.rmtree(d, ignore_errors=True)
 
 
+def batched_leg(reps: int = 2) -> dict:
+    """The 64-conformation ensemble: batched against sequential single runs."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+
+    import autogrid_amd as ag
+    from autogrid_amd import synth
+
+    work = tempfile.mkdtemp(prefix="agens_")
+    try:
+        ens = synth.make_ensemble(work, n_conf=64, seed=1005)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    inputs = [synth.conformation_state(ens, i, bonds=False) for i in range(len(ens.confs))]
+    n_maps = inputs[0].n_maps
+    sizes = [n_maps * st.points for st in inputs]
+    total = sum(sizes)
+    hip = hip_runtime()
+    hbuf = C.c_void_p()
+    if hip.hipHostMalloc(C.byref(hbuf), C.c_size_t(4 * total), 0) != 0:
+        raise RuntimeError("hipHostMalloc failed")
+    try:
+        flat = np.ctypeslib.as_array(C.cast(hbuf, C.POINTER(C.c_float)), shape=(total,))
+        outs, o = [], 0
+        for st, n in zip(inputs, sizes):
+            outs.append(flat[o:o + n].reshape(n_maps, st.n1[2], st.n1[1], st.n1[0]))
+            o += n
+        import dataclasses
+        pool = ThreadPoolExecutor(max_workers=min(16, len(os.sched_getaffinity(0))))
+
+        def with_bonds(st, tables):
+            rexp, rv, rv2, dis = ag.bond_vectors(st, st.coord, st.type, st.hbond)
+            ev, sol, reps_ = tables
+            return dataclasses.replace(st, rexp=rexp, rvector=rv, rvector2=rv2, disorder=dis, e_vdw_hb=ev,
+                                       sol_fn=sol, r_eps=reps_, _keep=[])
+
+        with ag.Context(0) as ctx:
+            def sequential():
+                for st, out in zip(inputs, outs):
+                    tables = ag.build_tables(st)
+                    ctx.maps_plain(with_bonds(st, tables), out=out)
+
+            def batched(tune):
+                tables = ag.build_tables(inputs[0])
+                states = list(pool.map(lambda st: with_bonds(st, tables), inputs))
+                ctx.maps_batch(states, tune=tune, outs=outs)
+
+            # autotune once (the choice is what a tuning cache would hold)
+            t0 = time.perf_counter()
+            tables0 = ag.build_tables(inputs[0])
+            with ctx.batch([with_bonds(st, tables0) for st in inputs]) as b:
+                tune = b.tune()
+            tune_wall = time.perf_counter() - t0
+            sequential()
+            seq_copy = flat.copy()
+            batched(tune)
+            diff = float(np.max(np.abs(flat - seq_copy)))
+            rel = float(np.max(np.abs(flat - seq_copy) / np.maximum(1.0, np.abs(seq_copy))))
+            t_seq, t_bat = [], []
+            for _ in range(reps):
+                t1 = time.perf_counter()
+                sequential()
+                t_seq.append(time.perf_counter() - t1)
+                t1 = time.perf_counter()
+                batched(tune)
+                t_bat.append(time.perf_counter() - t1)
+        pool.shutdown()
+    finally:
+        hip.hipHostFree(hbuf)
+    pairs = sum(st.points * st.n_atoms for st in inputs)
+    tb, ts = min(t_bat), min(t_seq)
+    return {"conformations": len(inputs), "points": sum(st.points for st in inputs), "pair_evals": pairs,
+            "atoms_range": [min(st.n_atoms for st in inputs), max(st.n_atoms for st in inputs)],
+            "npts_range": [min(st.n1[0] - 1 for st in inputs), max(st.n1[0] - 1 for st in inputs)],
+            "batched_s": tb, "sequential_s": ts, "speedup": ts / tb,
+            "batched_pair_evals_per_s": pairs / tb,
+            "tune": {"cell_size": tune["cell_size"], "cube": tune["cube"]}, "tune_ms": tune["tune_ms"],
+            "tune_wall_s": tune_wall, "max_abs_diff_vs_sequential": diff, "max_rel_diff_vs_sequential": rel,
+            "reps": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +164,7 @@ def main():
     ap.add_argument("--workload", default="full", choices=["small", "full", "large"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cli", action="store_true", help="skip the drop-in run with ASCII output")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-conformation batched comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -153,6 +242,7 @@ def main():
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
+    batch = None if args.no_batch else batched_leg()
     base = None if args.no_cpu_baseline else cpu_baseline(args.workload)
     value = ms_per_step / 1000.0
     out = {
@@ -173,6 +263,7 @@ def main():
         "pair_evals_per_s": P * N / (far_avg * 1e-3),
         "bin_audit": audit,
         "wall_with_ascii_output_s": wall_cli if cli_ok else None,
+        "batched": batch,
         "cpu_baseline": base,
     }
     print(json.dumps(out))

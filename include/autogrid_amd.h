@@ -225,7 +225,9 @@ ag_status ag_maps_plain(ag_ctx *ctx, const ag_receptor *rec,
  * then builds the cell list and runs the near-field and far-field kernels
  * into maps_dev, a device buffer of [n_maps][P] floats, and -- when the map
  * set asks for them -- fmap_dev ([P] float) and cmap_dev ([P] int32), else
- * NULL.  A plan belongs to its context and must be destroyed before it. */
+ * NULL.  A plan belongs to its context and must be destroyed before it.
+ * The same calls execute and destroy batches (ag_batch_create), whose
+ * buffers hold every conformation (ag_plan_layout). */
 typedef struct ag_plan ag_plan;
 ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                          const ag_mapset *mapset, const ag_tables *tables, ag_plan **out);
@@ -301,6 +303,58 @@ ag_status ag_maps_vina(ag_ctx *ctx, const ag_receptor *rec,
 ag_status ag_plan_create_potential(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                                    const ag_mapset *mapset, const ag_tables *tables, int potential,
                                    ag_plan **out);
+
+/* ------------------------------------------------------------------------ */
+/* Host precompute (the reference's main.cpp:1607-1639, 2040-2736).         */
+/* ------------------------------------------------------------------------ */
+/* e_vdW_Hb [n_rec_types][n_atom_maps][AG_NEINT] float (main.cpp:2040-2266),
+ * sol_fn [AG_NDIEL] (main.cpp:2272-2276) and, when setup->dddiel,
+ * r_eps [AG_NDIEL] = 332/eps(r) (main.cpp:1607-1639); r_eps may be NULL
+ * otherwise. */
+ag_status ag_build_tables(const ag_setup *setup, const ag_mapset *mapset, float *e_vdw_hb,
+                          double *sol_fn, double *r_eps);
+/* rexp, rvector, rvector2 and disorder flags of one receptor
+ * (main.cpp:2297-2736); arrays as in ag_receptor. */
+ag_status ag_bond_vectors(const ag_setup *setup, int n_atoms, const double *coord,
+                          const int32_t *type, const int32_t *hbond, int32_t *rexp,
+                          double *rvector, double *rvector2, uint8_t *disorder);
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many receptor conformations in one graph launch.               */
+/* ------------------------------------------------------------------------ */
+/* The reference computes one receptor per process run; a batch computes n
+ * receptors (conformations: their own atoms, coordinates and box, sharing
+ * the map set and tables) in one replay of a captured hipGraph whose
+ * near-field and far-field kernels are persistent workgroups pulling
+ * bricks / tiles of all conformations from a work counter.  Outputs are
+ * concatenated: conformation i's maps start at map_offsets[i] floats
+ * ([n_maps][P_i]), its fmap / cmap values at point_offsets[i]. */
+typedef struct ag_tune {
+    double  cell_size;        /* cell-list edge in A, 2..8                  */
+    int32_t cube;             /* 1: near-kernel waves on 4x4x4 point cubes,
+                                 0: on 8x8x1 slabs                          */
+    int32_t reserved;
+    double  tune_ms;          /* out (ag_plan_tune): autotuner time         */
+} ag_tune;
+
+/* tune NULL: the autotuner times the candidate layouts on a sample of the
+ * batch (up to 8 conformations spread over its sizes) and keeps the
+ * fastest.  Results are bitwise repeatable for a fixed layout. */
+ag_status ag_batch_create(ag_ctx *ctx, int n, const ag_receptor *recs, const ag_grid *grids,
+                          const ag_mapset *mapset, const ag_tables *tables, int potential,
+                          const ag_tune *tune, ag_plan **out);
+/* Output layout of a plan or batch: offsets may be NULL. */
+ag_status ag_plan_layout(const ag_plan *plan, int64_t *map_offsets, int64_t *point_offsets,
+                         int64_t *map_floats, int64_t *points);
+int ag_plan_conformations(const ag_plan *plan);
+ag_status ag_plan_tune(const ag_plan *plan, ag_tune *out);
+/* Batch with host outputs: create, execute once, copy conformation i's maps
+ * to maps_out[i] ([n_maps][P_i] floats; fmap_out[i] / cmap_out[i] when the
+ * map set asks for them, else those arrays may be NULL), destroy. */
+ag_status ag_maps_batch(ag_ctx *ctx, int n, const ag_receptor *recs, const ag_grid *grids,
+                        const ag_mapset *mapset, const ag_tables *tables,
+                        float *const *maps_out, float *const *fmap_out,
+                        int32_t *const *cmap_out, int potential, const ag_tune *tune);
 
 /* Map output: the value text of main.cpp:3486-3497 -- "0" when |e| < 1e-4,
  * else "%.3f" of (float)(floor(e*1000+0.5)/1000) -- for n values into buf.

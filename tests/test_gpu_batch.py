@@ -1,0 +1,124 @@
+"""Batched map sets (ag_batch_create / ag_maps_batch) on the GPU.
+
+A batch runs many receptor conformations, with their own atoms and boxes,
+through one captured graph.  For a fixed near-kernel layout with the plain
+path's 4 A cells every per-point sum runs in the same order as in a single
+plan, so each conformation's maps must equal ag_maps_plain's bit for bit;
+other cell sizes reorder the FP32 sums and are held to the oracle under the
+parity tolerance.  Graph replays are checked against the first result.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+import autogrid_amd as ag
+import oracle_lib as ol
+import parity
+from autogrid_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ens(tmp_path_factory):
+    d = tmp_path_factory.mktemp("ens")
+    return synth.make_ensemble(str(d), n_conf=6, seed=3001, base_atoms=3000, n_range=(600, 3000),
+                               npts_range=(16, 40))
+
+
+@pytest.fixture(scope="module")
+def ensemble(ens):
+    return [synth.conformation_state(ens, i) for i in range(len(ens.confs))]
+
+
+def _hip():
+    import ctypes as C
+    return C.CDLL("libamdhip64.so.7")
+
+
+@pytest.mark.parametrize("cube", [False, True])
+def test_batch_equals_plain_bitwise(ensemble, cube):
+    print(f"[gpu] batch vs plain cube={cube}", flush=True)
+    with ag.Context(0) as ctx:
+        got = ctx.maps_batch(ensemble, tune={"cell_size": 4.0, "cube": cube})
+        for st, g in zip(ensemble, got):
+            ref = ctx.maps_plain(st)
+            assert np.array_equal(g.view(np.uint32), ref.view(np.uint32))
+
+
+def test_batch_other_cells_against_oracle(ensemble):
+    print("[gpu] batch cell 3.0 vs oracle", flush=True)
+    with ag.Context(0) as ctx:
+        got = ctx.maps_batch(ensemble, tune={"cell_size": 3.0, "cube": True})
+    for st, g in zip(ensemble, got):
+        nz = st.n1[2]
+        for z in (0, nz // 2, nz - 1):
+            ref = ol.maps(st, z, z + 1)[:, z]
+            for k in range(st.n_maps):
+                ok, mx, rms, bad = parity.within(g[k, z], ref[k])
+                assert ok, f"map {k} plane {z}: max {mx}, {bad} points over tolerance"
+
+
+def test_batch_autotuned_and_graph_replays(ensemble):
+    print("[gpu] batch autotune + replays", flush=True)
+    import ctypes as C
+    hip = _hip()
+    with ag.Context(0) as ctx:
+        with ctx.batch(ensemble) as b:
+            tune = b.tune()
+            print("tune", tune, flush=True)
+            assert tune["cell_size"] in (3.0, 4.0, 5.0)
+            nbytes = b.map_floats * 4
+            d = C.c_void_p()
+            assert hip.hipMalloc(C.byref(d), C.c_size_t(nbytes)) == 0
+            try:
+                b.execute(d.value)
+                first = np.empty(b.map_floats, np.float32)
+                assert hip.hipMemcpy(first.ctypes.data_as(C.c_void_p), d, C.c_size_t(nbytes), 2) == 0
+                for rep in range(3):
+                    # eager work between replays: a plain map set on the same context
+                    ctx.maps_plain(ensemble[rep])
+                    assert hip.hipMemset(d, 0, C.c_size_t(nbytes)) == 0
+                    assert hip.hipDeviceSynchronize() == 0
+                    b.execute(d.value)
+                    again = np.empty_like(first)
+                    assert hip.hipMemcpy(again.ctypes.data_as(C.c_void_p), d, C.c_size_t(nbytes), 2) == 0
+                    assert np.array_equal(first.view(np.uint32), again.view(np.uint32)), f"replay {rep}"
+            finally:
+                hip.hipFree(d)
+            offs = b.map_offsets
+        # the graph's output equals the oracle within tolerance
+        for i, st in enumerate(ensemble[:3]):
+            g = first[offs[i]:offs[i] + st.n_maps * st.points].reshape(st.n_maps, st.n1[2], st.n1[1], st.n1[0])
+            z = st.n1[2] // 3
+            ref = ol.maps(st, z, z + 1)[:, z]
+            for k in range(st.n_maps):
+                assert parity.within(g[k, z], ref[k])[0]
+
+
+def test_batch_distance_grids_and_vina(ens, ensemble):
+    print("[gpu] batch fmap/cmap + vina", flush=True)
+    sts = [dataclasses.replace(s, floating_grid=1, constriction_grid=1, constriction_cutoff=6.5, _keep=[])
+           for s in ensemble[:3]]
+    with ag.Context(0) as ctx:
+        ctx.maps_batch(sts, tune={"cell_size": 4.0, "cube": False})
+        fms, cms = ctx.last_fmaps, ctx.last_cmaps
+        for st, fm, cm in zip(sts, fms, cms):
+            ctx.maps_plain(st)
+            assert np.array_equal(fm, ctx.last_fmap)
+            assert np.array_equal(cm, ctx.last_cmap)
+    # Vina tables of the base map set, shared by every conformation
+    vt = ag.vina_tables(ensemble[0], ens.lig_types, ens.rec_types)
+    vs = [dataclasses.replace(s, e_vdw_hb=vt, _keep=[]) for s in ensemble[:3]]
+    with ag.Context(0) as ctx:
+        got = ctx.maps_batch(vs, tune={"cell_size": 4.0, "cube": True}, potential=ag.AG_POTENTIAL_VINA)
+        for st, g in zip(vs, got):
+            assert np.array_equal(g.view(np.uint32), ctx.maps_vina(st).view(np.uint32))
+    z = vs[0].n1[2] // 2
+    ref = ol.maps_vina(vs[0], z, z + 1)[:, z]
+    for k in range(vs[0].n_maps):
+        assert parity.within(got[0][k, z], ref[k])[0]
+

@@ -25,7 +25,7 @@ from . import _abi
 from ._abi import AgError, check, lib
 
 __all__ = ["AgError", "Context", "Job", "Plan", "PreparedState", "run_gpf", "format_map_values", "xs_radius",
-           "build_tables", "bond_vectors",
+           "build_tables", "bond_vectors", "bond_vectors_batch", "Analytic",
            "vina_tables", "AG_POTENTIAL_AD4", "AG_POTENTIAL_VINA",
            "LIB_PATH", "CLI_PATH"]
 
@@ -76,6 +76,7 @@ class PreparedState:
     r_smooth: float = 0.5
     coeff_desolv: float = 0.1322
     t_types: dict = field(default_factory=dict)
+    weights: tuple = (0.1662, 0.1209, 0.1406, 0.1322)   # library coeff_vdW, hbond, estat, desolv
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -210,7 +211,10 @@ class Job:
             for i in range(M * T):
                 p = setup.pairs[i]
                 pairs.reshape(-1)[i] = (p.cA, p.cB, p.Rij, p.eps, p.xA, p.xB, p.hbonder)
+        w = _abi.Weights()
+        check(lib().ag_job_weights(self._h, C.byref(w)))
         return PreparedState(
+            weights=(w.vdw, w.hbond, w.estat, w.desolv),
             coord=_arr(rec.coord, 3 * n, np.float64, (n, 3)), charge=_arr(rec.charge, n, np.float64),
             vol=_arr(rec.vol, n, np.float64), solpar=_arr(rec.solpar, n, np.float64),
             type=_arr(rec.type, n, np.int32), hbond=_arr(rec.hbond, n, np.int32),
@@ -413,6 +417,74 @@ def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program
         ctx.run_gpf(gpf_path, glg_path, program)
 
 
+class Analytic:
+    """Analytic (differentiable) map model of a state on one GPU
+    (ag_analytic_*): forward maps into device memory, adjoint gradients of
+    a loss whose cotangent dL/dmaps is given in device memory."""
+
+    def __init__(self, ctx: "Context", st: PreparedState):
+        rec, grid, ms, _, setup = st.structs()
+        self._keep = list(st._keep)
+        w = _abi.Weights(*st.weights)
+        h = C.c_void_p()
+        check(lib().ag_analytic_create(ctx.handle, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(setup),
+                                       C.byref(w), C.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        self.n_atoms = st.n_atoms
+        self.n_maps = st.n_maps
+        self.shape = (st.n_maps, st.n1[2], st.n1[1], st.n1[0])
+        self.points = st.points
+        M, T = st.pairs.shape
+        self.pair_shape = (M, T)
+        self.rij0 = np.ascontiguousarray(st.pairs["Rij"], np.float64)
+        wk = np.where(st.pairs["hbonder"] > 0, st.weights[1], st.weights[0])
+        self.eps0 = np.ascontiguousarray(st.pairs["eps"] / wk, np.float64)
+        self.weights0 = np.array(st.weights, np.float64)
+
+    def set(self, coord=None, charge=None, rij=None, eps=None, weights=None) -> None:
+        arrs = []
+
+        def ptr(a, n):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64).reshape(-1)
+            assert a.size == n
+            arrs.append(a)
+            return a.ctypes.data
+        M, T = self.pair_shape
+        w = _abi.Weights(*[float(x) for x in weights]) if weights is not None else None
+        check(lib().ag_analytic_set(self._h, ptr(coord, 3 * self.n_atoms), ptr(charge, self.n_atoms),
+                                    ptr(rij, M * T), ptr(eps, M * T), C.byref(w) if w is not None else None))
+
+    def forward(self, maps_ptr: int) -> None:
+        check(lib().ag_analytic_forward(self._h, C.c_void_p(maps_ptr)))
+
+    def adjoint(self, dmaps_ptr: int) -> dict:
+        M, T = self.pair_shape
+        g = {"coord": np.zeros((self.n_atoms, 3)), "charge": np.zeros(self.n_atoms),
+             "rij": np.zeros((M, T)), "eps": np.zeros((M, T))}
+        w = _abi.Weights()
+        check(lib().ag_analytic_adjoint(self._h, C.c_void_p(dmaps_ptr), g["coord"].ctypes.data,
+                                        g["charge"].ctypes.data, g["rij"].ctypes.data, g["eps"].ctypes.data,
+                                        C.byref(w)))
+        g["weights"] = np.array([w.vdw, w.hbond, w.estat, w.desolv])
+        return g
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ag_analytic_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def build_tables(st: PreparedState):
     """(e_vdw_hb [T,M,1024] float32, sol_fn [8192], r_eps [8192]) of a state's setup (ag_build_tables)."""
     _, _, ms, _, setup = st.structs()
@@ -438,6 +510,55 @@ def bond_vectors(st: PreparedState, coord: np.ndarray, type_: np.ndarray, hbond:
     check(lib().ag_bond_vectors(C.byref(setup), n, coord.ctypes.data, type_.ctypes.data, hbond.ctypes.data,
                                 rexp.ctypes.data, rv.ctypes.data, rv2.ctypes.data, dis.ctypes.data))
     return rexp, rv, rv2, dis
+
+
+def bond_vectors_vjp(st: PreparedState, coord, g_rv, g_rv2) -> np.ndarray:
+    """Gradient of sum(g_rv . rvector + g_rv2 . rvector2) with respect to the
+    coordinates (ag_bond_vectors_vjp)."""
+    _, _, _, _, setup = st.structs()
+    n = int(coord.shape[0])
+    coord = np.ascontiguousarray(coord, np.float64)
+    typ = np.ascontiguousarray(st.type, np.int32)
+    hb = np.ascontiguousarray(st.hbond, np.int32)
+    g_rv = np.ascontiguousarray(g_rv, np.float64)
+    g_rv2 = np.ascontiguousarray(g_rv2, np.float64)
+    out = np.zeros((n, 3))
+    check(lib().ag_bond_vectors_vjp(C.byref(setup), n, coord.ctypes.data, typ.ctypes.data, hb.ctypes.data,
+                                    g_rv.ctypes.data, g_rv2.ctypes.data, out.ctypes.data))
+    return out
+
+
+def bond_vectors_batch(states: list, n_threads: int = 0) -> list:
+    """Copies of the states with bond vectors from ag_bond_vectors_batch
+    (host threads); the first state's setup (types, options) applies to all."""
+    import dataclasses
+    n = len(states)
+    _, _, _, _, setup = states[0].structs()
+    keep0 = list(states[0]._keep)
+    recs = (_abi.Receptor * n)()
+    outs = (_abi.BondOut * n)()
+    new, keep = [], [keep0]
+    for i, st in enumerate(states):
+        na = st.n_atoms
+        coord = np.ascontiguousarray(st.coord, np.float64)
+        typ = np.ascontiguousarray(st.type, np.int32)
+        hb = np.ascontiguousarray(st.hbond, np.int32)
+        rexp = np.zeros(na, np.int32)
+        rv = np.zeros((na, 3))
+        rv2 = np.zeros((na, 3))
+        dis = np.zeros(na, np.uint8)
+        keep.append((coord, typ, hb))
+        r = _abi.Receptor()
+        r.n_atoms = na
+        r.coord = _abi._ptr(coord, C.c_double)
+        r.type = _abi._ptr(typ, C.c_int32)
+        r.hbond = _abi._ptr(hb, C.c_int32)
+        recs[i] = r
+        outs[i] = _abi.BondOut(_abi._ptr(rexp, C.c_int32), _abi._ptr(rv, C.c_double), _abi._ptr(rv2, C.c_double),
+                               _abi._ptr(dis, C.c_uint8))
+        new.append(dataclasses.replace(st, rexp=rexp, rvector=rv, rvector2=rv2, disorder=dis, _keep=[]))
+    check(lib().ag_bond_vectors_batch(C.byref(setup), n, recs, outs, int(n_threads)))
+    return new
 
 
 def xs_radius(type_name: str) -> tuple:

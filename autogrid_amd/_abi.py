@@ -78,6 +78,15 @@ class Tune(C.Structure):
     _fields_ = [("cell_size", C.c_double), ("cube", C.c_int32), ("reserved", C.c_int32), ("tune_ms", C.c_double)]
 
 
+class Weights(C.Structure):
+    _fields_ = [("vdw", C.c_double), ("hbond", C.c_double), ("estat", C.c_double), ("desolv", C.c_double)]
+
+
+class BondOut(C.Structure):
+    _fields_ = [("rexp", C.POINTER(C.c_int32)), ("rvector", C.POINTER(C.c_double)),
+                ("rvector2", C.POINTER(C.c_double)), ("disorder", C.POINTER(C.c_uint8))]
+
+
 class Setup(C.Structure):
     _fields_ = [("n_rec_types", C.c_int32), ("n_atom_maps", C.c_int32),
                 ("pairs", C.POINTER(Pair)), ("r_smooth", C.c_double), ("coeff_desolv", C.c_double),
@@ -139,8 +148,20 @@ def lib() -> C.CDLL:
         L.ag_plan_tune.argtypes = [vp, C.POINTER(Tune)]
         L.ag_maps_batch.argtypes = [vp, C.c_int, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
                                     C.POINTER(Tables), vp, vp, vp, C.c_int, C.POINTER(Tune)]
+        L.ag_bond_vectors_batch.argtypes = [C.POINTER(Setup), C.c_int, C.POINTER(Receptor), C.POINTER(BondOut),
+                                            C.c_int]
+        L.ag_analytic_create.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
+                                         C.POINTER(Setup), C.POINTER(Weights), C.POINTER(vp)]
+        L.ag_analytic_set.argtypes = [vp, vp, vp, vp, vp, C.POINTER(Weights)]
+        L.ag_analytic_forward.argtypes = [vp, vp]
+        L.ag_analytic_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(Weights)]
+        L.ag_analytic_destroy.argtypes = [vp]
+        L.ag_job_weights.argtypes = [vp, C.POINTER(Weights)]
+        L.ag_bond_vectors_vjp.argtypes = [C.POINTER(Setup), C.c_int, vp, vp, vp, vp, vp, vp]
+        L.ag_bond_vectors_vjp.restype = C.c_int
         for name in ("ag_build_tables", "ag_bond_vectors", "ag_batch_create", "ag_plan_layout", "ag_plan_tune",
-                     "ag_maps_batch"):
+                     "ag_maps_batch", "ag_bond_vectors_batch", "ag_analytic_create", "ag_analytic_set",
+                     "ag_analytic_forward", "ag_analytic_adjoint", "ag_analytic_destroy", "ag_job_weights"):
             getattr(L, name).restype = C.c_int
         for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_plan_audit", "ag_context_timing",
                      "ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",

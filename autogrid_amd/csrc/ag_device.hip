@@ -38,14 +38,7 @@
 #include <vector>
 
 #include "ag_internal.h"
-
-#define HIP_CHECK(x)                                                                    \
-    do {                                                                                \
-        hipError_t e_ = (x);                                                            \
-        if (e_ != hipSuccess)                                                           \
-            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + \
-                                        " at " #x);                                     \
-    } while (0)
+#include "ag_devcommon.h"
 
 namespace {
 
@@ -134,6 +127,7 @@ struct NearParams {
     const ConfDev *confs;
     const int *brick_prefix;      // [nconf + 1]
     int nconf, total_bricks;
+    int item_base;                // first brick of this launch
     double cell_size;
     const int *cell_start;        // [total cells + 1]
     const NearAtom *atoms;        // sorted by (cell, index)
@@ -165,11 +159,12 @@ struct FarParams {
     const ConfDev *confs;
     const int *tile_prefix;       // [nconf + 1], tiles of kFarThreads runs
     int nconf, total_tiles;
+    int item_base, item_end;      // tiles [item_base, item_end) of this launch
     int *queue;                   // null: one tile per workgroup; else a persistent work counter
     const float4 *atoms;          // (xE, yE, zE, w = q * coeff_estat * 128)
     const float *tab;             // [tab_len]
     int tab_len;
-    float tmax;                   // tab_len - 1
+    unsigned tmax4;               // 4 * (tab_len - 1): byte offset of the last entry
     float *maps;                  // e map slot: in = near part, out = total
     int e_slot;
 };
@@ -183,15 +178,27 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
     return x * q + min(x, r) + k;
 }
 
-// The far kernel's per-pair bin input, reproduced bit for bit by the near
+// Table offset of the far kernel's bin: with r = rb2 * rsq(rb2) (128 r in
+// bins), y = fma(rb2, rsq(rb2), 2^21 - 1/8) lies in [2^21, 2^22), where the
+// float spacing is 1/4, so the low 23 bits of y hold m = round(4 r - 1/2)
+// (ties to even).  For n <= r < n + 1, 4n - 1/2 <= 4r - 1/2 < 4n + 7/2 puts m
+// in {4n, ..., 4n + 3} (the tie at r = n goes to the even 4n), so
+// m & ~3 = 4 floor(r): the byte offset of bin floor(r) in a float table, in
+// one fma and one and.  Valid for r < 2^21 - 1.
+constexpr float kBinMagic = 2097151.875f;         // 2^21 - 1/8
+__device__ __forceinline__ unsigned bin4_of(float rb2, float rs) {
+    return __float_as_uint(__builtin_fmaf(rb2, rs, kBinMagic)) & 0x7FFFFCu;
+}
+
+// The far kernel's per-pair table offset, reproduced bit for bit by the near
 // kernel when it decides which atoms the far kernel left out.
-__device__ __forceinline__ float far_rb(float ax, float ay, float az, float px, float py, float pz) {
+__device__ __forceinline__ unsigned far_bin4(float ax, float ay, float az, float px, float py, float pz) {
 #pragma clang fp contract(off)
     const float dy = ay - py, dz = az - pz;
     const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
     const float dx = ax - px;
     const float rb2 = __builtin_fmaf(dx, dx, dyz2);
-    return rb2 * __builtin_amdgcn_rsqf(rb2);
+    return bin4_of(rb2, __builtin_amdgcn_rsqf(rb2));
 }
 
 // ---------------------------------------------------------------------------
@@ -389,7 +396,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     // over a work counter keeps more uniform state live than the scalar
     // register file holds and spills; the far kernel, whose state is
     // small, is the persistent one.)
-    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    const int item = p.item_base + xcd_remap(blockIdx.x, gridDim.x);
     if (item >= p.total_bricks) return;
     const int ci = find_conf(as_const(p.brick_prefix), p.nconf, item);
     const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
@@ -587,7 +594,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 bool near_e;
                 if (g < 1022.f) near_e = true;
                 else if (g > 1027.f) near_e = false;
-                else near_e = far_rb(a.xE, a.yE, a.zE, pxE, pyE, pzE) < (float)kFarBinMin;
+                else near_e = far_bin4(a.xE, a.yE, a.zE, pxE, pyE, pzE) < 4u * kFarBinMin;
                 if (near_e) {
                     // 1/max(r, 0.5) with r clamped at APPROX_ZERO
                     const float inv_rmax = fminf(__builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f)), 2.f);
@@ -872,13 +879,13 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
                     pzE = (float)(iz - p.ne[2]) * p.s128;
         for (int a = 0; a < p.n_atoms; a++) {
             const float4 at = p.far_atoms[a];
-            const float rb = far_rb(at.x, at.y, at.z, pxE, pyE, pzE);
+            const int fb = (int)(far_bin4(at.x, at.y, at.z, pxE, pyE, pzE) >> 2);
             double dx, dy, dz;
             const double r64 = ref_dist(p.coord[3 * a], p.coord[3 * a + 1], p.coord[3 * a + 2], cx, cy, cz, dx, dy, dz);
             const int b64 = min((int)(r64 * 128.0), AG_NDIEL - 1);
-            if (rb >= (float)kFarBinMin) {
+            if (fb >= kFarBinMin) {
                 far_pairs++;
-                far_flips += min((int)rb, AG_NDIEL - 1) != b64;
+                far_flips += min(fb, AG_NDIEL - 1) != b64;
             } else {
                 const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
                 const float r2 = fmaxf(__builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf)), 1.0e-12f);
@@ -915,22 +922,25 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
 // reference clamps them (main.cpp:2956); without it the table is extended
 // to the largest bin any pair can reach, every entry past 8191 holding the
 // reference's last value, and the clamp instruction disappears.
-template <bool CLAMP>
+// CAP: LDS table capacity in entries (>= tab_len).  The table is the only
+// LDS variable, so it starts at address 0 and a bin's byte offset is its LDS
+// address; one int after it broadcasts the work counter.
+template <bool CLAMP, int CAP>
 __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
-    extern __shared__ float T[];
-    __shared__ int item_s;
+    __shared__ float T[CAP + 1];
+    int &item_s = reinterpret_cast<int *>(T)[CAP];
     for (int i = threadIdx.x; i < p.tab_len; i += kFarThreads) T[i] = p.tab[i];
     int item;
     if (p.queue) {
-        if (threadIdx.x == 0) item_s = atomicAdd(p.queue, 1);
+        if (threadIdx.x == 0) item_s = p.item_base + atomicAdd(p.queue, 1);
         __syncthreads();
         item = item_s;
     } else {
         __syncthreads();
-        item = xcd_remap(blockIdx.x, gridDim.x);
+        item = p.item_base + xcd_remap(blockIdx.x, gridDim.x);
     }
     item = __builtin_amdgcn_readfirstlane(item);
-    while (item < p.total_tiles) {
+    while (item < p.item_end) {
         const int ci = find_conf(as_const(p.tile_prefix), p.nconf, item);
         const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
         const int run = (item - as_const(p.tile_prefix)[ci]) * kFarThreads + threadIdx.x;
@@ -965,18 +975,19 @@ __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
                     const float dy = at.y - py, dz = at.z - pz;
                     const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
                     float rs[kFarK];
-                    int bin[kFarK];
+                    unsigned off[kFarK];
 #pragma unroll
                     for (int k = 0; k < kFarK; k++) {
                         const float dx = at.x - px[k];
                         const float rb2 = __builtin_fmaf(dx, dx, dyz2);
                         rs[k] = __builtin_amdgcn_rsqf(rb2);
-                        const float rb = rb2 * rs[k];
-                        bin[k] = CLAMP ? (int)fminf(rb, p.tmax) : (int)rb;
+                        off[k] = bin4_of(rb2, rs[k]);
+                        if (CLAMP) off[k] = min(off[k], p.tmax4);
                     }
                     float tv[kFarK];
 #pragma unroll
-                    for (int k = 0; k < kFarK; k++) tv[k] = T[bin[k]];
+                    for (int k = 0; k < kFarK; k++)
+                        tv[k] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(T) + off[k]);
 #pragma unroll
                     for (int k = 0; k < kFarK; k++) part[k] = __builtin_fmaf(at.w, rs[k] * tv[k], part[k]);
                 }
@@ -995,7 +1006,7 @@ __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
         }
         if (!p.queue) break;
         __syncthreads();
-        if (threadIdx.x == 0) item_s = atomicAdd(p.queue, 1);
+        if (threadIdx.x == 0) item_s = p.item_base + atomicAdd(p.queue, 1);
         __syncthreads();
         item = __builtin_amdgcn_readfirstlane(item_s);
     }
@@ -1007,30 +1018,6 @@ __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
 // context, plans and host orchestration
 // ---------------------------------------------------------------------------
 
-struct DevBuf {
-    void *ptr = nullptr;
-    size_t cap = 0;
-    void *get(size_t bytes) {
-        if (bytes == 0) bytes = 16;
-        if (bytes > cap) {
-            if (ptr) HIP_CHECK(hipFree(ptr));
-            ptr = nullptr;
-            cap = 0;
-            if (hipMalloc(&ptr, bytes) != hipSuccess) {
-                ptr = nullptr;
-                ag::fail(AG_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
-            }
-            cap = bytes;
-        }
-        return ptr;
-    }
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() {
-        if (ptr) (void)hipFree(ptr);
-    }
-};
 
 // Near-kernel layout chosen by the autotuner.
 struct Tune {
@@ -1038,14 +1025,6 @@ struct Tune {
     bool cube = false;          // lanes of a wave on a 4x4x4 cube instead of an 8x8x1 slab
 };
 
-struct ag_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    DevBuf maps;                       // output staging of ag_maps_plain
-    hipEvent_t ev[4] = {};
-    float last_ms[4] = {0, 0, 0, 0};   // cell list, near, far, total of the last execute
-    int n_cu = 0;
-};
 
 // Receptor conformations (one for a plan, many for a batch) on their grids,
 // with a shared map set and tables, resident on one device.
@@ -1094,7 +1073,9 @@ ag_status context_create(ag_ctx **out, int device) {
     ag_ctx *c = new ag_ctx();
     c->device = device;
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
+    for (auto &e : c->chunk_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     *out = c;
     return AG_OK;
@@ -1104,9 +1085,13 @@ void context_destroy(ag_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->chunk_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     delete c;
 }
 
@@ -1191,6 +1176,22 @@ static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStr
     HIP_CHECK(hipGetLastError());
 }
 
+// Far-kernel variants by LDS table capacity: 12288 entries (48 KB, three
+// workgroups per CU) when the table fits, else 16384 (64 KB); the clamped
+// table of huge boxes holds the reference's 8192 entries.
+static const void *far_kernel(bool clamp, int tab_len) {
+    if (clamp) return reinterpret_cast<const void *>(&k_far<true, AG_NDIEL>);
+    if (tab_len <= 12288) return reinterpret_cast<const void *>(&k_far<false, 12288>);
+    return reinterpret_cast<const void *>(&k_far<false, kFarTabMax>);
+}
+
+static void launch_far(bool clamp, int tab_len, int blocks, const FarParams &fp, hipStream_t st) {
+    if (clamp) k_far<true, AG_NDIEL><<<blocks, kFarThreads, 0, st>>>(fp);
+    else if (tab_len <= 12288) k_far<false, 12288><<<blocks, kFarThreads, 0, st>>>(fp);
+    else k_far<false, kFarTabMax><<<blocks, kFarThreads, 0, st>>>(fp);
+    HIP_CHECK(hipGetLastError());
+}
+
 // Per-conformation geometry, work prefixes and cell arrays for the plan's
 // cell size.
 static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms) {
@@ -1265,9 +1266,8 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms) {
     p.far.nconf = n;
     p.far.total_tiles = p.total_tiles;
     int occ = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &occ, reinterpret_cast<const void *>(p.far_clamp ? &k_far<true> : &k_far<false>), kFarThreads,
-        sizeof(float) * (size_t)std::max(p.far.tab_len, 1)));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, far_kernel(p.far_clamp, p.far.tab_len),
+                                                           kFarThreads, 0));
     p.far_grid = std::max(1, std::min(p.total_tiles, std::max(occ, 1) * p.ctx->n_cu));
 }
 
@@ -1385,7 +1385,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     }
     p.nplain = nplain_max <= 4 ? 4 : (nplain_max <= 8 ? 8 : 12);
     float *d_tab = upload(st, p.tab, gtab.data(), gtab.size());
-    p.queue_d.get(sizeof(int));
+    p.queue_d.get(sizeof(int) * kMaxChunks);
 
     NearParams np;
     memset(&np, 0, sizeof np);
@@ -1492,7 +1492,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         fp.atoms = d_far;
         fp.tab = d_ft;
         fp.tab_len = tab_len;
-        fp.tmax = (float)(tab_len - 1);
+        fp.tmax4 = 4u * (unsigned)(tab_len - 1);
         fp.e_slot = ms->elec_index;
     }
     plan_geometry(p, grids, natoms.data());
@@ -1500,9 +1500,41 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     return pl.release();
 }
 
-// Enqueue one map set on the context's stream.  With events, HIP events
-// bracket the cell list, the near-field and the far-field phases.
-static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far) {
+
+// Host destinations of a batch: conformation i's maps / fmap / cmap.
+struct HostOut {
+    float *const *maps;
+    float *const *fmap;
+    int32_t *const *cmap;
+};
+
+// Contiguous conformation ranges of about equal work (points x atoms).
+static std::vector<std::pair<int, int>> plan_chunks(const ag_plan &p, int want) {
+    std::vector<std::pair<int, int>> out;
+    const int n = p.nconf;
+    want = std::max(1, std::min({want, n, kMaxChunks}));
+    double total = 0.0;
+    for (const ConfDev &cf : p.confs) total += (double)cf.P * (cf.n_atoms + 64);
+    int c0 = 0;
+    double acc = 0.0;
+    for (int c = 0; c < n; c++) {
+        acc += (double)p.confs[c].P * (p.confs[c].n_atoms + 64);
+        const int k = (int)out.size();
+        if (c == n - 1 || (k < want - 1 && acc >= total * (k + 1) / want)) {
+            out.emplace_back(c0, c + 1);
+            c0 = c + 1;
+        }
+    }
+    return out;
+}
+
+// Enqueue one map set on the context's stream, conformation chunk by chunk.
+// With events, HIP events bracket the cell list, the near-field and the
+// far-field phases (single chunk).  With host outputs, each chunk's maps
+// are copied to the host on the context's copy stream as soon as the chunk
+// is done, while the next chunk computes.
+static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far,
+                    const std::vector<std::pair<int, int>> &chunks, const HostOut *host) {
     ag_ctx *c = p->ctx;
     hipStream_t st = c->stream;
     const int N = p->N;
@@ -1512,12 +1544,14 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
     int *d_slot_atom = (int *)p->slot_atom.ptr;
     int *d_status = (int *)p->status.ptr;
     int *d_queue = (int *)p->queue_d.ptr;
+    const int nchunks = (int)chunks.size();
+    if (nchunks < 1 || nchunks > kMaxChunks) ag::fail(AG_ERR_BAD_INPUT, "bad chunking of the batch");
     const unsigned cgrid = (unsigned)((p->total_cells + tpb - 1) / tpb);
     if (events) HIP_CHECK(hipEventRecord(c->ev[0], st));
     k_zero_int<<<cgrid, tpb, 0, st>>>(d_count, p->total_cells);
     k_zero_int<<<cgrid, tpb, 0, st>>>(d_cursor, p->total_cells);
     k_zero_int<<<1, 64, 0, st>>>(d_status, 1);
-    k_zero_int<<<1, 64, 0, st>>>(d_queue, 1);
+    k_zero_int<<<1, 64, 0, st>>>(d_queue, kMaxChunks);
     if (N > 0)
         k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>((const ConfDev *)p->confs_d.ptr,
                                                           p->nconf > 1 ? (const int *)p->atom_conf_d.ptr : nullptr, N,
@@ -1531,62 +1565,92 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
                                                       (NearAtom *)p->near_atoms.ptr);
     HIP_CHECK(hipGetLastError());
     if (events) HIP_CHECK(hipEventRecord(c->ev[1], st));
-    for (int g = 0; g < p->groups; g++) {
-        NearParams np = p->near[g];
-        np.maps = d_maps;
-        np.cmap = (g == 0 && p->want_cmap) ? d_cmap : nullptr;
-        if (!(np.has_affinity || np.e_slot >= 0 || np.cmap)) continue;
-        launch_near(*p, p->total_bricks, np, st);
-    }
-    for (int ci = 0; ci < p->nconf; ci++) {
-        const ConfDev &cf = p->confs[ci];
-        const unsigned pgrid = (unsigned)((cf.P + 255) / 256);
-        if (p->want_fmap) {
-            FminParams fm;
-            memset(&fm, 0, sizeof fm);
-            for (int d = 0; d < 3; d++) {
-                fm.n1[d] = cf.n1[d];
-                fm.ne[d] = cf.ne[d];
-            }
-            fm.spacing = (float)cf.spacing;
-            fm.runs_per_row = cf.runs_per_row;
-            fm.nruns = cf.nruns;
-            fm.atoms = (const float4 *)p->fmin_atoms.ptr + cf.atom_base;
-            fm.n_atoms = cf.n_atoms;
-            fm.fmap = d_fmap + cf.pt_off;
-            k_fmin<<<(fm.nruns + 255) / 256, 256, 0, st>>>(fm);
+    for (int k = 0; k < nchunks; k++) {
+        const int c0 = chunks[k].first, c1 = chunks[k].second;
+        const int b0 = p->brick_prefix[c0], b1 = p->brick_prefix[c1];
+        for (int g = 0; g < p->groups; g++) {
+            NearParams np = p->near[g];
+            np.maps = d_maps;
+            np.cmap = (g == 0 && p->want_cmap) ? d_cmap : nullptr;
+            np.item_base = b0;
+            if (!(np.has_affinity || np.e_slot >= 0 || np.cmap) || b1 <= b0) continue;
+            launch_near(*p, b1 - b0, np, st);
         }
-        if (!p->cov_slots.empty()) {
-            CovParams cv;
-            memset(&cv, 0, sizeof cv);
-            for (int d = 0; d < 3; d++) {
-                cv.n1[d] = cf.n1[d];
-                cv.ne[d] = cf.ne[d];
-                cv.pos[d] = p->cov_pos[d];
+        for (int ci = c0; ci < c1; ci++) {
+            const ConfDev &cf = p->confs[ci];
+            const unsigned pgrid = (unsigned)((cf.P + 255) / 256);
+            if (p->want_fmap) {
+                FminParams fm;
+                memset(&fm, 0, sizeof fm);
+                for (int d = 0; d < 3; d++) {
+                    fm.n1[d] = cf.n1[d];
+                    fm.ne[d] = cf.ne[d];
+                }
+                fm.spacing = (float)cf.spacing;
+                fm.runs_per_row = cf.runs_per_row;
+                fm.nruns = cf.nruns;
+                fm.atoms = (const float4 *)p->fmin_atoms.ptr + cf.atom_base;
+                fm.n_atoms = cf.n_atoms;
+                fm.fmap = d_fmap + cf.pt_off;
+                k_fmin<<<(fm.nruns + 255) / 256, 256, 0, st>>>(fm);
             }
-            cv.spacing = cf.spacing;
-            cv.half_width = p->cov_half_width;
-            cv.barrier = p->cov_barrier;
-            cv.n_cov = (int)p->cov_slots.size();
-            for (int k = 0; k < cv.n_cov; k++) cv.slot[k] = p->cov_slots[k];
-            cv.P = cf.P;
-            cv.maps = d_maps + cf.map_off;
-            k_covalent<<<pgrid, 256, 0, st>>>(cv);
+            if (!p->cov_slots.empty()) {
+                CovParams cv;
+                memset(&cv, 0, sizeof cv);
+                for (int d = 0; d < 3; d++) {
+                    cv.n1[d] = cf.n1[d];
+                    cv.ne[d] = cf.ne[d];
+                    cv.pos[d] = p->cov_pos[d];
+                }
+                cv.spacing = cf.spacing;
+                cv.half_width = p->cov_half_width;
+                cv.barrier = p->cov_barrier;
+                cv.n_cov = (int)p->cov_slots.size();
+                for (int q = 0; q < cv.n_cov; q++) cv.slot[q] = p->cov_slots[q];
+                cv.P = cf.P;
+                cv.maps = d_maps + cf.map_off;
+                k_covalent<<<pgrid, 256, 0, st>>>(cv);
+            }
+            for (int m : p->zero_slots)
+                k_fill<<<pgrid, 256, 0, st>>>(d_maps + cf.map_off + (size_t)m * cf.P, cf.P, 0.f);
         }
-        for (int m : p->zero_slots) k_fill<<<pgrid, 256, 0, st>>>(d_maps + cf.map_off + (size_t)m * cf.P, cf.P, 0.f);
-    }
-    HIP_CHECK(hipGetLastError());
-    if (events) HIP_CHECK(hipEventRecord(c->ev[2], st));
-    if (p->run_far && !skip_far) {
-        FarParams fp = p->far;
-        fp.maps = d_maps;
-        fp.queue = p->persistent ? d_queue : nullptr;
-        const int blocks = p->persistent ? p->far_grid : p->total_tiles;
-        if (p->far_clamp) k_far<true><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
-        else k_far<false><<<blocks, kFarThreads, sizeof(float) * fp.tab_len, st>>>(fp);
         HIP_CHECK(hipGetLastError());
+        if (events && k == nchunks - 1) HIP_CHECK(hipEventRecord(c->ev[2], st));
+        const int t0 = p->tile_prefix[c0], t1 = p->tile_prefix[c1];
+        if (p->run_far && !skip_far && t1 > t0) {
+            FarParams fp = p->far;
+            fp.maps = d_maps;
+            fp.item_base = t0;
+            fp.item_end = t1;
+            fp.queue = p->persistent ? d_queue + k : nullptr;
+            const int blocks = p->persistent ? std::min(p->far_grid, t1 - t0) : t1 - t0;
+            launch_far(p->far_clamp, fp.tab_len, blocks, fp, st);
+        }
+        if (host) {
+            HIP_CHECK(hipEventRecord(c->chunk_ev[k], st));
+            HIP_CHECK(hipStreamWaitEvent(c->copy_stream, c->chunk_ev[k], 0));
+            for (int ci = c0; ci < c1; ci++) {
+                const ConfDev &cf = p->confs[ci];
+                HIP_CHECK(hipMemcpyAsync(host->maps[ci], d_maps + cf.map_off, sizeof(float) * (size_t)p->n_maps * cf.P,
+                                         hipMemcpyDeviceToHost, c->copy_stream));
+                if (d_fmap)
+                    HIP_CHECK(hipMemcpyAsync(host->fmap[ci], d_fmap + cf.pt_off, sizeof(float) * (size_t)cf.P,
+                                             hipMemcpyDeviceToHost, c->copy_stream));
+                if (d_cmap)
+                    HIP_CHECK(hipMemcpyAsync(host->cmap[ci], d_cmap + cf.pt_off, sizeof(int32_t) * (size_t)cf.P,
+                                             hipMemcpyDeviceToHost, c->copy_stream));
+            }
+        }
+    }
+    if (host) {
+        HIP_CHECK(hipEventRecord(c->chunk_ev[kMaxChunks], c->copy_stream));
+        HIP_CHECK(hipStreamWaitEvent(st, c->chunk_ev[kMaxChunks], 0));
     }
     if (events) HIP_CHECK(hipEventRecord(c->ev[3], st));
+}
+
+static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far) {
+    enqueue(p, d_maps, d_fmap, d_cmap, events, skip_far, {{0, p->nconf}}, nullptr);
 }
 
 static void check_status(ag_plan *p) {
@@ -1850,8 +1914,22 @@ void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag
     }
 }
 
+static bool is_pinned(const void *ptr) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 // Batch with host outputs: maps_out[i] receives conformation i's
 // [n_maps][P_i] floats (fmap_out / cmap_out its [P_i] values when asked).
+// When every destination is pinned host memory, one graph holds the whole
+// pipeline: the conformations run in chunks of about equal work and each
+// chunk's maps are copied to the host on a second stream while the next
+// chunk computes.  Otherwise the graph computes everything and the copies
+// follow it.
 void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
                 const ag_tables *tab, float *const *maps_out, float *const *fmap_out, int32_t *const *cmap_out,
                 int potential, const ag_tune *tune) {
@@ -1869,19 +1947,60 @@ void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids,
     char *base = (char *)d_maps + bytes;
     float *d_fmap = ms->floating_grid ? (float *)base : nullptr;
     int32_t *d_cmap = ms->constriction_grid ? (int32_t *)(base + pbytes) : nullptr;
+    bool pinned = true;
+    for (int i = 0; i < n && pinned; i++)
+        pinned = is_pinned(maps_out[i]) && (!d_fmap || is_pinned(fmap_out[i])) && (!d_cmap || is_pinned(cmap_out[i]));
+    hipStream_t st = c->stream;
+    if (pinned) {
+        const HostOut host{maps_out, fmap_out, cmap_out};
+        const auto chunks = plan_chunks(*p, 8);
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue(p.get(), d_maps, d_fmap, d_cmap, false, false, chunks, &host);
+        } catch (...) {
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(st, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(st, &g));
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess)
+            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
+        hipError_t le = hipEventRecord(c->ev[0], st);
+        if (le == hipSuccess) le = hipGraphLaunch(ge, st);
+        if (le == hipSuccess) le = hipEventRecord(c->ev[3], st);
+        if (le != hipSuccess) {
+            (void)hipGraphExecDestroy(ge);
+            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(le) + " at hipGraphLaunch");
+        }
+        try {
+            check_status(p.get());
+        } catch (...) {
+            (void)hipGraphExecDestroy(ge);
+            throw;
+        }
+        (void)hipGraphExecDestroy(ge);
+        HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
+        c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = -1.f;
+        return;
+    }
     plan_execute(p.get(), d_maps, d_fmap, d_cmap);
     for (int i = 0; i < n; i++) {
         const ConfDev &cf = p->confs[i];
         HIP_CHECK(hipMemcpyAsync(maps_out[i], d_maps + cf.map_off, sizeof(float) * (size_t)p->n_maps * cf.P,
-                                 hipMemcpyDeviceToHost, c->stream));
+                                 hipMemcpyDeviceToHost, st));
         if (d_fmap)
             HIP_CHECK(hipMemcpyAsync(fmap_out[i], d_fmap + cf.pt_off, sizeof(float) * (size_t)cf.P,
-                                     hipMemcpyDeviceToHost, c->stream));
+                                     hipMemcpyDeviceToHost, st));
         if (d_cmap)
             HIP_CHECK(hipMemcpyAsync(cmap_out[i], d_cmap + cf.pt_off, sizeof(int32_t) * (size_t)cf.P,
-                                     hipMemcpyDeviceToHost, c->stream));
+                                     hipMemcpyDeviceToHost, st));
     }
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipStreamSynchronize(st));
 }
 
 }  // namespace agdev

@@ -152,10 +152,20 @@ void build_vina_tables(const ag_setup &s, const ag_mapset &ms, const double *xs_
 void job_prepare_vina(Job &job);
 void build_sol_fn(double coeff_desolv, double *out);
 void build_r_eps(double *out);
+// How one atom's bond vectors were built from atom positions (kind and the
+// atoms involved), so gradients can flow back through them.
+struct BondRecipe {
+    enum Kind : int { kNone = 0, kDiff = 1, kCarbonyl = 2, kEther = 3, kA1 = 4 };
+    int kind = kNone;
+    int a = -1, b = -1, c = -1;   // atoms of the construction (see bond_vectors_vjp)
+    int nb = 0;                   // bonded atoms of an A1 nitrogen
+};
 void build_bond_vectors(const ag_setup &s, int n, const double *coord,
                         const int32_t *type, const int32_t *hbond,
                         int32_t *rexp, double *rvector, double *rvector2,
-                        uint8_t *disorder, FILE *log);
+                        uint8_t *disorder, FILE *log, BondRecipe *recipes = nullptr);
+void bond_vectors_vjp(int n, const double *coord, const BondRecipe *rec, const double *g_rv,
+                      const double *g_rv2, double *g_coord);
 // writer
 int format_map_value(float v, char *out);     // returns bytes incl. '\n'
 // Formats and writes every map file; wait_ready(j), when given, blocks until
@@ -174,6 +184,7 @@ void log_plane_progress(Job &job, double seconds);
 // Device side (ag_device.hip).
 struct ag_ctx;
 struct ag_plan;
+struct ag_analytic;
 namespace agdev {
 ag_status context_create(ag_ctx **out, int device);
 void context_destroy(ag_ctx *ctx);
@@ -197,4 +208,14 @@ void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid,
 void maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                 const ag_mapset *ms, const ag_tables *tab, float *maps,
                 float *fmap, int32_t *cmap, int outputs_on_device, int potential = AG_POTENTIAL_AD4);
+// analytic mode (ag_analytic.hip)
+ag_analytic *analytic_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                             const ag_setup *setup, const double w0[4]);
+void analytic_destroy(ag_analytic *a);
+void analytic_set(ag_analytic *a, const double *coord, const double *charge, const double *rij, const double *eps0,
+                  const double *w);
+void analytic_forward(ag_analytic *a, float *d_maps);
+void analytic_adjoint(ag_analytic *a, const float *d_dmaps, double *g_coord, double *g_charge, double *g_rij,
+                      double *g_eps0, double g_w[4]);
+void analytic_info(const ag_analytic *a, int *n_atoms, int *n_maps, int64_t *points, int *n_pairs);
 }  // namespace agdev

@@ -3,6 +3,7 @@
 // Every entry point catches the library's exceptions and turns them into an
 // ag_status plus a thread-local message; none of them calls exit().
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -295,6 +296,68 @@ ag_status ag_bond_vectors(const ag_setup *setup, int n, const double *coord, con
     });
 }
 
+ag_status ag_bond_vectors_batch(const ag_setup *setup, int n, const ag_receptor *recs, const ag_bond_out *outs,
+                                int n_threads) {
+    if (!setup || n < 0 || (n > 0 && (!recs || !outs))) {
+        ag::set_last_error("bad arguments to ag_bond_vectors_batch");
+        return AG_ERR_BAD_INPUT;
+    }
+    for (int i = 0; i < n; i++) {
+        const ag_receptor &r = recs[i];
+        const ag_bond_out &o = outs[i];
+        if (r.n_atoms < 0 || (r.n_atoms > 0 && (!r.coord || !r.type || !r.hbond || !o.rexp || !o.rvector ||
+                                                !o.rvector2 || !o.disorder))) {
+            ag::set_last_error("bad receptor or output in ag_bond_vectors_batch");
+            return AG_ERR_BAD_INPUT;
+        }
+    }
+    return guarded([&] {
+        int nt = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
+        nt = std::max(1, std::min(nt, n));
+        std::atomic<int> next{0};
+        std::vector<std::string> errors(nt);
+        std::vector<ag_status> codes(nt, AG_OK);
+        auto work = [&](int t) {
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= n) return;
+                try {
+                    ag::build_bond_vectors(*setup, recs[i].n_atoms, recs[i].coord, recs[i].type, recs[i].hbond,
+                                           outs[i].rexp, outs[i].rvector, outs[i].rvector2, outs[i].disorder,
+                                           nullptr);
+                } catch (const ag::Error &e) {
+                    codes[t] = e.status;
+                    errors[t] = e.what();
+                    return;
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+        work(0);
+        for (auto &x : th) x.join();
+        for (int t = 0; t < nt; t++)
+            if (codes[t] != AG_OK) ag::fail(codes[t], errors[t]);
+    });
+}
+
+ag_status ag_bond_vectors_vjp(const ag_setup *setup, int n, const double *coord, const int32_t *type,
+                              const int32_t *hbond, const double *g_rv, const double *g_rv2, double *g_coord) {
+    if (!setup || n < 0 || (n > 0 && (!coord || !type || !hbond || !g_rv || !g_rv2 || !g_coord))) {
+        ag::set_last_error("bad arguments to ag_bond_vectors_vjp");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        std::vector<int32_t> rexp(n);
+        std::vector<double> rv(3 * (size_t)n), rv2(3 * (size_t)n);
+        std::vector<uint8_t> dis(n);
+        std::vector<ag::BondRecipe> rec(n);
+        ag::build_bond_vectors(*setup, n, coord, type, hbond, rexp.data(), rv.data(), rv2.data(), dis.data(),
+                               nullptr, rec.data());
+        ag::bond_vectors_vjp(n, coord, rec.data(), g_rv, g_rv2, g_coord);
+    });
+}
+
 ag_status ag_batch_create(ag_ctx *ctx, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
                           const ag_tables *tab, int potential, const ag_tune *tune, ag_plan **out) {
     if (!ctx || n < 1 || !recs || !grids || !ms || !tab || !out) {
@@ -345,6 +408,76 @@ ag_status ag_maps_batch(ag_ctx *ctx, int n, const ag_receptor *recs, const ag_gr
     return guarded([&] {
         agdev::maps_batch(ctx, n, recs, grids, ms, tab, maps_out, fmap_out, cmap_out, potential, tune);
     });
+}
+
+ag_status ag_analytic_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                             const ag_setup *setup, const ag_weights *w, ag_analytic **out) {
+    if (!ctx || !rec || !grid || !ms || !setup || !w || !out || (rec->n_atoms > 0 && (!rec->coord || !rec->charge ||
+        !rec->vol || !rec->solpar || !rec->type || !rec->hbond)) || (ms->n_atom_maps > 0 && (!ms->maps || !setup->pairs))) {
+        ag::set_last_error("bad arguments to ag_analytic_create");
+        return AG_ERR_BAD_INPUT;
+    }
+    *out = nullptr;
+    const double w0[4] = {w->vdw, w->hbond, w->estat, w->desolv};
+    return guarded([&] { *out = agdev::analytic_create(ctx, rec, grid, ms, setup, w0); });
+}
+
+ag_status ag_analytic_set(ag_analytic *a, const double *coord, const double *charge, const double *rij,
+                          const double *eps, const ag_weights *w) {
+    if (!a) {
+        ag::set_last_error("null analytic plan");
+        return AG_ERR_BAD_INPUT;
+    }
+    double wv[4];
+    if (w) {
+        wv[0] = w->vdw;
+        wv[1] = w->hbond;
+        wv[2] = w->estat;
+        wv[3] = w->desolv;
+    }
+    return guarded([&] { agdev::analytic_set(a, coord, charge, rij, eps, w ? wv : nullptr); });
+}
+
+ag_status ag_analytic_forward(ag_analytic *a, float *maps_dev) {
+    if (!a || !maps_dev) {
+        ag::set_last_error("null argument to ag_analytic_forward");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::analytic_forward(a, maps_dev); });
+}
+
+ag_status ag_analytic_adjoint(ag_analytic *a, const float *dmaps_dev, double *g_coord, double *g_charge,
+                              double *g_rij, double *g_eps, ag_weights *g_w) {
+    if (!a || !dmaps_dev) {
+        ag::set_last_error("null argument to ag_analytic_adjoint");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        double gw[4];
+        agdev::analytic_adjoint(a, dmaps_dev, g_coord, g_charge, g_rij, g_eps, gw);
+        if (g_w) {
+            g_w->vdw = gw[0];
+            g_w->hbond = gw[1];
+            g_w->estat = gw[2];
+            g_w->desolv = gw[3];
+        }
+    });
+}
+
+ag_status ag_analytic_destroy(ag_analytic *a) {
+    return guarded([&] { agdev::analytic_destroy(a); });
+}
+
+ag_status ag_job_weights(const ag_job *j, ag_weights *out) {
+    if (!j || !out) {
+        ag::set_last_error("null argument to ag_job_weights");
+        return AG_ERR_BAD_INPUT;
+    }
+    out->vdw = j->job.lib.coeff_vdW;
+    out->hbond = j->job.lib.coeff_hbond;
+    out->estat = j->job.lib.coeff_estat;
+    out->desolv = j->job.lib.coeff_desolv;
+    return AG_OK;
 }
 
 ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev) {

@@ -139,7 +139,10 @@ struct Bonds {
 
 void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int32_t *type,
                         const int32_t *hbond, int32_t *rexp, double *rvector, double *rvector2,
-                        uint8_t *disorder, FILE *log) {
+                        uint8_t *disorder, FILE *log, BondRecipe *recipes) {
+    if (n <= 0) return;
+    if (recipes)
+        for (int i = 0; i < n; i++) recipes[i] = BondRecipe{};
     const int hydrogen = s.t_hydrogen, nonHB_hydrogen = s.t_nonhb_hydrogen;
     const int carbon = s.t_carbon, arom_carbon = s.t_arom_carbon, oxygen = s.t_oxygen;
     const int nitrogen = s.t_nitrogen, nonHB_nitrogen = s.t_nonhb_nitrogen, sulphur = s.t_sulphur;
@@ -163,6 +166,7 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int
                 if (!distance_le(C(ia), C(ib), 1.378, d, rd)) continue;
                 for (int i = 0; i < 3; i++) rv[i] = d[i];
                 normalize(rv);
+                if (recipes) recipes[ia] = BondRecipe{BondRecipe::kDiff, ib, -1, -1, 0};
                 if (type[ib] == nitrogen) rexp[ia] = 2;
                 else if (type[ib] == oxygen || type[ib] == sulphur) rexp[ia] = 4;
                 else rexp[ia] = 2;
@@ -192,10 +196,12 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int
             if (nbond == 1) {
                 // Carbonyl O=C-X: rvector = O - C, rvector2 = (O-C) x (C-X).
                 sub_normalize(rv, C(ia), C(i1));
+                if (recipes) recipes[ia] = BondRecipe{BondRecipe::kCarbonyl, i1, -1, -1, 0};
                 for (int j = from; j <= to; j++) {
                     if (j == i1 || j == ia) continue;
                     double d[3], rdcx;
                     if (!distance_le(C(i1), C(j), type[j] == hydrogen ? 1.3 : 1.7, d, rdcx)) continue;
+                    if (recipes) recipes[ia].b = j;
                     normalize(d);
                     rv2[0] = rv[1] * d[2] - rv[2] * d[1];
                     rv2[1] = rv[2] * d[0] - rv[0] * d[2];
@@ -210,10 +216,12 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int
                     else ib = i1;
                     bonds.add(ia, ib);
                     sub_normalize(rv, C(ia), C(ib));
+                    if (recipes) recipes[ia] = BondRecipe{BondRecipe::kDiff, ib, -1, -1, 0};
                 } else {
                     // Hydroxyl / ether X1-O-X2: lone-pair plane along X1->X2,
                     // rvector from the X1-X2 line to O.
                     sub_normalize(rv2, C(i2), C(i1));
+                    if (recipes) recipes[ia] = BondRecipe{BondRecipe::kEther, i1, i2, -1, 0};
                     double rdot = 0.;
                     for (int i = 0; i < 3; i++) rdot += (C(ia)[i] - C(i1)[i]) * rv2[i];
                     for (int i = 0; i < 3; i++) rv[i] = C(ia)[i] - ((rdot * rv2[i]) + C(i1)[i]);
@@ -225,13 +233,16 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int
             // negated normalised mean of the (N - X) vectors.
             int nbond = 0;
             double bondv[3][3];
+            int bonded[3] = {-1, -1, -1};
             for (int ib = from; ib <= to; ib++) {
                 if (ib == ia) continue;
                 double rd;
                 const double lim = (type[ib] == hydrogen || type[ib] == nonHB_hydrogen) ? 1.3 : 1.7;
                 if (!distance_le(C(ia), C(ib), lim, bondv[nbond], rd)) continue;
+                bonded[nbond] = ib;
                 if (++nbond == 3) break;
             }
+            if (recipes && nbond > 0) recipes[ia] = BondRecipe{BondRecipe::kA1, bonded[0], bonded[1], bonded[2], nbond};
             if (nbond == 0 && log)
                 fprintf(log, "\nWARNING: Nitrogen atom found with no bonded atoms, atom serial number %d\n", ia + 1);
             double mean[3] = {0, 0, 0};
@@ -401,6 +412,136 @@ void job_prepare_vina(Job &job) {
     build_vina_tables(setup, ms, xs_map.data(), xs_rec.data(), job.e_vdw_hb.data());
     job.vina = true;
     job.logf("\n Using Vina potential for calculation.\n\n");
+}
+
+}  // namespace ag
+
+namespace ag {
+
+// --- gradients through the bond vectors ------------------------------------
+
+static void norm_vjp(const double v[3], const double g[3], double out[3]) {
+    // u = v / |v|: dL/dv = (g - (g.u) u) / |v|
+    const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (len < kApproxZero) {
+        for (int k = 0; k < 3; k++) out[k] = 0.0;
+        return;
+    }
+    const double u[3] = {v[0] / len, v[1] / len, v[2] / len};
+    const double d = g[0] * u[0] + g[1] * u[1] + g[2] * u[2];
+    for (int k = 0; k < 3; k++) out[k] = (g[k] - d * u[k]) / len;
+}
+
+static void unit(const double v[3], double u[3]) {
+    const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int k = 0; k < 3; k++) u[k] = len < kApproxZero ? 0.0 : v[k] / len;
+}
+
+// Adds into g_coord the gradient that flows from dL/drvector and
+// dL/drvector2 through the bond-vector construction of each atom (the
+// geometry of its recipe is held fixed: which atoms are bonded does not
+// change under an infinitesimal move).
+void bond_vectors_vjp(int n, const double *coord, const BondRecipe *rec, const double *g_rv,
+                      const double *g_rv2, double *g_coord) {
+    auto X = [&](int i) { return coord + 3 * (size_t)i; };
+    auto G = [&](int i) { return g_coord + 3 * (size_t)i; };
+    for (int i = 0; i < n; i++) {
+        const BondRecipe &r = rec[i];
+        const double *gr = g_rv + 3 * (size_t)i, *gr2 = g_rv2 + 3 * (size_t)i;
+        switch (r.kind) {
+            case BondRecipe::kDiff: {   // rv = N(x_i - x_a)
+                double v[3], gv[3];
+                for (int k = 0; k < 3; k++) v[k] = X(i)[k] - X(r.a)[k];
+                norm_vjp(v, gr, gv);
+                for (int k = 0; k < 3; k++) {
+                    G(i)[k] += gv[k];
+                    G(r.a)[k] -= gv[k];
+                }
+            } break;
+            case BondRecipe::kCarbonyl: {   // rv = N(x_i - x_a); rv2 = N(rv x N(x_a - x_b))
+                double v[3], u[3];
+                for (int k = 0; k < 3; k++) v[k] = X(i)[k] - X(r.a)[k];
+                unit(v, u);
+                double gu[3] = {gr[0], gr[1], gr[2]};
+                if (r.b >= 0) {
+                    double tv[3], t[3];
+                    for (int k = 0; k < 3; k++) tv[k] = X(r.a)[k] - X(r.b)[k];
+                    unit(tv, t);
+                    const double c[3] = {u[1] * t[2] - u[2] * t[1], u[2] * t[0] - u[0] * t[2], u[0] * t[1] - u[1] * t[0]};
+                    double gc[3];
+                    norm_vjp(c, gr2, gc);
+                    // c = u x t: du = t x gc, dt = gc x u
+                    gu[0] += t[1] * gc[2] - t[2] * gc[1];
+                    gu[1] += t[2] * gc[0] - t[0] * gc[2];
+                    gu[2] += t[0] * gc[1] - t[1] * gc[0];
+                    const double gt[3] = {gc[1] * u[2] - gc[2] * u[1], gc[2] * u[0] - gc[0] * u[2],
+                                          gc[0] * u[1] - gc[1] * u[0]};
+                    double gtv[3];
+                    norm_vjp(tv, gt, gtv);
+                    for (int k = 0; k < 3; k++) {
+                        G(r.a)[k] += gtv[k];
+                        G(r.b)[k] -= gtv[k];
+                    }
+                }
+                double gv[3];
+                norm_vjp(v, gu, gv);
+                for (int k = 0; k < 3; k++) {
+                    G(i)[k] += gv[k];
+                    G(r.a)[k] -= gv[k];
+                }
+            } break;
+            case BondRecipe::kEther: {   // rv2 = N(x_b - x_a); rv = N(o - ((o - a).rv2 rv2 + a))
+                const double *o = X(i), *a = X(r.a), *b = X(r.b);
+                double w[3], rv2[3];
+                for (int k = 0; k < 3; k++) w[k] = b[k] - a[k];
+                unit(w, rv2);
+                double oa[3], rdot = 0.0;
+                for (int k = 0; k < 3; k++) {
+                    oa[k] = o[k] - a[k];
+                    rdot += oa[k] * rv2[k];
+                }
+                double v[3];
+                for (int k = 0; k < 3; k++) v[k] = o[k] - (rdot * rv2[k] + a[k]);
+                double gv[3];
+                norm_vjp(v, gr, gv);
+                double go[3], ga[3], grv2[3] = {gr2[0], gr2[1], gr2[2]};
+                double grdot = 0.0;
+                for (int k = 0; k < 3; k++) {
+                    go[k] = gv[k];
+                    ga[k] = -gv[k];
+                    grdot -= gv[k] * rv2[k];
+                    grv2[k] -= rdot * gv[k];
+                }
+                for (int k = 0; k < 3; k++) {
+                    go[k] += grdot * rv2[k];
+                    ga[k] -= grdot * rv2[k];
+                    grv2[k] += grdot * oa[k];
+                }
+                double gw[3];
+                norm_vjp(w, grv2, gw);
+                for (int k = 0; k < 3; k++) {
+                    G(i)[k] += go[k];
+                    G(r.a)[k] += ga[k] - gw[k];
+                    G(r.b)[k] += gw[k];
+                }
+            } break;
+            case BondRecipe::kA1: {   // rv = -N(sum_k (x_i - x_k) / nb)
+                const int idx[3] = {r.a, r.b, r.c};
+                double m[3] = {0, 0, 0};
+                for (int q = 0; q < r.nb; q++)
+                    for (int k = 0; k < 3; k++) m[k] += (X(i)[k] - X(idx[q])[k]) / r.nb;
+                const double ng[3] = {-gr[0], -gr[1], -gr[2]};
+                double gm[3];
+                norm_vjp(m, ng, gm);
+                for (int k = 0; k < 3; k++) {
+                    G(i)[k] += gm[k];
+                    for (int q = 0; q < r.nb; q++) G(idx[q])[k] -= gm[k] / r.nb;
+                }
+            } break;
+            default:
+                break;
+        }
+    }
 }
 
 }  // namespace ag

@@ -18,6 +18,8 @@ AD4.1_bound parameters, smooth 0.5, distance-dependent dielectric):
   A C N NA OA S SA HD F Cl Br I P + e + d.
 * ``large``  -- 20,000 atoms in an 80 A sphere, npts 256 (257^3 points),
   same maps as ``full``.
+* ``analytic`` -- the ``full`` receptor, npts 64 (65^3 points), maps
+  A C HD N OA + e + d (the analytic-mode workload).
 """
 from __future__ import annotations
 
@@ -45,6 +47,8 @@ WORKLOADS = {
     "small": Workload("small", 1000, 28.0, 40, SMALL_TYPES, 1002),
     "full": Workload("full", 5000, 45.0, 126, FULL_TYPES, 1003),
     "large": Workload("large", 20000, 80.0, 256, FULL_TYPES, 1004),
+    # the full workload's receptor on a 65^3 box for the analytic mode
+    "analytic": Workload("analytic", 5000, 45.0, 64, ["A", "C", "HD", "N", "OA"], 1003),
 }
 
 

@@ -319,6 +319,26 @@ ag_status ag_bond_vectors(const ag_setup *setup, int n_atoms, const double *coor
                           const int32_t *type, const int32_t *hbond, int32_t *rexp,
                           double *rvector, double *rvector2, uint8_t *disorder);
 
+/* Bond vectors of n receptors on n_threads host threads (<= 0: all cores);
+ * receptor i reads recs[i].n_atoms / coord / type / hbond and writes
+ * outs[i]. */
+typedef struct ag_bond_out {
+    int32_t *rexp;
+    double  *rvector;
+    double  *rvector2;
+    uint8_t *disorder;
+} ag_bond_out;
+ag_status ag_bond_vectors_batch(const ag_setup *setup, int n, const ag_receptor *recs,
+                                const ag_bond_out *outs, int n_threads);
+
+/* Gradient through the bond vectors: adds to g_coord [n][3] the gradient
+ * of sum(g_rvector . rvector + g_rvector2 . rvector2) with respect to the
+ * coordinates, the bonding found at coord held fixed. */
+ag_status ag_bond_vectors_vjp(const ag_setup *setup, int n_atoms, const double *coord,
+                              const int32_t *type, const int32_t *hbond,
+                              const double *g_rvector, const double *g_rvector2,
+                              double *g_coord);
+
 /* ------------------------------------------------------------------------ */
 /* Batches: many receptor conformations in one graph launch.               */
 /* ------------------------------------------------------------------------ */
@@ -350,11 +370,64 @@ int ag_plan_conformations(const ag_plan *plan);
 ag_status ag_plan_tune(const ag_plan *plan, ag_tune *out);
 /* Batch with host outputs: create, execute once, copy conformation i's maps
  * to maps_out[i] ([n_maps][P_i] floats; fmap_out[i] / cmap_out[i] when the
- * map set asks for them, else those arrays may be NULL), destroy. */
+ * map set asks for them, else those arrays may be NULL), destroy.  When all
+ * outputs are pinned host memory (hipHostMalloc), the copies are part of
+ * the graph and overlap the computation chunk by chunk. */
 ag_status ag_maps_batch(ag_ctx *ctx, int n, const ag_receptor *recs, const ag_grid *grids,
                         const ag_mapset *mapset, const ag_tables *tables,
                         float *const *maps_out, float *const *fmap_out,
                         int32_t *const *cmap_out, int potential, const ag_tune *tune);
+
+/* ------------------------------------------------------------------------ */
+/* Analytic (differentiable) mode.                                           */
+/* ------------------------------------------------------------------------ */
+/* The map set evaluated from the functional forms behind the reference's
+ * tables (main.cpp:2040-2276) at the exact distance, with the reference's
+ * per-point assembly (main.cpp:2805-3431):
+ *   E_mt(r) = min(EINTCLAMP, Phi(s(r))), Phi = eps/(a-b) (b (R/s)^a - a (R/s)^b)
+ *             (or cA/s^a - cB/s^b for nbp_coeffs pairs), s(r) the continuous
+ *             form of the min-window smoothing: r + w left of the minimum r*,
+ *             r* within w of it, r - w right of it, w = floor(64 r_smooth)/128;
+ *   sol(r)  = coeff_desolv exp(-r^2 / 25.92);
+ *   e       = sum over all atoms q coeff_estat (332/eps(r)) / max(r, 0.5);
+ * every 8 A term multiplied by S(r) = 1 - u^3 (10 - 15u + 6u^2),
+ * u = clamp((r - 7.5)/0.5, 0, 1), instead of the hard cutoff.  Bond vectors
+ * are rebuilt from the coordinates (main.cpp:2297-2664) and differentiated
+ * with the bonding held fixed.  disorder_h is not part of this mode.
+ *
+ * Parameters: coordinates [N][3], charges [N], per (map, receptor type)
+ * Rij and unweighted eps [n_atom_maps][n_rec_types], and the four AD4
+ * weights; the effective eps of a pair is the vdW (12-6) or H-bond (12-10)
+ * weight times its unweighted eps.  The adjoint gathers, for each atom, over
+ * the grid points within 8 A (and over all points for the e map) with
+ * fixed-order sums: no atomics, bitwise repeatable. */
+typedef struct ag_weights {
+    double vdw, hbond, estat, desolv;   /* coeff_vdW, coeff_hbond, coeff_estat, coeff_desolv */
+} ag_weights;
+typedef struct ag_analytic ag_analytic;
+
+/* `weights` are the coefficients the setup's pair eps were weighted with
+ * (the parameter library's, ag_job_weights); rec's bond-vector arrays are
+ * not read. */
+ag_status ag_analytic_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
+                             const ag_mapset *mapset, const ag_setup *setup,
+                             const ag_weights *weights, ag_analytic **out);
+/* Replace any of the parameters (NULL keeps the current value). */
+ag_status ag_analytic_set(ag_analytic *a, const double *coord, const double *charge,
+                          const double *rij, const double *eps, const ag_weights *weights);
+/* Maps into maps_dev ([n_maps][P] float, device). */
+ag_status ag_analytic_forward(ag_analytic *a, float *maps_dev);
+/* Given dL/dmaps ([n_maps][P] float, device) at the last forward: dL/dcoord
+ * [N][3], dL/dcharge [N], dL/dRij and dL/deps (unweighted)
+ * [n_atom_maps][n_rec_types] (0 for nbp_coeffs pairs) and dL/dweights.  Any
+ * output may be NULL. */
+ag_status ag_analytic_adjoint(ag_analytic *a, const float *dmaps_dev, double *g_coord,
+                              double *g_charge, double *g_rij, double *g_eps,
+                              ag_weights *g_weights);
+ag_status ag_analytic_destroy(ag_analytic *a);
+/* The parameter library's AD4 weights of a job (read_parameter_library.cc
+ * FE_coeff_* lines). */
+ag_status ag_job_weights(const ag_job *job, ag_weights *out);
 
 /* Map output: the value text of main.cpp:3486-3497 -- "0" when |e| < 1e-4,
  * else "%.3f" of (float)(floor(e*1000+0.5)/1000) -- for n values into buf.

@@ -633,6 +633,229 @@ int orc_maps_vina(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *
     return 0;
 }
 
+/* --- Analytic model (the engine's differentiable mode) -------------------
+ *
+ * The table path's functions evaluated at the exact distance instead of
+ * through 1/128 A bins:
+ *   E_mt(r)  = min(EINTCLAMP, Phi(s(r))), Phi the 12-6 / 12-10 pair
+ *              potential of main.cpp:2135-2190 (cA, cB from Rij, eps, or as
+ *              given by nbp_coeffs), and s(r) the continuous form of the
+ *              min-window smoothing of main.cpp:2200-2216: for a potential
+ *              with one minimum at r*, min over [r - w, r + w] of Phi is
+ *              Phi(r + w) left of the well, Phi(r*) inside it and
+ *              Phi(r - w) right of it; w = floor(r_smooth * 64) / 128;
+ *   sol(r)   = coeff_desolv exp(-r^2 / (2 * 3.6^2))    (main.cpp:2272-2276);
+ *   eps(r)   the Mehler-Solmajer dielectric at r      (distdepdiel.cpp:30-57);
+ * and the hard 8 A cutoff replaced by the switch S(r) = 1 - u^3 (10 - 15 u
+ * + 6 u^2), u = clamp((r - 7.5) / 0.5, 0, 1), multiplying every 8 A term.
+ * The per-point assembly (closest H, racc / rdon / Hramp, the H-bond case
+ * split with its min / max, desolvation, d map) is main.cpp:2985-3431 with
+ * those functions in place of the tables.  The e map is
+ * sum_i q_i coeff_estat (332 / eps(r)) / max(r, 0.5) over all atoms
+ * (main.cpp:2940-2982).  disorder_h is not part of the analytic model.
+ *
+ * Parity: pinned against the table path (orc_maps) away from bin, cap and
+ * switch effects; the functions themselves are restatements of the
+ * reference's formulas. */
+
+static double an_phi(const ag_pair *pp, double s) {
+    const int a = pp->xA, b = pp->xB;
+    if (pp->cA == 0 && pp->cB == 0) {
+        const double rho = pp->Rij / s;
+        return pp->eps / (double)(a - b) * ((double)b * pow(rho, a) - (double)a * pow(rho, b));
+    }
+    return pp->cA / pow(s, a) - pp->cB / pow(s, b);
+}
+
+static double an_rstar(const ag_pair *pp) {
+    const int a = pp->xA, b = pp->xB;
+    if (pp->cA == 0 && pp->cB == 0) return pp->Rij;
+    return pow((double)a * pp->cA / ((double)b * pp->cB), 1.0 / (double)(a - b));
+}
+
+static double an_energy(const ag_pair *pp, double r, double w) {
+    const double rs = an_rstar(pp);
+    double s;
+    if (r < rs - w) s = r + w;
+    else if (r > rs + w) s = r - w;
+    else s = rs;
+    return dmin(EINTCLAMP, an_phi(pp, s));
+}
+
+static double an_switch(double r) {
+    double u = (r - 7.5) / 0.5;
+    if (u <= 0) return 1.0;
+    if (u >= 1) return 0.0;
+    return 1.0 - u * u * u * (10.0 - 15.0 * u + 6.0 * u * u);
+}
+
+static double an_reps(double r, int dddiel, double inv_diel_cal) {
+    if (!dddiel) return inv_diel_cal;
+    const double lambda = 0.003627, epsilon0 = 78.4, A = -8.5525, rk = 7.7839;
+    const double B = epsilon0 - A;
+    double eps = A + B / (1.0 + rk * exp(-lambda * B * r));
+    if (eps < APPROX_ZERO) eps = 1.0;
+    return 332.0 / eps;
+}
+
+int orc_analytic_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_pair *pairs,
+                      int T, double r_smooth, double coeff_desolv, int npts, const int64_t *points, double *out) {
+    const int N = rec->n_atoms, M = ms->n_atom_maps;
+    const int n1x = grid->n1[0], n1y = grid->n1[1];
+    const double spacing = grid->spacing;
+    const double w = floor(r_smooth * A_DIV / 2.) / A_DIV;
+    const double ln_half = log(0.5);
+    const double PI_halved = 3.14159265358979323846 / 2.;
+    const int elecPE = ms->elec_index, dsolvPE = ms->dsolv_index;
+    if (ms->disorder_h) return -2;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int k = 0; k < npts; k++) {
+        const int64_t mapi = points[k];
+        const int ix = (int)(mapi % n1x), iy = (int)((mapi / n1x) % n1y), iz = (int)(mapi / ((int64_t)n1x * n1y));
+        double c[3] = {(ix - grid->ne[0]) * spacing, (iy - grid->ne[1]) * spacing, (iz - grid->ne[2]) * spacing};
+        double en[AG_MAX_MAPS], hmin[AG_MAX_MAPS], hmax[AG_MAX_MAPS];
+        int hflag[AG_MAX_MAPS];
+        for (int m = 0; m < ms->n_maps; m++) en[m] = 0.0;
+        for (int m = 0; m < M; m++) {
+            hmin[m] = 999999.;
+            hmax[m] = -999999.;
+            hflag[m] = 0;
+            if (ms->maps[m].is_covalent) {
+                double d[3], rcov;
+                for (int ii = 0; ii < 3; ii++) d[ii] = ms->cov_pos[ii] - c[ii];
+                rcov = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / ms->cov_half_width;
+                if (rcov < APPROX_ZERO) rcov = APPROX_ZERO;
+                en[m] = ms->cov_barrier * (1. - exp(ln_half * rcov * rcov));
+            }
+        }
+        /* e map: all atoms */
+        if (elecPE >= 0) {
+            double e = 0.0;
+            for (int ia = 0; ia < N; ia++) {
+                const double *a = rec->coord + 3 * (size_t)ia;
+                const double dx = a[0] - c[0], dy = a[1] - c[1], dz = a[2] - c[2];
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                e += rec->charge[ia] * ms->coeff_estat * an_reps(r, ms->dddiel, ms->inv_diel_cal) / dmax(r, 0.5);
+            }
+            en[elecPE] = e;
+        }
+        /* closest donor H, FP64, lowest index on ties */
+        double rminH = 999999.;
+        int closestH = -1;
+        for (int ia = 0; ia < N; ia++) {
+            if (rec->hbond[ia] != 1 && rec->hbond[ia] != 2) continue;
+            const double *a = rec->coord + 3 * (size_t)ia;
+            const double dx = a[0] - c[0], dy = a[1] - c[1], dz = a[2] - c[2];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (r < rminH) {
+                rminH = r;
+                closestH = ia;
+            }
+        }
+        for (int ia = 0; ia < N; ia++) {
+            const double *a = rec->coord + 3 * (size_t)ia;
+            const double *rv = rec->rvector + 3 * (size_t)ia;
+            const double *rv2 = rec->rvector2 + 3 * (size_t)ia;
+            const int hb = rec->hbond[ia], at = rec->type[ia];
+            double d[3] = {a[0] - c[0], a[1] - c[1], a[2] - c[2]};
+            double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (r >= 8.0) continue;
+            if (r < APPROX_ZERO) r = APPROX_ZERO;
+            const double S = an_switch(r);
+            double dnorm[3];
+            for (int i = 0; i < 3; i++) dnorm[i] = d[i] / r;
+            double racc = 1., rdon = 1., Hramp = 1.;
+            if (hb == 2) { /* D1 */
+                double cos_theta = 0.;
+                for (int i = 0; i < 3; i++) cos_theta -= dnorm[i] * rv[i];
+                if (cos_theta <= 0.) {
+                    racc = 0.;
+                } else {
+                    switch (rec->rexp[ia]) {
+                        case 2: racc = cos_theta * cos_theta; break;
+                        case 4: { double t = cos_theta * cos_theta; racc = t * t; } break;
+                        default: racc = cos_theta; break;
+                    }
+                    if (ia != closestH && closestH >= 0) {
+                        double ct = 0.;
+                        for (int i = 0; i < 3; i++) ct += rec->rvector[3 * (size_t)closestH + i] * rv[i];
+                        ct = dmin(ct, 1.0);
+                        ct = dmax(ct, -1.0);
+                        Hramp = 0.5 - 0.5 * cos(acos(ct) * 120. / 90.);
+                    }
+                }
+            } else if (hb == 4) { /* A1 */
+                double cos_theta = 0.;
+                for (int i = 0; i < 3; i++) cos_theta -= dnorm[i] * rv[i];
+                rdon = cos_theta <= 0. ? 0. : cos_theta * cos_theta;
+            } else if (hb == 5) { /* A2, ordered (disorder_h off) */
+                double cos_theta = 0.;
+                for (int i = 0; i < 3; i++) cos_theta -= dnorm[i] * rv[i];
+                double t0 = 0.;
+                for (int i = 0; i < 3; i++) t0 += dnorm[i] * rv2[i];
+                t0 = dmin(dmax(t0, -1.0), 1.0);
+                t0 = PI_halved - acos(t0);
+                rdon = 0.;
+                if (cos_theta >= 0.) {
+                    double cr[3] = {dnorm[1] * rv2[2] - dnorm[2] * rv2[1], dnorm[2] * rv2[0] - dnorm[0] * rv2[2],
+                                    dnorm[0] * rv2[1] - dnorm[1] * rv2[0]};
+                    double rd2 = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+                    if (rd2 < APPROX_ZERO) rd2 = APPROX_ZERO;
+                    const double inv = 1. / sqrt(rd2);
+                    double ti = 0.;
+                    for (int i = 0; i < 3; i++) ti += cr[i] * inv * rv[i];
+                    ti = dmin(dmax(ti, -1.0), 1.0);
+                    ti = fabs(acos(ti) - PI_halved);
+                    rdon = (0.9 + 0.1 * sin(ti + ti)) * cos(t0);
+                } else if (cos_theta >= -0.34202) {
+                    const double u = 0.116978 - cos_theta * cos_theta;
+                    rdon = 562.25 * u * u * u * cos(t0);
+                }
+            }
+            for (int m = 0; m < M; m++) {
+                if (ms->maps[m].is_covalent) continue;
+                int md = m; /* map receiving the desolvation term */
+                if (ms->separate_desolvation) {
+                    md = m + 1;
+                }
+                const ag_pair *pp = pairs + (size_t)m * T + at;
+                const double E = an_energy(pp, r, w) * S;
+                const double rsph = dmax(0.0, dmin(1.0, E / 100.));
+                const int mh = ms->maps[m].hbond;
+                if (ms->maps[m].is_hbonder) {
+                    if ((mh == 3 || mh == 5 || mh == 6) && (hb == 1 || hb == 2 || hb == 6)) {
+                        en[m] += E * Hramp * (racc + (1. - racc) * rsph);
+                    } else if ((mh == 4 || mh == 6) && (hb == 1 || hb == 2 || hb == 6)) {
+                        const double v = E * (racc + (1. - racc) * rsph);
+                        hmin[m] = dmin(hmin[m], v);
+                        hmax[m] = dmax(hmax[m], v);
+                        hflag[m] = 1;
+                    } else if ((mh == 1 || mh == 2 || mh == 6) && (hb > 2 || hb == 6)) {
+                        const double v = E * (rdon + (1. - rdon) * rsph);
+                        hmin[m] = dmin(hmin[m], v);
+                        hmax[m] = dmax(hmax[m], v);
+                        hflag[m] = 1;
+                    } else {
+                        en[m] += E;
+                    }
+                } else {
+                    en[m] += E;
+                }
+                const double sol = coeff_desolv * exp(-(r * r) / (2. * 3.6 * 3.6)) * S;
+                en[md] += ms->maps[md].solpar_probe * rec->vol[ia] * sol +
+                          (rec->solpar[ia] + ms->solpar_q * fabs(rec->charge[ia])) * ms->maps[md].vol_probe * sol;
+                if (ms->separate_desolvation) m++;
+            }
+            if (dsolvPE >= 0)
+                en[dsolvPE] += ms->solpar_q * rec->vol[ia] * coeff_desolv * exp(-(r * r) / (2. * 3.6 * 3.6)) * S;
+        }
+        for (int m = 0; m < M; m++)
+            if (hflag[m]) en[m] += hmin[m] + dmax(hmax[m], 0.);
+        for (int m = 0; m < ms->n_maps; m++) out[(size_t)m * npts + k] = en[m];
+    }
+    return 0;
+}
+
 /* main.cpp:3486-3497 with HAVE_ROUND unset (main.cpp:85-89) */
 int orc_format_value(double e, char *out) {
     if (fabs(e) < 0.0001) {

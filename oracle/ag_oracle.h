@@ -50,6 +50,13 @@ int orc_vina_tables(const ag_setup *s, const ag_mapset *ms, const double *xs_map
 int orc_maps_vina(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                   const ag_tables *tab, double *maps, double *fmap, int32_t *cmap,
                   int z0, int z1);
+/* Analytic model (the engine's differentiable mode) in FP64 at the given
+ * point indices: out [n_maps][npts].  pairs [n_atom_maps][T] hold the
+ * effective (weighted) eps; coeff_estat comes from the map set.  Returns -2
+ * when disorder_h is set (not part of the analytic model). */
+int orc_analytic_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                      const ag_pair *pairs, int T, double r_smooth, double coeff_desolv,
+                      int npts, const int64_t *points, double *out);
 /* One map value as the reference writes it (main.cpp:3486-3497). */
 int orc_format_value(double e, char *out);
 /* One floating-grid value (main.cpp:3502-3507). */

@@ -43,6 +43,8 @@ def olib() -> C.CDLL:
         L.orc_xs_radius.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
         L.orc_vina_tables.argtypes = [C.POINTER(_abi.Setup), C.POINTER(_abi.MapSet), vp, vp, vp]
         L.orc_maps_vina.argtypes = L.orc_maps.argtypes
+        L.orc_analytic_maps.argtypes = [C.POINTER(_abi.Receptor), C.POINTER(_abi.Grid), C.POINTER(_abi.MapSet),
+                                        vp, C.c_int, C.c_double, C.c_double, C.c_int, vp, vp]
         L.orc_format_value.argtypes = [C.c_double, C.c_char_p]
         L.orc_format_rmin.argtypes = [C.c_double, C.c_char_p]
         _lib = L
@@ -168,3 +170,36 @@ def gpf_types(gpf_path: str):
         elif w and w[0] == "receptor_types":
             recp = w[1:]
     return lig, recp
+
+
+def analytic_maps(st: ag.PreparedState, points, rij=None, eps0=None, weights=None, coord=None, charge=None):
+    """FP64 analytic model at the given flat point indices: [n_maps, npts].
+
+    rij / eps0 ([M, T], eps0 unweighted) and weights (vdw, hbond, estat,
+    desolv) default to the state's; bond vectors are rebuilt by the oracle
+    from the (possibly perturbed) coordinates."""
+    import dataclasses
+    w = np.array(st.weights if weights is None else weights, np.float64)
+    pairs = st.pairs.copy()
+    wk = np.where(pairs["hbonder"] > 0, st.weights[1], st.weights[0])
+    e0 = pairs["eps"] / wk if eps0 is None else np.asarray(eps0, np.float64)
+    if rij is not None:
+        pairs["Rij"] = rij
+    pairs["eps"] = e0 * np.where(pairs["hbonder"] > 0, w[1], w[0])
+    s2 = dataclasses.replace(st, coord=st.coord if coord is None else np.asarray(coord, np.float64),
+                             charge=st.charge if charge is None else np.asarray(charge, np.float64),
+                             coeff_estat=float(w[2]), _keep=[])
+    rexp, rv, rv2, dis = bond_vectors(s2)
+    s2 = dataclasses.replace(s2, rexp=rexp, rvector=rv, rvector2=rv2, disorder=dis, _keep=[])
+    rec, grid, ms, tab, setup = s2.structs()
+    M, T = pairs.shape
+    pa = (_abi.Pair * max(1, M * T))()
+    for i, p in enumerate(pairs.reshape(-1)):
+        pa[i] = _abi.Pair(*[p[k].item() for k in ("cA", "cB", "Rij", "eps", "xA", "xB", "hbonder")], 0)
+    pts = np.ascontiguousarray(points, np.int64)
+    out = np.zeros((st.n_maps, pts.size))
+    rc = olib().orc_analytic_maps(C.byref(rec), C.byref(grid), C.byref(ms), C.cast(pa, C.c_void_p), T,
+                                  s2.r_smooth, float(w[3]), pts.size, pts.ctypes.data, out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle analytic model refused the state")
+    return out

@@ -59,3 +59,14 @@ def test_ensemble_is_seeded_and_in_range(tmp_path):
     moved_apart = np.abs(d1 - d0) > 1e-9
     # a loop boundary may fall between a parent and its hydrogen; elsewhere the bond is rigid
     assert moved_apart.sum() <= 6
+
+
+def test_bond_vectors_batch_matches_single(tmp_path):
+    ens = synth.make_ensemble(str(tmp_path), n_conf=5, seed=3002, base_atoms=2000, n_range=(300, 2000),
+                              npts_range=(16, 24))
+    raw = [synth.conformation_state(ens, i, bonds=False) for i in range(5)]
+    got = ag.bond_vectors_batch(raw, n_threads=3)
+    for st, g in zip(raw, got):
+        rexp, rv, rv2, dis = ag.bond_vectors(st, st.coord, st.type, st.hbond)
+        assert np.array_equal(g.rexp, rexp) and np.array_equal(g.rvector, rv)
+        assert np.array_equal(g.rvector2, rv2) and np.array_equal(g.disorder, dis)

@@ -19,6 +19,7 @@ import torch  # noqa: F401  (first: one HIP runtime for torch and the engine)
 import numpy as np
 
 from . import Analytic, Context, PreparedState
+from .poses import cotangent, make_poses, splat
 
 
 class AnalyticMaps(torch.autograd.Function):
@@ -48,59 +49,15 @@ class AnalyticMaps(torch.autograd.Function):
         return t(g["coord"]), t(g["charge"]), t(g["rij"]), t(g["eps"]), t(g["weights"]), None
 
 
-def make_poses(st: PreparedState, map_of_type: dict, n_poses: int = 256, seed: int = 1005,
-               atoms_range=(20, 60), sigma_q: float = 0.15, spread: float = 3.0):
-    """Seeded ligand poses inside the box: per pose a centre drawn uniformly
-    in the box (a margin of ``spread`` + 1 grid step from the faces) and
-    atoms scattered around it (Gaussian, ``spread``/2 A), types drawn from
-    ``map_of_type`` keys, charges N(0, sigma_q).  Returns a list of
-    (map index [n], charge [n], position [n, 3] grid frame)."""
-    rng = np.random.default_rng(seed)
-    names = list(map_of_type)
-    lo = np.array([-st.ne[d] * st.spacing for d in range(3)])
-    hi = np.array([(st.n1[d] - 1 - st.ne[d]) * st.spacing for d in range(3)])
-    margin = spread + st.spacing
-    poses = []
-    for _ in range(n_poses):
-        n = int(rng.integers(atoms_range[0], atoms_range[1] + 1))
-        c = rng.uniform(lo + margin, hi - margin)
-        x = c + rng.normal(0.0, spread / 2.0, size=(n, 3))
-        x = np.clip(x, lo + 1e-3, hi - st.spacing - 1e-3)
-        t = rng.integers(0, len(names), size=n)
-        poses.append((np.array([map_of_type[names[k]] for k in t]), rng.normal(0.0, sigma_q, size=n), x))
-    return poses
-
-
 class PoseScorer:
     """Scores of ligand poses against a map set: score = S @ maps.ravel(),
-    S the sparse trilinear-interpolation matrix [n_poses, n_maps * P]."""
+    S the sparse trilinear-interpolation matrix [n_poses, n_maps * P]
+    (poses.splat)."""
 
     def __init__(self, st: PreparedState, poses, device="cuda"):
-        P = st.points
-        n1x, n1y = st.n1[0], st.n1[1]
-        rows, cols, vals = [], [], []
-        e, d = st.elec_index, st.dsolv_index
-        for k, (mi, q, x) in enumerate(poses):
-            g = (x + np.array([st.ne[0], st.ne[1], st.ne[2]]) * st.spacing) / st.spacing
-            i0 = np.floor(g).astype(np.int64)
-            f = g - i0
-            for dz in (0, 1):
-                for dy in (0, 1):
-                    for dx in (0, 1):
-                        w = (f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * \
-                            (f[:, 2] if dz else 1 - f[:, 2])
-                        pt = (i0[:, 0] + dx) + n1x * ((i0[:, 1] + dy) + n1y * (i0[:, 2] + dz))
-                        for m, scale in ((mi, 1.0), (e, q), (d, np.abs(q))):
-                            if isinstance(m, (int, np.integer)) and m < 0:
-                                continue
-                            rows.append(np.full(pt.size, k))
-                            cols.append(np.asarray(m) * P + pt)
-                            vals.append(w * scale)
-        r = np.concatenate(rows)
-        c = np.concatenate(cols)
-        v = np.concatenate(vals)
+        r, c, v = splat(st, poses)
         self.n_poses = len(poses)
-        self.shape = (self.n_poses, st.n_maps * P)
+        self.shape = (self.n_poses, st.n_maps * st.points)
         idx = torch.from_numpy(np.stack([r, c]))
         self.S = torch.sparse_coo_tensor(idx, torch.from_numpy(v).to(torch.float32), self.shape,
                                          device=device).coalesce()
@@ -111,10 +68,7 @@ class PoseScorer:
 
     def cotangent(self, dscore: np.ndarray) -> np.ndarray:
         """dL/dmaps (flat) for dL/dscore, on the host."""
-        r, c, v = self.S_cpu
-        out = np.zeros(self.shape[1])
-        np.add.at(out, c, v * dscore[r])
-        return out
+        return cotangent(self.S_cpu, self.shape[1], dscore)
 
 
 def fit_weights(ctx: Context, st: PreparedState, poses, scale_true, steps: int = 200, lr: float = 0.02):

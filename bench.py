@@ -23,6 +23,9 @@ Also reported on the same JSON line:
                 sequential single-receptor runs (tables, bond vectors,
                 ag_maps_plain with its uploads and copies), both without
                 ASCII output.
+  analytic      the differentiable mode on the full receptor with a 65^3
+                box: forward and forward+adjoint times (cotangent from 256
+                ligand poses) and a 200-step Adam fit of the AD4 weights.
 """
 from __future__ import annotations
 
@@ -154,6 +157,72 @@ def batched_leg(reps: int = 2) -> dict:
             "reps": reps}
 
 
+def analytic_leg(reps: int = 10, fit_steps: int = 200) -> dict:
+    """Analytic mode on the full workload's receptor with a 65^3 box (maps
+    A C HD N OA + e + d): forward and forward+adjoint times with the
+    cotangent of 256 seeded ligand poses' summed trilinear scores, and the
+    200-step Adam fit of the four weights (torch, in a child process)."""
+    import numpy as np
+
+    import autogrid_amd as ag
+    from autogrid_amd import poses as ps
+    from autogrid_amd import synth
+
+    work = tempfile.mkdtemp(prefix="agan_")
+    try:
+        info = synth.make_workload("analytic", work)
+        cwd = os.getcwd()
+        os.chdir(work)
+        try:
+            st = ag.Job(info["gpf"]).prepare().state()
+        finally:
+            os.chdir(cwd)
+        lig = {t: i for i, t in enumerate(synth.WORKLOADS["analytic"].ligand_types)}
+        poses = ps.make_poses(st, lig, n_poses=256, seed=1005)
+        S = ps.splat(st, poses)
+        cot = ps.cotangent(S, st.n_maps * st.points, np.ones(len(poses))).astype(np.float32)
+        hip = hip_runtime()
+        nbytes = 4 * st.n_maps * st.points
+        dm, dg = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(dm), C.c_size_t(nbytes)) == 0
+        assert hip.hipMalloc(C.byref(dg), C.c_size_t(nbytes)) == 0
+        assert hip.hipMemcpy(dg, cot.ctypes.data_as(C.c_void_p), C.c_size_t(nbytes), 1) == 0
+        with ag.Context(0) as ctx, ag.Analytic(ctx, st) as an:
+            an.forward(dm.value)
+            an.adjoint(dg.value)
+            fw, fa, fk, ak = [], [], [], []
+            for _ in range(reps):
+                hip.hipDeviceSynchronize()
+                t0 = time.perf_counter()
+                an.forward(dm.value)
+                t1 = time.perf_counter()
+                fk.append(ctx.timing()["total_ms"])
+                an.adjoint(dg.value)
+                t2 = time.perf_counter()
+                ak.append(ctx.timing()["total_ms"])
+                fw.append(t1 - t0)
+                fa.append(t2 - t0)
+        hip.hipFree(dm)
+        hip.hipFree(dg)
+        out = {"workload": f"analytic: {st.n_atoms} atoms, {st.n1[0]}^3 points, {st.n_maps} maps, 256 poses",
+               "forward_s": min(fw), "forward_adjoint_s": min(fa), "ratio": min(fa) / min(fw),
+               "forward_kernels_ms": min(fk), "adjoint_kernels_ms": min(ak),
+               "kernel_ratio": (min(fk) + min(ak)) / min(fk)}
+        if fit_steps:
+            p = subprocess.run([sys.executable, "-m", "autogrid_amd.fit", "--workload", "analytic", "--steps",
+                                str(fit_steps), "--poses", "256", "--workdir", work], cwd=ROOT,
+                               capture_output=True, text=True, timeout=900)
+            if p.returncode == 0:
+                fit = json.loads(p.stdout.strip().splitlines()[-1])
+                out["fit"] = {k: fit[k] for k in ("steps", "loss_initial", "loss_final", "w_rel_err_max",
+                                                   "s_per_step", "scale_true")}
+            else:
+                out["fit"] = {"error": p.stderr[-400:]}
+        return out
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +232,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cli", action="store_true", help="skip the drop-in run with ASCII output")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-conformation batched comparison")
+    ap.add_argument("--no-analytic", action="store_true", help="skip the analytic forward/adjoint leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -241,6 +311,7 @@ def main():
         shutil.rmtree(work, ignore_errors=True)
 
     batch = None if args.no_batch else batched_leg()
+    analytic = None if args.no_analytic else analytic_leg()
     base = None if args.no_cpu_baseline else cpu_baseline(args.workload)
     value = ms_per_step / 1000.0
     out = {
@@ -262,6 +333,7 @@ def main():
         "bin_audit": audit,
         "wall_with_ascii_output_s": wall_cli if cli_ok else None,
         "batched": batch,
+        "analytic": analytic,
         "cpu_baseline": base,
     }
     print(json.dumps(out))

@@ -52,12 +52,12 @@ constexpr int kHCap = 256;                        // donor-H candidates per bric
 constexpr int kChunk = 128;                       // candidate records staged per pass
 constexpr int kNearRepsLen = 1040;                // electrostatic bins used by the near kernel
 constexpr float kNearRadius = 8.03f;              // staging radius (A)
-constexpr int kFarK = 8;                          // points per lane, far kernel
-constexpr int kFarThreads = 512;
-constexpr int kFarTabMax = 16384;                 // LDS table entries without the clamp
+constexpr int kFarK = 16;                         // points per lane, far kernel
+constexpr int kFarRuns = 256;                     // runs (lanes of kFarK points) per far-kernel item
+constexpr int kFarThreads = 2 * kFarRuns;         // two halves of the atom list per run
+constexpr int kFarTabMax = 16380;                 // LDS table entries without the clamp (two workgroups per CU)
 constexpr int kFarAtomChunk = 64;                 // Kahan block
 constexpr int kFarBinMin = 1025;                  // far kernel takes bins >= this
-constexpr float kFarBias = 1.0e-3f;               // added to (128 r)^2, keeps rsq finite
 constexpr float kGuardBins = 1.0e-3f;             // FP64 recompute band around bin edges
 
 // Map roles.
@@ -105,6 +105,15 @@ struct ConfDev {
 // reads become scalar loads the compiler may repeat instead of holding the
 // values in registers across the hot loops.
 #define AG_CONST __attribute__((address_space(4)))
+// float4 layout without HIP_vector_type's constructors, for copies out of
+// the constant address space.
+struct __align__(16) PlainF4 {
+    float x, y, z, w;
+};
+__device__ __forceinline__ PlainF4 ld4(const AG_CONST PlainF4 *p, int i) {
+    const AG_CONST PlainF4 &q = p[i];
+    return PlainF4{q.x, q.y, q.z, q.w};
+}
 template <typename T>
 __device__ __forceinline__ const AG_CONST T *as_const(const T *p) {
     return (const AG_CONST T *)(p);
@@ -157,7 +166,7 @@ struct NearParams {
 
 struct FarParams {
     const ConfDev *confs;
-    const int *tile_prefix;       // [nconf + 1], tiles of kFarThreads runs
+    const int *tile_prefix;       // [nconf + 1], items of kFarRuns runs
     int nconf, total_tiles;
     int item_base, item_end;      // tiles [item_base, item_end) of this launch
     int *queue;                   // null: one tile per workgroup; else a persistent work counter
@@ -190,14 +199,37 @@ __device__ __forceinline__ unsigned bin4_of(float rb2, float rs) {
     return __float_as_uint(__builtin_fmaf(rb2, rs, kBinMagic)) & 0x7FFFFCu;
 }
 
+// Far-kernel geometry of a run of kFarK points along x.  The run's points
+// sit at cx + dk, dk = (k - kFarMid) s128 (k = 0..kFarK-1, cx the run's
+// centre), and
+//   rb2 = (u - dk)^2 + dy^2 + dz^2 + bias = fma(u, -2 dk, (u^2 + dyz2) + dk^2),
+// u = ax - cx: the points k and kFarK-1-k share dk^2, so kFarK points cost
+// kFarK/2 adds and kFarK fmas (1.5 operations a point instead of a subtract
+// and an fma).  The rounding of u^2 + dyz2 + dk^2 is below
+// 2^-23 (u^2 + dk^2 + dyz2); with u = dk the exact value is only dyz2, so the
+// bias (a floor for dyz2) is 2^-20 kFarMid^2 s128^2, four times that error
+// at the largest dk, which keeps rb2 > 0 and rsq finite.  In bins (128 r)
+// it moves r by bias / (2 r), below 1e-4 of a bin for the far field's
+// r >= 1025 bins at 0.375 A spacing.
+constexpr float kFarMid = 0.5f * (kFarK - 1);
+
+__device__ __forceinline__ float far_bias(float s128) {
+    const float d = kFarMid * s128;
+    return fmaxf(1.0e-3f, d * d * 0x1p-20f);
+}
+__device__ __forceinline__ float far_dk(int k, float s128) { return ((float)k - kFarMid) * s128; }
+__device__ __forceinline__ float far_cx(int ix0, int ne0, float s128) { return ((float)(ix0 - ne0) + kFarMid) * s128; }
+
 // The far kernel's per-pair table offset, reproduced bit for bit by the near
-// kernel when it decides which atoms the far kernel left out.
-__device__ __forceinline__ unsigned far_bin4(float ax, float ay, float az, float px, float py, float pz) {
-#pragma clang fp contract(off)
+// kernel when it decides which atoms the far kernel left out (same
+// operations in the same order; the build turns off FP contraction).
+__device__ __forceinline__ unsigned far_bin4(float ax, float ay, float az, float cx, float dk, float py, float pz,
+                                             float bias) {
     const float dy = ay - py, dz = az - pz;
-    const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
-    const float dx = ax - px;
-    const float rb2 = __builtin_fmaf(dx, dx, dyz2);
+    const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, bias));
+    const float u = ax - cx;
+    const float W = __builtin_fmaf(u, u, dyz2) + dk * dk;
+    const float rb2 = __builtin_fmaf(u, -2.f * dk, W);
     return bin4_of(rb2, __builtin_amdgcn_rsqf(rb2));
 }
 
@@ -423,10 +455,13 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     const double cx = (double)(ix - nex) * spacing;
     const double cy = (double)(iy - ney) * spacing;
     const double cz = (double)(iz - nez) * spacing;
-    // the far kernel's point coordinates
-    const float pxE = (float)(ix - nex) * s128;
+    // the far kernel's point coordinates: run centre, offset in the run, y, z
+    const int kxE = ix % kFarK;
+    const float cxE = far_cx(ix - kxE, nex, s128);
+    const float dkE = far_dk(kxE, s128);
     const float pyE = (float)(iy - ney) * s128;
     const float pzE = (float)(iz - nez) * s128;
+    const float biasE = far_bias(s128);
 
     // ---- stage candidate atoms: cells overlapping the brick box + radius ----
     int c0[3], c1[3];
@@ -542,7 +577,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
         hrv[0] = hrvs[best_h][0]; hrv[1] = hrvs[best_h][1]; hrv[2] = hrvs[best_h][2];
     }
 
-    float e_near = 0.f, d_acc = 0.f;
+    float e_near = 0.f, d_acc = 0.f, dv_acc = 0.f, dq_acc = 0.f;
     int ccount = 0;
     int *const cmap = p.cmap ? p.cmap + cf.pt_off : nullptr;
     float *const maps = p.maps + cf.map_off;
@@ -594,7 +629,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 bool near_e;
                 if (g < 1022.f) near_e = true;
                 else if (g > 1027.f) near_e = false;
-                else near_e = far_bin4(a.xE, a.yE, a.zE, pxE, pyE, pzE) < 4u * kFarBinMin;
+                else near_e = far_bin4(a.xE, a.yE, a.zE, cxE, dkE, pyE, pzE, biasE) < 4u * kFarBinMin;
                 if (near_e) {
                     // 1/max(r, 0.5) with r clamped at APPROX_ZERO
                     const float inv_rmax = fminf(__builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f)), 2.f);
@@ -687,7 +722,12 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             const int indx_n = min(bin, AG_NEINT - 1);
             const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
             const float S = sol_s[indx_r];
-            const float avol = unif(a.vol), asolq = unif(a.solq);
+            // desolvation (main.cpp:3375-3390): slot m gains
+            // (sp[m] vol + volp[m] solq) S per pair, i.e. sp[m] sum(vol S) +
+            // volp[m] sum(solq S) over the point's pairs: two sums per point
+            // instead of a term per slot, combined after the loop
+            dv_acc = __builtin_fmaf(unif(a.vol), S, dv_acc);
+            dq_acc = __builtin_fmaf(unif(a.solq), S, dq_acc);
             // vdW/H-bond table values of this pair for the pass's slots
             const float *row = tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots;
             float E[kMapSlots];
@@ -700,47 +740,58 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                     E[4 * q4] = E[4 * q4 + 1] = E[4 * q4 + 2] = E[4 * q4 + 3] = 0.f;
                 }
             }
-            // plain slots: table value + desolvation (main.cpp:3375-3390)
+            // plain slots: the table value
 #pragma unroll
-            for (int m = 0; m < NPLAIN; m++) {
-                const float t = __builtin_fmaf(p.sp[m], avol, p.volp[m] * asolq);
-                acc[m] = __builtin_fmaf(t, S, __builtin_fmaf(p.kE[m], E[m], acc[m]));
-            }
+            for (int m = 0; m < NPLAIN; m++) acc[m] = __builtin_fmaf(p.kE[m], E[m], acc[m]);
             // H-bonding slots (main.cpp:3286-3374): the (probe, atom) case is a
-            // set of 0/1 coefficients chosen per atom class, applied without
-            // branches.  A disordered partner uses the hydrogen-type row at
-            // r - 1.10 A (main.cpp:3313-3349), FP64 like the reference.
-            float Eh[kHbSlots] = {0.f, 0.f, 0.f, 0.f};
-            if (dis && p.hydrogen_type >= 0) {
-                double rdx, rdy, rdz;
-                const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
-                const int hb_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
-                const float4 v = reinterpret_cast<const float4 *>(
-                    tab + ((size_t)p.hydrogen_type * AG_NEINT + hb_bin) * kMapSlots)[kPlainSlots / 4];
-                Eh[0] = v.x; Eh[1] = v.y; Eh[2] = v.z; Eh[3] = v.w;
-            }
-            const float dfac = racc - rdon;
+            // set of 0/1 coefficients chosen per atom class.  The class is the
+            // same for every lane, so the case is a scalar branch: a plain
+            // table add, nothing, or the directional mix.  A disordered
+            // partner uses the hydrogen-type row at r - 1.10 A
+            // (main.cpp:3313-3349), FP64 like the reference.
             const int dsh = dis ? 4 : 0;
+            bool any_mix = false;
 #pragma unroll
             for (int k = 0; k < kHbSlots; k++) {
-                const int m = kPlainSlots + k;
                 const int code = (p.hcode[k][hb] >> dsh) & 15;
-                const float kP = (code & HC_PLAIN) ? 1.f : 0.f;
-                const float kH = (code & HC_HRAMP) ? 1.f : 0.f;
-                const float kF = (code & HC_RACC) ? 1.f : 0.f;
-                const bool kM = (code & HC_MINMAX) != 0;
-                const float e = E[m];
-                const float rsph = __builtin_amdgcn_fmed3f(e * 0.01f, 0.f, 1.f);
-                const float f = __builtin_fmaf(kF, dfac, rdon);
-                const float mix = __builtin_fmaf(1.f - f, rsph, f);
-                const float eh = dis ? Eh[k] : e;
-                acc[m] = __builtin_fmaf(kP, e, acc[m]);
-                acc[m] = __builtin_fmaf(kH, (eh * Hramp) * mix, acc[m]);
-                const float v = e * mix;
-                hmin[k] = kM ? fminf(hmin[k], v) : hmin[k];
-                hmax[k] = kM ? fmaxf(hmax[k], v) : hmax[k];
-                const float t = __builtin_fmaf(p.sp[m], avol, p.volp[m] * asolq);
-                acc[m] = __builtin_fmaf(t, S, acc[m]);
+                any_mix |= (code & ~HC_PLAIN) != 0;
+            }
+            if (!any_mix) {
+#pragma unroll
+                for (int k = 0; k < kHbSlots; k++) {
+                    const int code = (p.hcode[k][hb] >> dsh) & 15;
+                    if (code & HC_PLAIN) acc[kPlainSlots + k] += E[kPlainSlots + k];
+                }
+            } else {
+                float Eh[kHbSlots] = {0.f, 0.f, 0.f, 0.f};
+                if (dis && p.hydrogen_type >= 0) {
+                    double rdx, rdy, rdz;
+                    const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                    const int hb_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
+                    const float4 v = reinterpret_cast<const float4 *>(
+                        tab + ((size_t)p.hydrogen_type * AG_NEINT + hb_bin) * kMapSlots)[kPlainSlots / 4];
+                    Eh[0] = v.x; Eh[1] = v.y; Eh[2] = v.z; Eh[3] = v.w;
+                }
+                const float dfac = racc - rdon;
+#pragma unroll
+                for (int k = 0; k < kHbSlots; k++) {
+                    const int m = kPlainSlots + k;
+                    const int code = (p.hcode[k][hb] >> dsh) & 15;
+                    const float e = E[m];
+                    if (code & HC_PLAIN) acc[m] += e;
+                    if ((code & ~HC_PLAIN) == 0) continue;
+                    // 0/1 coefficients as scalar float bit patterns
+                    const float kF = __int_as_float((code & HC_RACC) ? 0x3f800000 : 0);
+                    const float rsph = __builtin_amdgcn_fmed3f(e * 0.01f, 0.f, 1.f);
+                    const float f = __builtin_fmaf(kF, dfac, rdon);
+                    const float mix = __builtin_fmaf(1.f - f, rsph, f);
+                    if (code & HC_HRAMP) acc[m] = __builtin_fmaf((dis ? Eh[k] : e) * Hramp, mix, acc[m]);
+                    if (code & HC_MINMAX) {
+                        const float v = e * mix;
+                        hmin[k] = fminf(hmin[k], v);
+                        hmax[k] = fmaxf(hmax[k], v);
+                    }
+                }
             }
             if (p.d_slot >= 0) d_acc += unif(a.dvol) * S;
         }
@@ -751,7 +802,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
 #pragma unroll
         for (int m = 0; m < kMapSlots; m++) {
             if ((m >= NPLAIN && m < kPlainSlots) || p.out_slot[m] < 0) continue;
-            float v = acc[m];
+            float v = acc[m] + __builtin_fmaf(p.sp[m], dv_acc, p.volp[m] * dq_acc);
             if (m >= kPlainSlots && hmin[m - kPlainSlots] < 999999.f)
                 v += hmin[m - kPlainSlots] + fmaxf(hmax[m - kPlainSlots], 0.f);
             maps[(long long)p.out_slot[m] * P + pt] = v;
@@ -875,11 +926,13 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
         const int iz = (int)(pt / ((long long)p.n1[0] * p.n1[1]));
         const double cx = (double)(ix - p.ne[0]) * p.spacing, cy = (double)(iy - p.ne[1]) * p.spacing,
                      cz = (double)(iz - p.ne[2]) * p.spacing;
-        const float pxE = (float)(ix - p.ne[0]) * p.s128, pyE = (float)(iy - p.ne[1]) * p.s128,
-                    pzE = (float)(iz - p.ne[2]) * p.s128;
+        const int kx = ix % kFarK;
+        const float cxE = far_cx(ix - kx, p.ne[0], p.s128), dkE = far_dk(kx, p.s128);
+        const float pyE = (float)(iy - p.ne[1]) * p.s128, pzE = (float)(iz - p.ne[2]) * p.s128;
+        const float biasE = far_bias(p.s128);
         for (int a = 0; a < p.n_atoms; a++) {
             const float4 at = p.far_atoms[a];
-            const int fb = (int)(far_bin4(at.x, at.y, at.z, pxE, pyE, pzE) >> 2);
+            const int fb = (int)(far_bin4(at.x, at.y, at.z, cxE, dkE, pyE, pzE, biasE) >> 2);
             double dx, dy, dz;
             const double r64 = ref_dist(p.coord[3 * a], p.coord[3 * a + 1], p.coord[3 * a + 2], cx, cy, cz, dx, dy, dz);
             const int b64 = min((int)(r64 * 128.0), AG_NDIEL - 1);
@@ -922,14 +975,25 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
 // reference clamps them (main.cpp:2956); without it the table is extended
 // to the largest bin any pair can reach, every entry past 8191 holding the
 // reference's last value, and the clamp instruction disappears.
-// CAP: LDS table capacity in entries (>= tab_len).  The table is the only
-// LDS variable, so it starts at address 0 and a bin's byte offset is its LDS
-// address; one int after it broadcasts the work counter.
+// CAP: LDS table capacity in entries (>= tab_len).  The table is the first
+// thing in the kernel's only LDS array, so it starts at address 0 and a
+// bin's byte offset is its LDS address; after it one int broadcasts the work
+// counter and kFarRuns * kFarK floats carry the second half's sums.
+//
+// An item is kFarRuns runs; the workgroup's two halves (four waves each)
+// take the same runs, the first half the atoms [0, a_mid), the second
+// [a_mid, n), a_mid rounded up to a Kahan block.  The second half leaves its
+// compensated sums in LDS and the first adds them in a fixed order, so the
+// result does not depend on scheduling.  Splitting the atoms keeps four
+// waves per SIMD on grids that give only two per SIMD at one wave per run.
 template <bool CLAMP, int CAP>
-__global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
-    __shared__ float T[CAP + 1];
+__global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_eu(4))) k_far(FarParams p) {
+    __shared__ float T[CAP + 1 + kFarRuns * kFarK];
     int &item_s = reinterpret_cast<int *>(T)[CAP];
+    float *const R = T + CAP + 1;
     for (int i = threadIdx.x; i < p.tab_len; i += kFarThreads) T[i] = p.tab[i];
+    const int half = __builtin_amdgcn_readfirstlane(threadIdx.x / kFarRuns);   // uniform per wave
+    const int lane = threadIdx.x % kFarRuns;
     int item;
     if (p.queue) {
         if (threadIdx.x == 0) item_s = p.item_base + atomicAdd(p.queue, 1);
@@ -943,43 +1007,59 @@ __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
     while (item < p.item_end) {
         const int ci = find_conf(as_const(p.tile_prefix), p.nconf, item);
         const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
-        const int run = (item - as_const(p.tile_prefix)[ci]) * kFarThreads + threadIdx.x;
-        if (run < cf.nruns) {
-            const int row = run / cf.runs_per_row;
-            const int ix0 = (run % cf.runs_per_row) * kFarK;
-            const int n1x = cf.n1[0], n1y = cf.n1[1];
-            const int iy = row % n1y;
-            const int iz = row / n1y;
+        const int run = (item - as_const(p.tile_prefix)[ci]) * kFarRuns + lane;
+        const bool active = run < cf.nruns;
+        const int n1x = cf.n1[0], n1y = cf.n1[1];
+        const int row = run / cf.runs_per_row;
+        const int ix0 = (run % cf.runs_per_row) * kFarK;
+        const int iy = row % n1y;
+        const int iz = row / n1y;
+        float *const emap = p.maps + cf.map_off + (long long)p.e_slot * cf.P;
+        const long long base = (long long)ix0 + (long long)n1x * ((long long)iy + (long long)n1y * iz);
+        float s[kFarK], c[kFarK];
+        if (active) {
             const float py = (float)(iy - cf.ne[1]) * cf.s128;
             const float pz = (float)(iz - cf.ne[2]) * cf.s128;
-            float px[kFarK], s[kFarK], c[kFarK];
-            float *const emap = p.maps + cf.map_off + (long long)p.e_slot * cf.P;
-            const long long base = (long long)ix0 + (long long)n1x * ((long long)iy + (long long)n1y * iz);
+            const float cx = far_cx(ix0, cf.ne[0], cf.s128);
+            const float bias = far_bias(cf.s128);
+            // -2 dk per point; dk^2 per mirror pair (k, kFarK-1-k), j = k - kFarK/2
+            float m[kFarK], d2[kFarK / 2];
 #pragma unroll
             for (int k = 0; k < kFarK; k++) {
-                px[k] = (float)(ix0 + k - cf.ne[0]) * cf.s128;
-                s[k] = (ix0 + k < n1x) ? emap[base + k] : 0.f;
+                const float dk = far_dk(k, cf.s128);
+                m[k] = -2.f * dk;
+                if (k >= kFarK / 2) d2[k - kFarK / 2] = dk * dk;
+                s[k] = (half == 0 && ix0 + k < n1x) ? emap[base + k] : 0.f;
                 c[k] = 0.f;
             }
-            const float4 *__restrict__ atoms = p.atoms + cf.atom_base;
+            // atoms are read-only here: scalar loads, one atom ahead
+            const AG_CONST PlainF4 *atoms = as_const(reinterpret_cast<const PlainF4 *>(p.atoms + cf.atom_base));
             const int n_atoms = cf.n_atoms;
-            for (int a0 = 0; a0 < n_atoms; a0 += kFarAtomChunk) {
-                const int a1 = min(a0 + kFarAtomChunk, n_atoms);
+            const int a_mid = min(n_atoms, ((n_atoms + 1) / 2 + kFarAtomChunk - 1) / kFarAtomChunk * kFarAtomChunk);
+            const int a_beg = half ? a_mid : 0;
+            const int a_end = half ? n_atoms : a_mid;
+            PlainF4 nxt = ld4(atoms, min(a_beg, max(a_end - 1, 0)));
+            for (int a0 = a_beg; a0 < a_end; a0 += kFarAtomChunk) {
+                const int a1 = min(a0 + kFarAtomChunk, a_end);
                 float part[kFarK];
 #pragma unroll
                 for (int k = 0; k < kFarK; k++) part[k] = 0.f;
-#pragma unroll 2
                 for (int a = a0; a < a1; a++) {
-#pragma clang fp contract(off)
-                    const float4 at = atoms[a];
+                    const PlainF4 at = nxt;
+                    nxt = ld4(atoms, min(a + 1, a_end - 1));
                     const float dy = at.y - py, dz = at.z - pz;
-                    const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, kFarBias));
+                    const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, bias));
+                    const float u = at.x - cx;
+                    const float U = __builtin_fmaf(u, u, dyz2);
+                    float W[kFarK / 2];
+#pragma unroll
+                    for (int j = 0; j < kFarK / 2; j++) W[j] = U + d2[j];
                     float rs[kFarK];
                     unsigned off[kFarK];
 #pragma unroll
                     for (int k = 0; k < kFarK; k++) {
-                        const float dx = at.x - px[k];
-                        const float rb2 = __builtin_fmaf(dx, dx, dyz2);
+                        const int j = k >= kFarK / 2 ? k - kFarK / 2 : kFarK / 2 - 1 - k;
+                        const float rb2 = __builtin_fmaf(u, m[k], W[j]);
                         rs[k] = __builtin_amdgcn_rsqf(rb2);
                         off[k] = bin4_of(rb2, rs[k]);
                         if (CLAMP) off[k] = min(off[k], p.tmax4);
@@ -993,16 +1073,25 @@ __global__ void __launch_bounds__(kFarThreads) k_far(FarParams p) {
                 }
 #pragma unroll
                 for (int k = 0; k < kFarK; k++) {
-#pragma clang fp contract(off)
                     const float y = part[k] - c[k];
                     const float t = s[k] + y;
                     c[k] = (t - s[k]) - y;
                     s[k] = t;
                 }
             }
+            if (half) {
 #pragma unroll
-            for (int k = 0; k < kFarK; k++)
-                if (ix0 + k < n1x) emap[base + k] = s[k];
+                for (int k = 0; k < kFarK; k++) R[k * kFarRuns + lane] = s[k] - c[k];
+            }
+        }
+        __syncthreads();
+        if (half == 0 && active) {
+#pragma unroll
+            for (int k = 0; k < kFarK; k++) {
+                const float y = R[k * kFarRuns + lane] - c[k];
+                const float t = s[k] + y;
+                if (ix0 + k < n1x) emap[base + k] = t;
+            }
         }
         if (!p.queue) break;
         __syncthreads();
@@ -1230,7 +1319,7 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms) {
         p.brick_prefix[c] = (int)brick;
         p.tile_prefix[c] = (int)tile;
         brick += (long long)cf.nb[0] * cf.nb[1] * cf.nb[2];
-        tile += (cf.nruns + kFarThreads - 1) / kFarThreads;
+        tile += (cf.nruns + kFarRuns - 1) / kFarRuns;
         const long long cells = (long long)cf.nc[0] * cf.nc[1] * cf.nc[2];
         if (cell_base + cells > INT32_MAX / 2 || brick > INT32_MAX / 2 || tile > INT32_MAX / 2)
             ag::fail(AG_ERR_UNSUPPORTED, "batch too large for one plan");

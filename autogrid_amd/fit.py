@@ -74,8 +74,17 @@ class PoseScorer:
 def fit_weights(ctx: Context, st: PreparedState, poses, scale_true, steps: int = 200, lr: float = 0.02):
     """Fit the four AD4 weights to the scores the maps give at
     weights0 * scale_true, starting from weights0, with Adam on log-weights.
-    Returns a dict with the loss history, the fitted and the true weights
-    and the time per step (forward + adjoint)."""
+
+    The loss is the mean relative squared error ((s - t) / (1 + |t|))^2.
+    Poses drawn uniformly in a box inside the receptor span scores from
+    about -3 to 1e6 kcal/mol (steric clashes, pair energies capped at 1e5):
+    under a plain squared error the clash poses fix the vdW and H-bond
+    weights and leave the electrostatic and desolvation weights below the
+    FP32 rounding of those scores (singular values of d score / d log w
+    span 1.4e6 to 8 on the 5,000-atom, 65^3 workload), so they are not
+    recovered.  Scaling each residual by its target gives every pose the
+    same say.  Returns a dict with the loss history, the fitted and the
+    true weights and the time per step (forward + adjoint)."""
     import time
     eng = Analytic(ctx, st)
     scorer = PoseScorer(st, poses)
@@ -87,6 +96,7 @@ def fit_weights(ctx: Context, st: PreparedState, poses, scale_true, steps: int =
     w_true = w0 * torch.as_tensor(scale_true, dtype=torch.float64)
     with torch.no_grad():
         target = scorer(AnalyticMaps.apply(coord, charge, rij, eps, w_true, eng)).detach()
+    inv = 1.0 / (1.0 + target.abs())
     theta = torch.zeros(4, dtype=torch.float64, requires_grad=True)
     opt = torch.optim.Adam([theta], lr=lr)
     hist = []
@@ -95,7 +105,7 @@ def fit_weights(ctx: Context, st: PreparedState, poses, scale_true, steps: int =
         opt.zero_grad()
         w = w0 * torch.exp(theta)
         s = scorer(AnalyticMaps.apply(coord, charge, rij, eps, w, eng))
-        loss = torch.mean((s - target) ** 2)
+        loss = torch.mean(((s - target) * inv) ** 2)
         loss.backward()
         opt.step()
         hist.append(float(loss.detach().cpu()))

@@ -250,8 +250,8 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_calibration}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the map set shards by "
-              "z-slab across GPUs without communication; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a map set would shard by "
+              "z-slab across GPUs without communication).", file=sys.stderr)
         sys.exit(2)
 
     import autogrid_amd as ag

@@ -1,6 +1,6 @@
 """The FP64 oracle against the reference autogrid4 and closed forms.
 
-Pins, per SURVEY-style parity plan:
+What pins the oracle:
   * golden.json holds SHA-256 hashes of the reference binary's map text for
     every case; the oracle's text must hash the same (byte-identical maps);
   * when oracle/_ref/autogrid4 is built here, it is also run live;

@@ -91,17 +91,16 @@ class PreparedState:
     def n_atom_maps(self) -> int:
         return int(len(self.maps))
 
-    def structs(self):
-        """ctypes (Receptor, Grid, MapSet, Tables, Setup) pointing into this state."""
+    def receptor_grid(self):
+        """ctypes (Receptor, Grid) pointing into this state, and the arrays
+        that must outlive them."""
         arrs = dict(
             coord=np.ascontiguousarray(self.coord, np.float64), charge=np.ascontiguousarray(self.charge, np.float64),
             vol=np.ascontiguousarray(self.vol, np.float64), solpar=np.ascontiguousarray(self.solpar, np.float64),
             type=np.ascontiguousarray(self.type, np.int32), hbond=np.ascontiguousarray(self.hbond, np.int32),
             rexp=np.ascontiguousarray(self.rexp, np.int32), rvector=np.ascontiguousarray(self.rvector, np.float64),
             rvector2=np.ascontiguousarray(self.rvector2, np.float64),
-            disorder=np.ascontiguousarray(self.disorder, np.uint8),
-            e_vdw_hb=np.ascontiguousarray(self.e_vdw_hb, np.float32),
-            sol_fn=np.ascontiguousarray(self.sol_fn, np.float64), r_eps=np.ascontiguousarray(self.r_eps, np.float64))
+            disorder=np.ascontiguousarray(self.disorder, np.uint8))
         P = _abi._ptr
         rec = _abi.Receptor(self.n_atoms, P(arrs["coord"], C.c_double), P(arrs["charge"], C.c_double),
                             P(arrs["vol"], C.c_double), P(arrs["solpar"], C.c_double),
@@ -109,6 +108,15 @@ class PreparedState:
                             P(arrs["rexp"], C.c_int32), P(arrs["rvector"], C.c_double),
                             P(arrs["rvector2"], C.c_double), P(arrs["disorder"], C.c_uint8))
         grid = _abi.Grid((C.c_int32 * 3)(*self.n1), (C.c_int32 * 3)(*self.ne), float(self.spacing))
+        return rec, grid, arrs
+
+    def structs(self):
+        """ctypes (Receptor, Grid, MapSet, Tables, Setup) pointing into this state."""
+        rec, grid, arrs = self.receptor_grid()
+        arrs.update(e_vdw_hb=np.ascontiguousarray(self.e_vdw_hb, np.float32),
+                    sol_fn=np.ascontiguousarray(self.sol_fn, np.float64),
+                    r_eps=np.ascontiguousarray(self.r_eps, np.float64))
+        P = _abi._ptr
         md = (_abi.MapDesc * max(1, self.n_atom_maps))()
         for i, m in enumerate(self.maps):
             md[i] = _abi.MapDesc(int(m["is_covalent"]), int(m["is_hbonder"]), int(m["hbond"]), 0,
@@ -126,8 +134,10 @@ class PreparedState:
         keep = [arrs, md]
         if self.pairs is not None:
             pa = (_abi.Pair * max(1, self.pairs.size))()
-            for i, p in enumerate(self.pairs.reshape(-1)):
-                pa[i] = _abi.Pair(*[p[k].item() for k in ("cA", "cB", "Rij", "eps", "xA", "xB", "hbonder")], 0)
+            view = np.frombuffer(pa, dtype=PAIR_ABI_DTYPE)
+            flat = self.pairs.reshape(-1)
+            for k in ("cA", "cB", "Rij", "eps", "xA", "xB", "hbonder"):
+                view[k][:flat.size] = flat[k]
             setup.pairs = C.cast(pa, C.POINTER(_abi.Pair))
             keep.append(pa)
         setup.r_smooth = self.r_smooth
@@ -145,6 +155,9 @@ MAP_DTYPE = np.dtype([("is_covalent", np.int32), ("is_hbonder", np.int32), ("hbo
                       ("solpar_probe", np.float64), ("vol_probe", np.float64)])
 PAIR_DTYPE = np.dtype([("cA", np.float64), ("cB", np.float64), ("Rij", np.float64), ("eps", np.float64),
                        ("xA", np.int32), ("xB", np.int32), ("hbonder", np.int32)])
+# the C layout of ag_pair (one int32 of padding)
+PAIR_ABI_DTYPE = np.dtype([("cA", np.float64), ("cB", np.float64), ("Rij", np.float64), ("eps", np.float64),
+                           ("xA", np.int32), ("xB", np.int32), ("hbonder", np.int32), ("reserved", np.int32)])
 
 
 def _arr(ptr, n, dtype, shape=None):
@@ -341,12 +354,14 @@ def _batch_structs(states: list):
     keep = []
     ms = tab = None
     for i, st in enumerate(states):
-        r, g, m, t, _ = st.structs()
-        keep.append(list(st._keep))
+        if i == 0:
+            r, g, ms, tab, _ = st.structs()
+            keep.append(list(st._keep))
+        else:
+            r, g, arrs = st.receptor_grid()
+            keep.append(arrs)
         recs[i] = r
         grids[i] = g
-        if i == 0:
-            ms, tab = m, t
     return recs, grids, ms, tab, keep
 
 

@@ -4,7 +4,12 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "ag_internal.h"
 
@@ -18,28 +23,111 @@
 
 constexpr int kMaxChunks = 16;                    // pipeline stages of a batch with host outputs
 
+// Device blocks that destroyed plans released, kept by their context for
+// the next plan: a run of plans (one per receptor, or one per batch) then
+// allocates from HIP once.  Best fit among blocks at most twice the request
+// (plus 1 MiB); at most kMaxHeld bytes are held, the rest is freed.
+struct DevPool {
+    static constexpr size_t kMaxHeld = size_t(16) << 30;
+    std::mutex mu;
+    std::vector<std::pair<size_t, void *>> blocks;   // (capacity, pointer)
+    size_t held = 0;
+    void *take(size_t bytes, size_t &cap) {
+        std::lock_guard<std::mutex> lk(mu);
+        int best = -1;
+        for (int i = 0; i < (int)blocks.size(); i++) {
+            const size_t c = blocks[i].first;
+            if (c >= bytes && c <= 2 * bytes + (size_t(1) << 20) && (best < 0 || c < blocks[best].first)) best = i;
+        }
+        if (best < 0) return nullptr;
+        void *p = blocks[best].second;
+        cap = blocks[best].first;
+        held -= cap;
+        blocks.erase(blocks.begin() + best);
+        return p;
+    }
+    void give(void *p, size_t cap) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (held + cap > kMaxHeld) {
+            (void)hipFree(p);
+            return;
+        }
+        blocks.emplace_back(cap, p);
+        held += cap;
+    }
+    ~DevPool() {
+        for (auto &b : blocks) (void)hipFree(b.second);
+    }
+};
+
+// Growable device buffer.  Capacities are rounded up to 1/8 of the next
+// power of two, so buffers of similar size can reuse each other's blocks.
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
+    std::shared_ptr<DevPool> pool;
+    static size_t rounded(size_t bytes) {
+        size_t p2 = 256;
+        while (p2 < bytes) p2 <<= 1;
+        const size_t step = std::max<size_t>(256, p2 / 8);
+        return (bytes + step - 1) / step * step;
+    }
+    void release() {
+        if (!ptr) return;
+        if (pool) pool->give(ptr, cap);
+        else (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
     void *get(size_t bytes) {
         if (bytes == 0) bytes = 16;
         if (bytes > cap) {
-            if (ptr) HIP_CHECK(hipFree(ptr));
-            ptr = nullptr;
-            cap = 0;
-            if (hipMalloc(&ptr, bytes) != hipSuccess) {
-                ptr = nullptr;
-                ag::fail(AG_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+            release();
+            if (pool) ptr = pool->take(bytes, cap);
+            if (!ptr) {
+                const size_t want = rounded(bytes);
+                if (hipMalloc(&ptr, want) != hipSuccess) {
+                    (void)hipGetLastError();
+                    ptr = nullptr;
+                    ag::fail(AG_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+                }
+                cap = want;
             }
-            cap = bytes;
         }
         return ptr;
     }
     DevBuf() = default;
+    explicit DevBuf(std::shared_ptr<DevPool> p) : pool(std::move(p)) {}
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() {
-        if (ptr) (void)hipFree(ptr);
+    ~DevBuf() { release(); }
+};
+
+// Grow-only pinned host buffer: staging for uploads, so they run as DMA
+// from page-locked memory.
+struct HostStage {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes) {
+        if (bytes > cap) {
+            if (ptr) (void)hipHostFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+            const size_t want = DevBuf::rounded(bytes);
+            if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                ptr = nullptr;
+                ag::fail(AG_ERR_OOM, "pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+            }
+            cap = want;
+        }
+        return ptr;
+    }
+    HostStage() = default;
+    HostStage(const HostStage &) = delete;
+    HostStage &operator=(const HostStage &) = delete;
+    ~HostStage() {
+        if (ptr) (void)hipHostFree(ptr);
     }
 };
 
@@ -47,8 +135,12 @@ struct ag_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;   // device-to-host copies overlapping compute
+    hipStream_t stream2 = nullptr;       // second compute stream: alternate chunks of a batch
+    hipEvent_t cells_ev = nullptr;       // cell list done (orders stream2 after it)
     hipEvent_t chunk_ev[kMaxChunks + 1] = {};   // ordering between the two streams
+    std::shared_ptr<DevPool> pool = std::make_shared<DevPool>();   // blocks of destroyed plans
     DevBuf maps;                       // output staging of ag_maps_plain
+    HostStage stage;                   // receptor arrays of a plan on their way up
     hipEvent_t ev[4] = {};
     float last_ms[4] = {0, 0, 0, 0};   // cell list, near, far, total of the last execute
     int n_cu = 0;

@@ -1118,6 +1118,8 @@ struct Tune {
 // Receptor conformations (one for a plan, many for a batch) on their grids,
 // with a shared map set and tables, resident on one device.
 struct ag_plan {
+    explicit ag_plan(std::shared_ptr<DevPool> pl) : pool(std::move(pl)) {}
+    std::shared_ptr<DevPool> pool;     // first: the buffers below draw on it
     ag_ctx *ctx = nullptr;
     int nconf = 0, N = 0, M = 0, T = 0, n_maps = 0, groups = 0;
     bool vina = false, persistent = false;
@@ -1127,10 +1129,12 @@ struct ag_plan {
     int total_cells = 0, total_bricks = 0, total_tiles = 0;
     Tune tune;
     double tune_ms = 0.0;              // time the autotuner spent
-    DevBuf confs_d, brick_prefix_d, tile_prefix_d, atom_conf_d, queue_d;
-    DevBuf coord, charge, vol, solpar, rvec, rvec2, type, hbond, rexp, disorder;
-    DevBuf atom_cell, cell_count, cell_start, cell_cursor, slot_atom, near_atoms, far_atoms, fmin_atoms;
-    DevBuf tab, sol, reps, fartab, status;
+    DevBuf confs_d{pool}, brick_prefix_d{pool}, tile_prefix_d{pool}, atom_conf_d{pool}, queue_d{pool};
+    DevBuf coord{pool}, charge{pool}, vol{pool}, solpar{pool}, rvec{pool}, rvec2{pool}, type{pool}, hbond{pool},
+        rexp{pool}, disorder{pool};
+    DevBuf atom_cell{pool}, cell_count{pool}, cell_start{pool}, cell_cursor{pool}, slot_atom{pool},
+        near_atoms{pool}, far_atoms{pool}, fmin_atoms{pool};
+    DevBuf tab{pool}, sol{pool}, reps{pool}, fartab{pool}, status{pool};
     AtomSrc src{};
     std::vector<NearParams> near;      // one per pass of 16 map slots
     int nplain = 12;                   // plain slots in use (4, 8 or 12)
@@ -1163,6 +1167,8 @@ ag_status context_create(ag_ctx **out, int device) {
     c->device = device;
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_CHECK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&c->cells_ev, hipEventDisableTiming));
     for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
     for (auto &e : c->chunk_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
@@ -1175,6 +1181,9 @@ void context_destroy(ag_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->cells_ev) (void)hipEventDestroy(c->cells_ev);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : c->chunk_ev)
@@ -1368,7 +1377,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     if (n < 1) ag::fail(AG_ERR_BAD_INPUT, "a plan needs at least one conformation");
     for (int i = 0; i < n; i++) validate(&recs[i], &grids[i], ms, tab, vina);
     HIP_CHECK(hipSetDevice(c->device));
-    std::unique_ptr<ag_plan> pl(new ag_plan());
+    std::unique_ptr<ag_plan> pl(new ag_plan(c->pool));
     ag_plan &p = *pl;
     p.ctx = c;
     p.nconf = n;
@@ -1387,36 +1396,51 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     for (int i = 0; i < n; i++) Nl += (natoms[i] = recs[i].n_atoms);
     if (Nl > INT32_MAX / 4) ag::fail(AG_ERR_UNSUPPORTED, "too many atoms in one batch");
     const int N = p.N = (int)Nl;
-    std::vector<double> coord(3 * (size_t)N), charge(N), vol(N), solpar(N), rvec(3 * (size_t)N), rvec2(3 * (size_t)N);
-    std::vector<int> type(N), hbond(N), rexp(N), atom_conf(N);
-    std::vector<unsigned char> dis(N);
+    // concatenated receptor arrays, staged in the context's pinned buffer
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t n3 = 3 * (size_t)N, nn = (size_t)N;
+    const size_t o_coord = carve(8 * n3), o_charge = carve(8 * nn), o_vol = carve(8 * nn), o_solpar = carve(8 * nn);
+    const size_t o_rvec = carve(8 * n3), o_rvec2 = carve(8 * n3), o_type = carve(4 * nn), o_hbond = carve(4 * nn);
+    const size_t o_rexp = carve(4 * nn), o_conf = carve(4 * nn), o_dis = carve(nn);
+    char *const h = (char *)c->stage.get(off + 256);
+    double *const coord = (double *)(h + o_coord), *const charge = (double *)(h + o_charge);
+    double *const vol = (double *)(h + o_vol), *const solpar = (double *)(h + o_solpar);
+    double *const rvec = (double *)(h + o_rvec), *const rvec2 = (double *)(h + o_rvec2);
+    int *const type = (int *)(h + o_type), *const hbond = (int *)(h + o_hbond), *const rexp = (int *)(h + o_rexp);
+    int *const atom_conf = (int *)(h + o_conf);
+    unsigned char *const dis = (unsigned char *)(h + o_dis);
     for (int i = 0, o = 0; i < n; o += natoms[i], i++) {
         const ag_receptor &r = recs[i];
         const size_t k = natoms[i];
         if (!k) continue;
-        std::copy(r.coord, r.coord + 3 * k, coord.begin() + 3 * (size_t)o);
-        std::copy(r.charge, r.charge + k, charge.begin() + o);
-        std::copy(r.vol, r.vol + k, vol.begin() + o);
-        std::copy(r.solpar, r.solpar + k, solpar.begin() + o);
-        std::copy(r.rvector, r.rvector + 3 * k, rvec.begin() + 3 * (size_t)o);
-        std::copy(r.rvector2, r.rvector2 + 3 * k, rvec2.begin() + 3 * (size_t)o);
-        std::copy(r.type, r.type + k, type.begin() + o);
-        std::copy(r.hbond, r.hbond + k, hbond.begin() + o);
-        std::copy(r.rexp, r.rexp + k, rexp.begin() + o);
-        std::copy(r.disorder, r.disorder + k, dis.begin() + o);
-        std::fill(atom_conf.begin() + o, atom_conf.begin() + o + k, i);
+        std::copy(r.coord, r.coord + 3 * k, coord + 3 * (size_t)o);
+        std::copy(r.charge, r.charge + k, charge + o);
+        std::copy(r.vol, r.vol + k, vol + o);
+        std::copy(r.solpar, r.solpar + k, solpar + o);
+        std::copy(r.rvector, r.rvector + 3 * k, rvec + 3 * (size_t)o);
+        std::copy(r.rvector2, r.rvector2 + 3 * k, rvec2 + 3 * (size_t)o);
+        std::copy(r.type, r.type + k, type + o);
+        std::copy(r.hbond, r.hbond + k, hbond + o);
+        std::copy(r.rexp, r.rexp + k, rexp + o);
+        std::copy(r.disorder, r.disorder + k, dis + o);
+        std::fill(atom_conf + o, atom_conf + o + k, i);
     }
-    double *d_coord = upload(st, p.coord, coord.data(), coord.size());
-    double *d_charge = upload(st, p.charge, charge.data(), charge.size());
-    double *d_vol = upload(st, p.vol, vol.data(), vol.size());
-    double *d_solpar = upload(st, p.solpar, solpar.data(), solpar.size());
-    double *d_rvec = upload(st, p.rvec, rvec.data(), rvec.size());
-    double *d_rvec2 = upload(st, p.rvec2, rvec2.data(), rvec2.size());
-    int *d_type = upload(st, p.type, type.data(), type.size());
-    int *d_hbond = upload(st, p.hbond, hbond.data(), hbond.size());
-    int *d_rexp = upload(st, p.rexp, rexp.data(), rexp.size());
-    unsigned char *d_dis = upload(st, p.disorder, dis.data(), dis.size());
-    if (n > 1) upload(st, p.atom_conf_d, atom_conf.data(), atom_conf.size());
+    double *d_coord = upload(st, p.coord, coord, n3);
+    double *d_charge = upload(st, p.charge, charge, nn);
+    double *d_vol = upload(st, p.vol, vol, nn);
+    double *d_solpar = upload(st, p.solpar, solpar, nn);
+    double *d_rvec = upload(st, p.rvec, rvec, n3);
+    double *d_rvec2 = upload(st, p.rvec2, rvec2, n3);
+    int *d_type = upload(st, p.type, type, nn);
+    int *d_hbond = upload(st, p.hbond, hbond, nn);
+    int *d_rexp = upload(st, p.rexp, rexp, nn);
+    unsigned char *d_dis = upload(st, p.disorder, dis, nn);
+    if (n > 1) upload(st, p.atom_conf_d, atom_conf, nn);
     p.src = AtomSrc{d_coord, d_charge, d_vol, d_solpar, d_rvec, d_rvec2, d_type, d_hbond, d_rexp, d_dis,
                     ms->coeff_estat, ms->solpar_q};
     p.atom_cell.get(sizeof(int) * (size_t)N);
@@ -1654,7 +1678,17 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
                                                       (NearAtom *)p->near_atoms.ptr);
     HIP_CHECK(hipGetLastError());
     if (events) HIP_CHECK(hipEventRecord(c->ev[1], st));
+    // With host outputs the chunks alternate between two compute streams,
+    // so one chunk's far-kernel tail overlaps the next chunk's work (chunks
+    // touch disjoint conformations; both streams follow the cell list).
+    const bool two = host && nchunks > 1;
+    if (two) {
+        HIP_CHECK(hipEventRecord(c->cells_ev, st));
+        HIP_CHECK(hipStreamWaitEvent(c->stream2, c->cells_ev, 0));
+    }
+    const hipStream_t st_main = st;
     for (int k = 0; k < nchunks; k++) {
+        st = (two && (k & 1)) ? c->stream2 : st_main;
         const int c0 = chunks[k].first, c1 = chunks[k].second;
         const int b0 = p->brick_prefix[c0], b1 = p->brick_prefix[c1];
         for (int g = 0; g < p->groups; g++) {
@@ -1731,7 +1765,10 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
             }
         }
     }
+    st = st_main;
     if (host) {
+        // the copy stream has waited for every chunk, so this joins both
+        // compute streams as well
         HIP_CHECK(hipEventRecord(c->chunk_ev[kMaxChunks], c->copy_stream));
         HIP_CHECK(hipStreamWaitEvent(st, c->chunk_ev[kMaxChunks], 0));
     }
@@ -1848,6 +1885,8 @@ void plan_destroy(ag_plan *p) {
     if (!p) return;
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
+    (void)hipStreamSynchronize(p->ctx->stream2);
+    (void)hipStreamSynchronize(p->ctx->copy_stream);
     delete p;
 }
 
@@ -2014,11 +2053,13 @@ static bool is_pinned(const void *ptr) {
 
 // Batch with host outputs: maps_out[i] receives conformation i's
 // [n_maps][P_i] floats (fmap_out / cmap_out its [P_i] values when asked).
-// When every destination is pinned host memory, one graph holds the whole
-// pipeline: the conformations run in chunks of about equal work and each
-// chunk's maps are copied to the host on a second stream while the next
-// chunk computes.  Otherwise the graph computes everything and the copies
-// follow it.
+// When every destination is pinned host memory the conformations run in
+// chunks of about equal work, launched directly and alternating between
+// the two compute streams, and each chunk's maps are copied to the host on
+// the copy stream behind an event while later chunks compute.  (A captured graph would run its
+// copy nodes in line with its kernels, serialising the two; a one-shot
+// batch has few launches to save, so it is not captured.)  Otherwise the
+// plan's graph computes everything and the copies follow it.
 void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
                 const ag_tables *tab, float *const *maps_out, float *const *fmap_out, int32_t *const *cmap_out,
                 int potential, const ag_tune *tune) {
@@ -2042,37 +2083,11 @@ void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids,
     hipStream_t st = c->stream;
     if (pinned) {
         const HostOut host{maps_out, fmap_out, cmap_out};
-        const auto chunks = plan_chunks(*p, 8);
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        try {
-            enqueue(p.get(), d_maps, d_fmap, d_cmap, false, false, chunks, &host);
-        } catch (...) {
-            hipGraph_t dead = nullptr;
-            (void)hipStreamEndCapture(st, &dead);
-            if (dead) (void)hipGraphDestroy(dead);
-            throw;
-        }
-        HIP_CHECK(hipStreamEndCapture(st, &g));
-        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess)
-            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
-        hipError_t le = hipEventRecord(c->ev[0], st);
-        if (le == hipSuccess) le = hipGraphLaunch(ge, st);
-        if (le == hipSuccess) le = hipEventRecord(c->ev[3], st);
-        if (le != hipSuccess) {
-            (void)hipGraphExecDestroy(ge);
-            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(le) + " at hipGraphLaunch");
-        }
-        try {
-            check_status(p.get());
-        } catch (...) {
-            (void)hipGraphExecDestroy(ge);
-            throw;
-        }
-        (void)hipGraphExecDestroy(ge);
+        const auto chunks = plan_chunks(*p, 12);
+        HIP_CHECK(hipEventRecord(c->ev[0], st));
+        enqueue(p.get(), d_maps, d_fmap, d_cmap, false, false, chunks, &host);
+        HIP_CHECK(hipEventRecord(c->ev[3], st));
+        check_status(p.get());
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
         c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = -1.f;
         return;

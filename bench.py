@@ -79,6 +79,22 @@ def cpu_baseline(workload: str, seconds_target: float = 20.0) -> dict | None:
         shutil.rmtree(d, ignore_errors=True)
 
 
+TRAFFIC_PROFILE = "profiles/full_mapset_5000atoms_127cubed.md"
+
+
+def profiled_traffic():
+    """HBM bytes per k_far launch on the full workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated by a device copy);
+    None when the summary is missing."""
+    import re
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            m = re.search(r"traffic_bytes_per_launch k_far: (\d+)", f.read())
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
+
+
 def batched_leg(reps: int = 2) -> dict:
     """The 64-conformation ensemble: batched against sequential single runs."""
     import numpy as np
@@ -326,7 +342,9 @@ def main():
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "frac_nonpacked_peak": achieved / FP32_PEAK_NONPACKED_TFLOPS,
-                     "flops_per_launch": flops, "avg_launch_ms": far_avg, "traffic": None},
+                     "flops_per_launch": flops, "avg_launch_ms": far_avg,
+                     "traffic": profiled_traffic() if args.workload == "full" else None,
+                     "traffic_source": TRAFFIC_PROFILE if args.workload == "full" else None},
         "kernels_ms": {"cell_list": sum(cell_ms) / len(cell_ms), "near": sum(near_ms) / len(near_ms),
                        "far": far_avg},
         "pair_evals_per_s": P * N / (far_avg * 1e-3),

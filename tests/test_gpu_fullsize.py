@@ -69,3 +69,23 @@ def test_small_audit_and_determinism(tmp_path):
     print("small audit", audit, flush=True)
     assert audit["near_unguarded_flips"] == 0
     assert audit["far_pairs"] > 0
+
+
+def test_fullsize_charge_symmetries(tmp_path):
+    """Size-independent properties of the full workload: every term of the
+    e map is linear in one charge and FP32 rounding commutes with negation
+    and with scaling by 2, so e(-q) = -e(q) and e(2q) = 2 e(q) bit for bit;
+    the affinity and d maps see charges only through |q| (desolvation), so
+    negating every charge leaves them bit for bit unchanged."""
+    print("[gpu] fullsize charge symmetries", flush=True)
+    st = _state("full", tmp_path)
+    e = st.elec_index
+    with ag.Context(0) as ctx:
+        base = ctx.maps_plain(st)
+        neg = ctx.maps_plain(dataclasses.replace(st, charge=-st.charge, _keep=[]))
+        dbl = ctx.maps_plain(dataclasses.replace(st, charge=2.0 * st.charge, _keep=[]))
+    assert np.array_equal(neg[e], -base[e])
+    assert np.array_equal(dbl[e], 2.0 * base[e])
+    others = [k for k in range(st.n_maps) if k != e]
+    assert np.array_equal(neg[others], base[others])
+    assert np.any(base[e] != 0)

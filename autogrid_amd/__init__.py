@@ -432,6 +432,57 @@ def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program
         ctx.run_gpf(gpf_path, glg_path, program)
 
 
+class Scorer:
+    """Trilinear ligand scoring against a map set on one GPU (ag_scorer_*).
+
+    ``poses`` is a list of (map index [n], charge [n], position [n, 3] grid
+    frame) as poses.make_poses returns.  score() writes the poses' scores
+    (and optionally each atom's gradient) into device memory; adjoint()
+    writes dL/dmaps = S^T dL/dscores."""
+
+    def __init__(self, ctx: "Context", st: PreparedState, poses, outside: float = 0.0):
+        starts = np.zeros(len(poses) + 1, np.int32)
+        starts[1:] = np.cumsum([len(p[0]) for p in poses])
+        xyz = np.ascontiguousarray(np.concatenate([p[2] for p in poses]), np.float64).reshape(-1, 3)
+        mi = np.ascontiguousarray(np.concatenate([p[0] for p in poses]), np.int32)
+        q = np.ascontiguousarray(np.concatenate([p[1] for p in poses]), np.float64)
+        grid = _abi.Grid((C.c_int32 * 3)(*st.n1), (C.c_int32 * 3)(*st.ne), float(st.spacing))
+        h = C.c_void_p()
+        check(lib().ag_scorer_create(ctx.handle, C.byref(grid), st.n_maps, st.elec_index, st.dsolv_index,
+                                     len(poses), starts.ctypes.data, xyz.ctypes.data, mi.ctypes.data, q.ctypes.data,
+                                     float(outside), C.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        self.n_poses = len(poses)
+        self.n_atoms = int(starts[-1])
+        self.shape = (st.n_maps, st.n1[2], st.n1[1], st.n1[0])
+
+    def set_positions(self, xyz) -> None:
+        a = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        assert a.size == 3 * self.n_atoms
+        check(lib().ag_scorer_set_positions(self._h, a.ctypes.data))
+
+    def score(self, maps_ptr: int, scores_ptr: int, grad_ptr: int | None = None) -> None:
+        check(lib().ag_scorer_score(self._h, C.c_void_p(maps_ptr), C.c_void_p(scores_ptr),
+                                    C.c_void_p(grad_ptr) if grad_ptr else None))
+
+    def adjoint(self, dscore_ptr: int, dmaps_ptr: int) -> None:
+        check(lib().ag_scorer_adjoint(self._h, C.c_void_p(dscore_ptr), C.c_void_p(dmaps_ptr)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ag_scorer_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 class Analytic:
     """Analytic (differentiable) map model of a state on one GPU
     (ag_analytic_*): forward maps into device memory, adjoint gradients of

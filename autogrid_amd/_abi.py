@@ -157,11 +157,19 @@ def lib() -> C.CDLL:
         L.ag_analytic_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(Weights)]
         L.ag_analytic_destroy.argtypes = [vp]
         L.ag_job_weights.argtypes = [vp, C.POINTER(Weights)]
+        L.ag_scorer_create.argtypes = [vp, C.POINTER(Grid), C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
+                                       C.c_double, C.POINTER(vp)]
+        L.ag_scorer_set_positions.argtypes = [vp, vp]
+        L.ag_scorer_score.argtypes = [vp, vp, vp, vp]
+        L.ag_scorer_adjoint.argtypes = [vp, vp, vp]
+        L.ag_scorer_destroy.argtypes = [vp]
         L.ag_bond_vectors_vjp.argtypes = [C.POINTER(Setup), C.c_int, vp, vp, vp, vp, vp, vp]
         L.ag_bond_vectors_vjp.restype = C.c_int
         for name in ("ag_build_tables", "ag_bond_vectors", "ag_batch_create", "ag_plan_layout", "ag_plan_tune",
                      "ag_maps_batch", "ag_bond_vectors_batch", "ag_analytic_create", "ag_analytic_set",
-                     "ag_analytic_forward", "ag_analytic_adjoint", "ag_analytic_destroy", "ag_job_weights"):
+                     "ag_analytic_forward", "ag_analytic_adjoint", "ag_analytic_destroy", "ag_job_weights",
+                     "ag_scorer_create", "ag_scorer_set_positions", "ag_scorer_score", "ag_scorer_adjoint",
+                     "ag_scorer_destroy"):
             getattr(L, name).restype = C.c_int
         for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_plan_audit", "ag_context_timing",
                      "ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",

@@ -185,6 +185,7 @@ void log_plane_progress(Job &job, double seconds);
 struct ag_ctx;
 struct ag_plan;
 struct ag_analytic;
+struct ag_scorer;
 namespace agdev {
 ag_status context_create(ag_ctx **out, int device);
 void context_destroy(ag_ctx *ctx);
@@ -218,4 +219,12 @@ void analytic_forward(ag_analytic *a, float *d_maps);
 void analytic_adjoint(ag_analytic *a, const float *d_dmaps, double *g_coord, double *g_charge, double *g_rij,
                       double *g_eps0, double g_w[4]);
 void analytic_info(const ag_analytic *a, int *n_atoms, int *n_maps, int64_t *points, int *n_pairs);
+// ligand scoring (ag_score.hip)
+ag_scorer *scorer_create(ag_ctx *c, const ag_grid *grid, int n_maps, int elec_index, int dsolv_index, int n_poses,
+                         const int32_t *pose_start, const double *xyz, const int32_t *map_index,
+                         const double *charge, double outside);
+void scorer_set_positions(ag_scorer *s, const double *xyz);
+void scorer_score(ag_scorer *s, const float *d_maps, float *d_scores, float *d_grad);
+void scorer_adjoint(ag_scorer *s, const float *d_dscore, float *d_dmaps);
+void scorer_destroy(ag_scorer *s);
 }  // namespace agdev

@@ -468,6 +468,48 @@ ag_status ag_analytic_destroy(ag_analytic *a) {
     return guarded([&] { agdev::analytic_destroy(a); });
 }
 
+ag_status ag_scorer_create(ag_ctx *ctx, const ag_grid *grid, int n_maps, int elec_index, int dsolv_index, int n_poses,
+                           const int32_t *pose_start, const double *xyz, const int32_t *map_index,
+                           const double *charge, double outside, ag_scorer **out) {
+    if (!ctx || !grid || !pose_start || !out) {
+        ag::set_last_error("null argument to ag_scorer_create");
+        return AG_ERR_BAD_INPUT;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        *out = agdev::scorer_create(ctx, grid, n_maps, elec_index, dsolv_index, n_poses, pose_start, xyz, map_index,
+                                    charge, outside);
+    });
+}
+
+ag_status ag_scorer_set_positions(ag_scorer *s, const double *xyz) {
+    if (!s) {
+        ag::set_last_error("null scorer");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::scorer_set_positions(s, xyz); });
+}
+
+ag_status ag_scorer_score(ag_scorer *s, const float *maps_dev, float *scores_dev, float *grad_dev) {
+    if (!s || !maps_dev || !scores_dev) {
+        ag::set_last_error("null argument to ag_scorer_score");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::scorer_score(s, maps_dev, scores_dev, grad_dev); });
+}
+
+ag_status ag_scorer_adjoint(ag_scorer *s, const float *dscore_dev, float *dmaps_dev) {
+    if (!s || !dscore_dev || !dmaps_dev) {
+        ag::set_last_error("null argument to ag_scorer_adjoint");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::scorer_adjoint(s, dscore_dev, dmaps_dev); });
+}
+
+ag_status ag_scorer_destroy(ag_scorer *s) {
+    return guarded([&] { agdev::scorer_destroy(s); });
+}
+
 ag_status ag_job_weights(const ag_job *j, ag_weights *out) {
     if (!j || !out) {
         ag::set_last_error("null argument to ag_job_weights");

@@ -18,7 +18,7 @@ import torch  # noqa: F401  (first: one HIP runtime for torch and the engine)
 
 import numpy as np
 
-from . import Analytic, Context, PreparedState
+from . import Analytic, Context, PreparedState, Scorer
 from .poses import cotangent, make_poses, splat
 
 
@@ -49,26 +49,49 @@ class AnalyticMaps(torch.autograd.Function):
         return t(g["coord"]), t(g["charge"]), t(g["rij"]), t(g["eps"]), t(g["weights"]), None
 
 
-class PoseScorer:
-    """Scores of ligand poses against a map set: score = S @ maps.ravel(),
-    S the sparse trilinear-interpolation matrix [n_poses, n_maps * P]
-    (poses.splat)."""
+class PoseScores(torch.autograd.Function):
+    """scores = PoseScores.apply(maps, scorer): the poses' trilinear scores
+    from the engine's scoring kernel; the backward pass is the kernel's
+    adjoint S^T dL/dscores."""
 
-    def __init__(self, st: PreparedState, poses, device="cuda"):
-        r, c, v = splat(st, poses)
-        self.n_poses = len(poses)
+    @staticmethod
+    def forward(ctx, maps, scorer: Scorer):
+        maps = maps.contiguous()
+        scores = torch.empty(scorer.n_poses, dtype=torch.float32, device=maps.device)
+        torch.cuda.synchronize()
+        scorer.score(maps.data_ptr(), scores.data_ptr())
+        ctx.scorer = scorer
+        return scores
+
+    @staticmethod
+    def backward(ctx, dscores):
+        dscores = dscores.contiguous().to(torch.float32)
+        dmaps = torch.empty(ctx.scorer.shape, dtype=torch.float32, device=dscores.device)
+        torch.cuda.synchronize()
+        ctx.scorer.adjoint(dscores.data_ptr(), dmaps.data_ptr())
+        return dmaps, None
+
+
+class PoseScorer:
+    """Scores of ligand poses against a map set, on the GPU: each atom of
+    type t, charge q at x adds map_t(x) + q e(x) + |q| d(x), trilinearly
+    interpolated (ag_scorer_*)."""
+
+    def __init__(self, ctx: Context, st: PreparedState, poses):
+        self.scorer = Scorer(ctx, st, poses)
+        self.n_poses = self.scorer.n_poses
+        self.S_cpu = splat(st, poses)
         self.shape = (self.n_poses, st.n_maps * st.points)
-        idx = torch.from_numpy(np.stack([r, c]))
-        self.S = torch.sparse_coo_tensor(idx, torch.from_numpy(v).to(torch.float32), self.shape,
-                                         device=device).coalesce()
-        self.S_cpu = (r, c, v)
 
     def __call__(self, maps: torch.Tensor) -> torch.Tensor:
-        return torch.sparse.mm(self.S, maps.reshape(-1, 1)).reshape(-1)
+        return PoseScores.apply(maps, self.scorer)
 
     def cotangent(self, dscore: np.ndarray) -> np.ndarray:
-        """dL/dmaps (flat) for dL/dscore, on the host."""
+        """dL/dmaps (flat) for dL/dscore, on the host (FP64 reference)."""
         return cotangent(self.S_cpu, self.shape[1], dscore)
+
+    def close(self):
+        self.scorer.close()
 
 
 def fit_weights(ctx: Context, st: PreparedState, poses, scale_true, steps: int = 200, lr: float = 0.02):
@@ -87,7 +110,7 @@ def fit_weights(ctx: Context, st: PreparedState, poses, scale_true, steps: int =
     true weights and the time per step (forward + adjoint)."""
     import time
     eng = Analytic(ctx, st)
-    scorer = PoseScorer(st, poses)
+    scorer = PoseScorer(ctx, st, poses)
     coord = torch.from_numpy(np.ascontiguousarray(st.coord, np.float64))
     charge = torch.from_numpy(np.ascontiguousarray(st.charge, np.float64))
     rij = torch.from_numpy(eng.rij0.copy())
@@ -111,6 +134,7 @@ def fit_weights(ctx: Context, st: PreparedState, poses, scale_true, steps: int =
         hist.append(float(loss.detach().cpu()))
     t = (time.perf_counter() - t0) / steps
     w_fit = (w0 * torch.exp(theta.detach())).numpy()
+    scorer.close()
     eng.close()
     return {"loss": hist, "w_fit": w_fit, "w_true": w_true.numpy(), "w0": w0.numpy(), "s_per_step": t}
 
@@ -157,7 +181,7 @@ def main(argv=None):
     with Context(0) as ctx:
         if a.check_grad:
             eng = Analytic(ctx, st)
-            scorer = PoseScorer(st, poses)
+            scorer = PoseScorer(ctx, st, poses)
             coord = torch.from_numpy(np.ascontiguousarray(st.coord, np.float64)).requires_grad_()
             charge = torch.from_numpy(np.ascontiguousarray(st.charge, np.float64)).requires_grad_()
             rij = torch.from_numpy(eng.rij0.copy()).requires_grad_()
@@ -175,6 +199,14 @@ def main(argv=None):
                 den = max(float(np.max(np.abs(ref))), 1e-30)
                 rel = max(rel, float(np.max(np.abs(t.grad.numpy() - ref))) / den)
             out["grad_match_max_rel"] = rel
+            # the scoring kernel's scores against the FP64 host splat
+            with torch.no_grad():
+                s_dev = scorer(maps).double().cpu().numpy()
+            r_, c_, v_ = scorer.S_cpu
+            s_ref = np.zeros(scorer.n_poses)
+            np.add.at(s_ref, r_, v_ * maps.detach().reshape(-1).double().cpu().numpy()[c_])
+            out["score_max_rel"] = float(np.max(np.abs(s_dev - s_ref) / np.maximum(1.0, np.abs(s_ref))))
+            scorer.close()
             eng.close()
         t0 = time.perf_counter()
         res = fit_weights(ctx, st, poses, scale, steps=a.steps, lr=a.lr)

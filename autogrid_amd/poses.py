@@ -35,7 +35,9 @@ def make_poses(st, map_of_type: dict, n_poses: int = 256, seed: int = 1005,
 
 
 def splat(st, poses):
-    """(rows, cols, vals) of S: row = pose, col = map * P + point."""
+    """(rows, cols, vals) of S: row = pose, col = map * P + point.  A map
+    index of -1 (an atom without an affinity map, or no e / d map) adds no
+    entries.  Atoms must lie inside the grid."""
     P = st.points
     n1x, n1y = st.n1[0], st.n1[1]
     rows, cols, vals = [], [], []
@@ -51,11 +53,11 @@ def splat(st, poses):
                         (f[:, 2] if dz else 1 - f[:, 2])
                     pt = (i0[:, 0] + dx) + n1x * ((i0[:, 1] + dy) + n1y * (i0[:, 2] + dz))
                     for m, scale in ((mi, 1.0), (e, q), (d, np.abs(q))):
-                        if isinstance(m, (int, np.integer)) and m < 0:
-                            continue
-                        rows.append(np.full(pt.size, k))
-                        cols.append(np.asarray(m) * P + pt)
-                        vals.append(w * scale)
+                        m = np.broadcast_to(np.asarray(m, np.int64), pt.shape)
+                        keep = m >= 0          # -1: no affinity map / no e or d map
+                        rows.append(np.full(int(keep.sum()), k))
+                        cols.append(m[keep] * P + pt[keep])
+                        vals.append(np.broadcast_to(w * scale, pt.shape)[keep])
     return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
 
 

@@ -203,9 +203,23 @@ def analytic_leg(reps: int = 10, fit_steps: int = 200) -> dict:
         assert hip.hipMalloc(C.byref(dm), C.c_size_t(nbytes)) == 0
         assert hip.hipMalloc(C.byref(dg), C.c_size_t(nbytes)) == 0
         assert hip.hipMemcpy(dg, cot.ctypes.data_as(C.c_void_p), C.c_size_t(nbytes), 1) == 0
-        with ag.Context(0) as ctx, ag.Analytic(ctx, st) as an:
+        ds = C.c_void_p()
+        assert hip.hipMalloc(C.byref(ds), C.c_size_t(4 * len(poses))) == 0
+        ones = np.ones(len(poses), np.float32)
+        assert hip.hipMemcpy(ds, ones.ctypes.data_as(C.c_void_p), C.c_size_t(ones.nbytes), 1) == 0
+        with ag.Context(0) as ctx, ag.Analytic(ctx, st) as an, ag.Scorer(ctx, st, poses) as sc:
             an.forward(dm.value)
             an.adjoint(dg.value)
+            # the trilinear scoring kernel and its adjoint on the same maps
+            ts, ta = [], []
+            for _ in range(reps):
+                t1 = time.perf_counter()
+                sc.score(dm.value, ds.value)
+                t2 = time.perf_counter()
+                sc.adjoint(ds.value, dg.value)
+                ts.append(t2 - t1)
+                ta.append(time.perf_counter() - t2)
+            assert hip.hipMemcpy(dg, cot.ctypes.data_as(C.c_void_p), C.c_size_t(nbytes), 1) == 0
             fw, fa, fk, ak = [], [], [], []
             for _ in range(reps):
                 hip.hipDeviceSynchronize()
@@ -220,10 +234,12 @@ def analytic_leg(reps: int = 10, fit_steps: int = 200) -> dict:
                 fa.append(t2 - t0)
         hip.hipFree(dm)
         hip.hipFree(dg)
+        hip.hipFree(ds)
         out = {"workload": f"analytic: {st.n_atoms} atoms, {st.n1[0]}^3 points, {st.n_maps} maps, 256 poses",
                "forward_s": min(fw), "forward_adjoint_s": min(fa), "ratio": min(fa) / min(fw),
                "forward_kernels_ms": min(fk), "adjoint_kernels_ms": min(ak),
-               "kernel_ratio": (min(fk) + min(ak)) / min(fk)}
+               "kernel_ratio": (min(fk) + min(ak)) / min(fk),
+               "score_poses_ms": 1e3 * min(ts), "score_adjoint_ms": 1e3 * min(ta)}
         if fit_steps:
             p = subprocess.run([sys.executable, "-m", "autogrid_amd.fit", "--workload", "analytic", "--steps",
                                 str(fit_steps), "--poses", "256", "--workdir", work], cwd=ROOT,

@@ -429,6 +429,37 @@ ag_status ag_analytic_destroy(ag_analytic *a);
  * FE_coeff_* lines). */
 ag_status ag_job_weights(const ag_job *job, ag_weights *out);
 
+/* Ligand scoring, the consumer side of the maps.  AutoDock reads AutoGrid
+ * maps by trilinear interpolation: the maps are [n_maps][n1z][n1y][n1x]
+ * floats, x fastest (main.cpp:322), on the grid the .fld file describes
+ * (main.cpp:1903-1956), point (i, j, k) at ((i - ne_x) s, (j - ne_y) s,
+ * (k - ne_z) s) in the grid frame of ag_receptor's coordinates.  A ligand
+ * atom of map index t (its type's affinity map, -1 for none) and charge q
+ * at x scores
+ *     E(x) = sum_c w_c(x) (M_t[c] + q M_e[c] + |q| M_d[c])
+ * over the 8 points c of its cell, w_c the trilinear weights; an atom whose
+ * cell is not on the grid scores `outside` with zero gradient.  A pose
+ * scores the sum of its atoms.  Pose p holds atoms [pose_start[p],
+ * pose_start[p+1]) of xyz [n][3] (grid frame, A), map_index [n], charge [n]
+ * (host arrays).  The reference has no such consumer; this is the step
+ * downstream of its output. */
+typedef struct ag_scorer ag_scorer;
+ag_status ag_scorer_create(ag_ctx *ctx, const ag_grid *grid, int n_maps, int elec_index,
+                           int dsolv_index, int n_poses, const int32_t *pose_start,
+                           const double *xyz, const int32_t *map_index,
+                           const double *charge, double outside, ag_scorer **out);
+/* New positions for the same atoms. */
+ag_status ag_scorer_set_positions(ag_scorer *s, const double *xyz);
+/* scores_dev [n_poses] float (device); grad_dev [n][3] float (device,
+ * dE/dx of each atom) or NULL.  Returns when the results are written. */
+ag_status ag_scorer_score(ag_scorer *s, const float *maps_dev, float *scores_dev,
+                          float *grad_dev);
+/* dmaps_dev [n_maps][P] float (device) := S^T dscore_dev ([n_poses] float,
+ * device), S the scores' matrix (d scores / d maps): a fixed-order sum per
+ * map entry, bitwise repeatable. */
+ag_status ag_scorer_adjoint(ag_scorer *s, const float *dscore_dev, float *dmaps_dev);
+ag_status ag_scorer_destroy(ag_scorer *s);
+
 /* Map output: the value text of main.cpp:3486-3497 -- "0" when |e| < 1e-4,
  * else "%.3f" of (float)(floor(e*1000+0.5)/1000) -- for n values into buf.
  * Returns the number of bytes written (each value ends in '\n'), or -1 if

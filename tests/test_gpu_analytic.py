@@ -11,8 +11,9 @@
   kink of the model -- the 1e5 cap, a min/max switch, max(r, 0.5) -- lies
   within h) are reported and skipped, and must stay a small minority;
 * the adjoint is bitwise repeatable;
-* the torch.autograd.Function path returns the adjoint's gradients, and a
-  short Adam fit of the four weights reduces the loss.
+* the torch.autograd.Function path (analytic maps, then the trilinear
+  scoring kernel) returns the adjoint's gradients, and a short Adam fit of
+  the four weights reduces the loss.
 """
 from __future__ import annotations
 
@@ -214,5 +215,8 @@ def test_torch_function_and_short_fit(tmp_path):
     print(out.stdout[-2000:], out.stderr[-2000:], flush=True)
     assert out.returncode == 0
     res = json.loads(out.stdout.strip().splitlines()[-1])
-    assert res["grad_match_max_rel"] < 1e-6
+    # the autograd path's dL/dmaps comes from the scoring kernel's adjoint
+    # (FP32 weights), the direct call's from the FP64 host splat
+    assert res["grad_match_max_rel"] < 1e-5
+    assert res["score_max_rel"] < 1e-5
     assert res["loss_final"] < 1e-2 * res["loss_initial"]

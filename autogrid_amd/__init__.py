@@ -71,6 +71,7 @@ class PreparedState:
     floating_grid: int = 0
     constriction_grid: int = 0
     constriction_cutoff: float = 7.5
+    bhtree: int = 0              # ag_mapset.bhtree: 0 default build, 1 USE_BHTREE, 2 USE_BHTREE + interior
     # ag_setup fields (what the table / bond-vector builders read)
     pairs: np.ndarray | None = None
     r_smooth: float = 0.5
@@ -123,7 +124,8 @@ class PreparedState:
                                  float(m["solpar_probe"]), float(m["vol_probe"]))
         ms = _abi.MapSet(self.n_atom_maps, C.cast(md, C.POINTER(_abi.MapDesc)), self.elec_index, self.dsolv_index,
                          self.n_maps, self.separate_desolvation, self.disorder_h, self.hydrogen_type,
-                         self.dddiel, self.floating_grid, self.constriction_grid, 0, self.inv_diel_cal,
+                         self.dddiel, self.floating_grid, self.constriction_grid, int(self.bhtree),
+                         self.inv_diel_cal,
                          self.coeff_estat, self.solpar_q, (C.c_double * 3)(*self.cov_pos),
                          self.cov_half_width, self.cov_barrier, self.constriction_cutoff)
         tab = _abi.Tables(int(self.e_vdw_hb.shape[0]), 0, P(arrs["e_vdw_hb"], C.c_float),
@@ -171,9 +173,9 @@ class Job:
     """A GPF read by the engine's front end (ag_job_open)."""
 
     def __init__(self, gpf_path: str, glg_path: str | None = None, program: str = "autogrid4",
-                 files: bool = False):
+                 files: bool = False, bhtree: bool = False):
         h = C.c_void_p()
-        flags = 0 if files else _abi.AG_JOB_NO_FILES
+        flags = (0 if files else _abi.AG_JOB_NO_FILES) | (_abi.AG_JOB_BHTREE if bhtree else 0)
         check(lib().ag_job_open(os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
                                 program.encode(), flags, C.byref(h)))
         self._h = h
@@ -242,7 +244,7 @@ class Job:
             sol_fn=_arr(tab.sol_fn, _abi.AG_NDIEL, np.float64), r_eps=_arr(tab.r_eps, _abi.AG_NDIEL, np.float64),
             cov_pos=tuple(ms.cov_pos), cov_half_width=ms.cov_half_width, cov_barrier=ms.cov_barrier,
             floating_grid=ms.floating_grid, constriction_grid=ms.constriction_grid,
-            constriction_cutoff=ms.constriction_cutoff, pairs=pairs, r_smooth=setup.r_smooth,
+            constriction_cutoff=ms.constriction_cutoff, bhtree=ms.bhtree, pairs=pairs, r_smooth=setup.r_smooth,
             coeff_desolv=setup.coeff_desolv,
             t_types={k: getattr(setup, k) for k in ("t_hydrogen", "t_nonhb_hydrogen", "t_carbon", "t_arom_carbon",
                                                     "t_oxygen", "t_sulphur", "t_nitrogen", "t_nonhb_nitrogen")})
@@ -334,9 +336,10 @@ class Context:
         check(lib().ag_context_timing(self._h, ms))
         return {"cell_ms": ms[0], "near_ms": ms[1], "far_ms": ms[2], "total_ms": ms[3]}
 
-    def run_gpf(self, gpf_path: str, glg_path: str | None = None, program: str = "autogrid4") -> None:
-        check(lib().ag_run_gpf(self._h, os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
-                               program.encode()))
+    def run_gpf(self, gpf_path: str, glg_path: str | None = None, program: str = "autogrid4",
+                bhtree: bool = False) -> None:
+        check(lib().ag_run_gpf_flags(self._h, os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
+                                     program.encode(), _abi.AG_JOB_BHTREE if bhtree else 0))
 
 
 def _tune_struct(tune: dict | None):
@@ -426,10 +429,11 @@ class Plan:
         self.close()
 
 
-def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program: str = "autogrid4") -> None:
-    """autogrid4 -p gpf -l glg, on one GPU."""
+def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program: str = "autogrid4",
+            bhtree: bool = False) -> None:
+    """autogrid4 -p gpf -l glg, on one GPU (bhtree: a -DUSE_BHTREE build's semantics)."""
     with Context(device) as ctx:
-        ctx.run_gpf(gpf_path, glg_path, program)
+        ctx.run_gpf(gpf_path, glg_path, program, bhtree)
 
 
 class Scorer:

@@ -20,6 +20,7 @@ AG_OK = 0
 STATUS_NAMES = {0: "AG_OK", 1: "AG_ERR_BAD_INPUT", 2: "AG_ERR_NO_DONOR_H", 3: "AG_ERR_OOM",
                 4: "AG_ERR_DEVICE", 5: "AG_ERR_IO", 6: "AG_ERR_UNSUPPORTED"}
 AG_JOB_NO_FILES = 1
+AG_JOB_BHTREE = 2
 AG_NEINT = 1024
 AG_POTENTIAL_AD4 = 0
 AG_POTENTIAL_VINA = 1
@@ -57,7 +58,7 @@ class MapSet(C.Structure):
                 ("elec_index", C.c_int32), ("dsolv_index", C.c_int32), ("n_maps", C.c_int32),
                 ("separate_desolvation", C.c_int32), ("disorder_h", C.c_int32),
                 ("hydrogen_type", C.c_int32), ("dddiel", C.c_int32), ("floating_grid", C.c_int32),
-                ("constriction_grid", C.c_int32), ("reserved", C.c_int32),
+                ("constriction_grid", C.c_int32), ("bhtree", C.c_int32),
                 ("inv_diel_cal", C.c_double), ("coeff_estat", C.c_double), ("solpar_q", C.c_double),
                 ("cov_pos", C.c_double * 3), ("cov_half_width", C.c_double),
                 ("cov_barrier", C.c_double), ("constriction_cutoff", C.c_double)]
@@ -121,6 +122,8 @@ def lib() -> C.CDLL:
         L.ag_job_close.argtypes = [vp]
         L.ag_job_close.restype = None
         L.ag_run_gpf.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p]
+        L.ag_run_gpf_flags.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32]
+        L.ag_run_gpf_flags.restype = C.c_int
         L.ag_maps_plain.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
                                     C.POINTER(Tables), vp, vp, vp, C.c_int]
         L.ag_format_map_values.argtypes = [vp, C.c_int64, C.c_char_p, C.c_int64]

@@ -855,6 +855,7 @@ static int an_case(int mh, int ah) {
 ag_analytic *analytic_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                              const ag_setup *setup, const double w0[4]) {
     if (ms->disorder_h) ag::fail(AG_ERR_UNSUPPORTED, "disorder_h is not part of the analytic model");
+    if (ms->bhtree) ag::fail(AG_ERR_UNSUPPORTED, "the tree build's semantics are not part of the analytic model");
     if (setup->n_atom_maps != ms->n_atom_maps) ag::fail(AG_ERR_BAD_INPUT, "setup and map set disagree");
     if (rec->n_atoms < 0 || rec->n_atoms > AG_MAX_ATOMS) ag::fail(AG_ERR_BAD_INPUT, "atom count out of range");
     for (int d = 0; d < 3; d++)

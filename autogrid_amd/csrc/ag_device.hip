@@ -160,6 +160,8 @@ struct NearParams {
     int disorder_h, hydrogen_type;
     int *cmap;                    // constriction count per point, or null (pass 0 only)
     double ccut;                  // constriction distance cutoff (<= 8 A)
+    int bhtree;                   // ag_mapset.bhtree: Loop 2 and cmap by the tree's FP32 test
+    unsigned char *collide;       // tree build, collisions tested: 1 per point within 1.5 A of an atom (pass 0)
     float *maps;
     int *status;
 };
@@ -462,6 +464,11 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     const float pyE = (float)(iy - ney) * s128;
     const float pzE = (float)(iz - nez) * s128;
     const float biasE = far_bias(s128);
+    // the tree build's query point: the grid point as float (main.cpp:2860)
+    const int bh = p.bhtree;
+    const float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz;
+    const float ccf = (float)p.ccut, ccf2 = ccf * ccf;
+    bool collide = false;
 
     // ---- stage candidate atoms: cells overlapping the brick box + radius ----
     int c0[3], c1[3];
@@ -615,8 +622,22 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 bin = (int)g;
                 cut = r > 8.0f;
             }
+            // the tree build's radius queries (bhtree.cpp:341-362): FP32 point
+            // and atom, each component within the radius, then the rounded
+            // squared distance against the rounded squared radius
+            bool in8 = false;
+            float rB = 0.f;
+            if (bh) {
+                const float e0 = fcx - (float)a.x, e1 = fcy - (float)a.y, e2 = fcz - (float)a.z;
+                const float a0 = fabsf(e0), a1 = fabsf(e1), a2 = fabsf(e2);
+                const float D = (e0 * e0 + e1 * e1) + e2 * e2;
+                in8 = a0 <= 8.f && a1 <= 8.f && a2 <= 8.f && D <= 64.f;
+                if (a0 <= 1.5f && a1 <= 1.5f && a2 <= 1.5f && D <= 2.25f) collide = true;
+                if (cmap) ccount += a0 <= ccf && a1 <= ccf && a2 <= ccf && D <= ccf2;
+                rB = sqrtf(D);   // correctly rounded, as the tree's sqrt of a float
+            }
             // constriction grid: atoms within the cutoff (main.cpp:2969-2973)
-            if (cmap) {
+            if (cmap && !bh) {
                 if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
                     double rdx, rdy, rdz;
                     ccount += ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz) <= p.ccut;
@@ -636,6 +657,13 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                     e_near += unif(a.qe) * inv_rmax * reps_s[min(bin, kNearRepsLen - 1)];
                 }
             }
+            // Loop 2 of the tree build takes the atoms its 8 A query returns,
+            // at their FP32 distance (main.cpp:3019-3028)
+            if (bh) {
+                cut = !in8;
+                r = rB;
+                bin = (int)((double)rB * 128.0);
+            }
             if (cut || !p.has_affinity) continue;
             const int meta = uni(a.meta);
             const int type = meta & 0xff, hb = (meta >> 8) & 0xff, rexp = (meta >> 16) & 0xff;
@@ -653,6 +681,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 if (fabsf(cos_t) < 1.0e-5f) {
                     double rdx, rdy, rdz;
                     double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                    if (bh) r64 = (double)rB;   // the tree build's r (main.cpp:3026)
                     if (r64 < 1.0e-6) r64 = 1.0e-6;
                     const double ir = 1. / r64;
                     const double *rv64 = p.rvec64 + 3 * (size_t)aidx;
@@ -766,7 +795,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 float Eh[kHbSlots] = {0.f, 0.f, 0.f, 0.f};
                 if (dis && p.hydrogen_type >= 0) {
                     double rdx, rdy, rdz;
-                    const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                    const double r64 = bh ? (double)rB : ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
                     const int hb_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
                     const float4 v = reinterpret_cast<const float4 *>(
                         tab + ((size_t)p.hydrogen_type * AG_NEINT + hb_bin) * kMapSlots)[kPlainSlots / 4];
@@ -809,9 +838,20 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
         }
         if (want_e) maps[(long long)p.e_slot * P + pt] = e_near;
         if (cmap) cmap[pt] = ccount;
+        if (p.collide) p.collide[cf.pt_off + pt] = collide ? 1 : 0;
         if (p.d_slot >= 0) maps[(long long)p.d_slot * P + pt] = d_acc;
     }
     }  // candidate lists within capacity
+}
+
+// The tree build's collision rule (main.cpp:2906-2927): a point within
+// 1.5 A of a receptor atom has e = 0 and floating value 1.5 (Loop 1 is
+// skipped there).  Runs after the near, far and floating-grid kernels.
+__global__ void k_collide(long long P, const unsigned char *mask, float *emap, float *fmap) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P || !mask[i]) return;
+    if (emap) emap[i] = 0.f;
+    if (fmap) fmap[i] = 1.5f;
 }
 
 __global__ void k_fill(float *out, long long n, float v) {
@@ -1135,6 +1175,9 @@ struct ag_plan {
     DevBuf atom_cell{pool}, cell_count{pool}, cell_start{pool}, cell_cursor{pool}, slot_atom{pool},
         near_atoms{pool}, far_atoms{pool}, fmin_atoms{pool};
     DevBuf tab{pool}, sol{pool}, reps{pool}, fartab{pool}, status{pool};
+    DevBuf collide{pool};              // tree build: collision mask per point
+    bool want_collide = false;
+    int e_index = -1;                  // e map slot, -1 if none
     AtomSrc src{};
     std::vector<NearParams> near;      // one per pass of 16 map slots
     int nplain = 12;                   // plain slots in use (4, 8 or 12)
@@ -1229,6 +1272,8 @@ static void validate(const ag_receptor *rec, const ag_grid *grid, const ag_mapse
     }
     if (vina && ms->separate_desolvation)
         ag::fail(AG_ERR_UNSUPPORTED, "Vina-potential maps do not combine with separate_desolvation_maps");
+    if (ms->bhtree < 0 || ms->bhtree > 2) ag::fail(AG_ERR_BAD_INPUT, "bhtree mode must be 0, 1 or 2");
+    if (vina && ms->bhtree) ag::fail(AG_ERR_UNSUPPORTED, "Vina-potential maps have no tree-build mode");
     if (M > 0 && !vina) {
         bool any_h = false;
         for (int i = 0; i < N && !any_h; i++) any_h = rec->hbond[i] == AG_HB_DS || rec->hbond[i] == AG_HB_D1;
@@ -1343,6 +1388,9 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms) {
     p.total_tiles = (int)tile;
     p.total_cells = cell_base;
     p.P_total = poff;
+    if (p.want_collide) {
+        p.near[0].collide = (unsigned char *)p.collide.get((size_t)p.P_total);
+    }
     hipStream_t st = p.ctx->stream;
     upload(st, p.confs_d, p.confs.data(), p.confs.size());
     upload(st, p.brick_prefix_d, p.brick_prefix.data(), p.brick_prefix.size());
@@ -1556,6 +1604,10 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     p.want_fmap = ms->floating_grid != 0;
     p.want_cmap = ms->constriction_grid != 0;
     if (p.want_cmap) p.near[0].ccut = ms->constriction_cutoff;
+    // the tree build's semantics (every pass: Loop 2; pass 0: cmap, collisions)
+    for (auto &q : p.near) q.bhtree = ms->bhtree;
+    p.want_collide = ms->bhtree == 1 && (ms->elec_index >= 0 || ms->floating_grid);
+    p.e_index = vina ? -1 : ms->elec_index;
     if (p.want_fmap && N > 0) {
         float4 *d_fa = (float4 *)p.fmin_atoms.get(sizeof(float4) * (size_t)N);
         k_fmin_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_fa);
@@ -1749,6 +1801,15 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
             const int blocks = p->persistent ? std::min(p->far_grid, t1 - t0) : t1 - t0;
             launch_far(p->far_clamp, fp.tab_len, blocks, fp, st);
         }
+        if (p->want_collide)
+            for (int ci = c0; ci < c1; ci++) {
+                const ConfDev &cf = p->confs[ci];
+                k_collide<<<(unsigned)((cf.P + 255) / 256), 256, 0, st>>>(
+                    cf.P, (const unsigned char *)p->collide.ptr + cf.pt_off,
+                    p->e_index >= 0 ? d_maps + cf.map_off + (size_t)p->e_index * cf.P : nullptr,
+                    d_fmap ? d_fmap + cf.pt_off : nullptr);
+            }
+        HIP_CHECK(hipGetLastError());
         if (host) {
             HIP_CHECK(hipEventRecord(c->chunk_ev[k], st));
             HIP_CHECK(hipStreamWaitEvent(c->copy_stream, c->chunk_ev[k], 0));

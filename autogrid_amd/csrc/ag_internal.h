@@ -76,6 +76,8 @@ struct Job {
     FILE *log = nullptr;
     bool own_log = false;
     bool make_files = true;
+    bool bhtree = false;                // semantics of a -DUSE_BHTREE build
+    bool map_receptor_interior = false; // main.cpp:580
     FILE *fld_fp = nullptr, *xyz_fp = nullptr, *fmap_fp = nullptr, *cmap_fp = nullptr;
     bool xyz_open = false;       // gridfld seen and .xyz not yet written
     int outlev = 1;

@@ -56,6 +56,7 @@ void job_views(const Job &job, ag_receptor *rec, ag_grid *grid, ag_mapset *ms, a
         ms->dddiel = job.dddiel;
         ms->floating_grid = job.floating_grid;
         ms->constriction_grid = job.constriction_grid;
+        ms->bhtree = job.bhtree ? (job.map_receptor_interior ? 2 : 1) : 0;
         ms->inv_diel_cal = job.inv_diel_cal;
         ms->coeff_estat = job.lib.coeff_estat;
         ms->solpar_q = job.solpar_q;
@@ -155,6 +156,7 @@ ag_status ag_job_open(const char *gpf_path, const char *glg_path, const char *pr
     }
     ag::Job &job = j->job;
     job.make_files = !(flags & AG_JOB_NO_FILES);
+    job.bhtree = (flags & AG_JOB_BHTREE) != 0;
     if (program_name) job.program = program_name;
     ag_status st = guarded([&] {
         if (glg_path) {
@@ -585,8 +587,13 @@ ag_status ag_job_write_maps(ag_job *j, const float *maps, const float *fmap, con
 }
 
 ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path, const char *program_name) {
+    return ag_run_gpf_flags(ctx, gpf_path, glg_path, program_name, 0);
+}
+
+ag_status ag_run_gpf_flags(ag_ctx *ctx, const char *gpf_path, const char *glg_path, const char *program_name,
+                           uint32_t flags) {
     ag_job *j = nullptr;
-    ag_status st = ag_job_open(gpf_path, glg_path, program_name, 0, &j);
+    ag_status st = ag_job_open(gpf_path, glg_path, program_name, flags & AG_JOB_BHTREE, &j);
     if (st != AG_OK) return st;
     ag::Job &job = j->job;
     st = guarded([&] {

@@ -14,17 +14,23 @@ static const char *kHelp =
     "\t\t-l log_filename\n"
     "\t\t-d (increment debug level)\n"
     "\t\t-h (display this message)\n"
-    "\t\t--version (print version information, copyright, and license)\n";
+    "\t\t--version (print version information, copyright, and license)\n"
+    "\t\t--bhtree (the semantics of a -DUSE_BHTREE build)\n";
 
 int main(int argc, char **argv) {
     const char *prog = argv[0];
     const char *gpf = nullptr, *glg = nullptr;
+    uint32_t flags = 0;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         if (a[0] != '-') break;
         if (a[1] == '-') a++;
         switch (a[1]) {
             case 'd': break;
+            case 'b':
+                if (strcmp(a, "-bhtree") != 0) { fprintf(stderr, "%s: unknown switch %s\n", prog, argv[i]); return 1; }
+                flags |= AG_JOB_BHTREE;
+                break;
             case 'u':
             case 'h': printf("usage: AutoGrid %s\n", kHelp); return 0;
             case 'v':
@@ -48,6 +54,6 @@ int main(int argc, char **argv) {
     }
     // The library opens device 0 only after the GPF has been read and
     // checked, so input errors are reported the same way with or without a GPU.
-    ag_status st = ag_run_gpf(nullptr, gpf ? gpf : "", glg, prog);
+    ag_status st = ag_run_gpf_flags(nullptr, gpf ? gpf : "", glg, prog, flags);
     return st == AG_OK ? 0 : 1;
 }

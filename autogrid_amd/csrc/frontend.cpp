@@ -743,11 +743,19 @@ void job_parse_gpf(Job &job, const std::string &gpf_path) {
                 break;
             }
 
-            case GPF_MAP_RECEPTOR_INTERIOR: {
+            case GPF_MAP_RECEPTOR_INTERIOR: {   // main.cpp:1643-1658
                 char tok[64] = "true";
                 sscanf(line, "%*s %63s", tok);
-                if (!(strcasecmp(tok, "true") == 0 || strcasecmp(tok, "on") == 0))
+                if (strcasecmp(tok, "true") == 0 || strcasecmp(tok, "on") == 0) {
+                    job.map_receptor_interior = true;
+                    job.logf("Mapping receptor interior instead of reporting collision if nearer than %.2f to "
+                             "any atom.", 1.5);
+                } else if (job.bhtree) {
+                    job.map_receptor_interior = false;
+                    job.logf("Reporting collision if nearer than %.2f to any atom.", 1.5);
+                } else {
                     fail(AG_ERR_BAD_INPUT, "'map_receptor_interior off'  is supported only with USE_BHTREE.");
+                }
                 break;
             }
 

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define AG_ABI_VERSION 2
+#define AG_ABI_VERSION 3
 
 /* Table sizes of the 8 A build (constants.h:148-171, USE_8A_NBCUTOFF). */
 #define AG_A_DIV 128           /* bins per Angstrom                         */
@@ -110,7 +110,13 @@ typedef struct ag_mapset {
     int32_t dddiel;           /* 1: distance-dependent dielectric          */
     int32_t floating_grid;    /* 1: also produce r_min (fmap)              */
     int32_t constriction_grid;/* 1: also produce the atom count (cmap)     */
-    int32_t reserved;
+    int32_t bhtree;           /* semantics of a -DUSE_BHTREE build (main.cpp:2893-2928, 3019-3042):
+                                 0: the default build's (no tree);
+                                 1: Loop 2 and the cmap count select atoms by the tree's FP32
+                                    distance (bhtree.cpp:341-362) and take their bins from it;
+                                    a point within 1.5 A of an atom (BH_collision_dist,
+                                    main.cpp:346) gets e = 0 and fmap = 1.5;
+                                 2: as 1 with map_receptor_interior on: no collision test */
     double  inv_diel_cal;     /* 332/diel for a constant dielectric        */
     double  coeff_estat;
     double  solpar_q;         /* qasp, charge term of desolvation          */
@@ -173,6 +179,10 @@ typedef struct ag_job ag_job;
 
 /* Flags of ag_job_open. */
 #define AG_JOB_NO_FILES 1u    /* parse only: open no map/.fld/.xyz files   */
+#define AG_JOB_BHTREE   2u    /* the semantics of a -DUSE_BHTREE build: the
+                                 ag_mapset bhtree mode, map_receptor_interior
+                                 off accepted (main.cpp:1643-1658), and the
+                                 log's USE_BHTREE line reading yes          */
 
 /* Read a GPF with the keyword semantics of main.cpp:778-1819 (order
  * dependence included), read the receptor PDBQT (main.cpp:823-1047) and any
@@ -197,6 +207,9 @@ void ag_job_close(ag_job *job);
  * empty gpf_path reads the GPF from standard input (setflags.cpp:78). */
 ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path,
                      const char *program_name);
+/* ag_run_gpf with ag_job_open flags (AG_JOB_BHTREE). */
+ag_status ag_run_gpf_flags(ag_ctx *ctx, const char *gpf_path, const char *glg_path,
+                           const char *program_name, uint32_t flags);
 
 /* ------------------------------------------------------------------------ */
 /* Hot path.                                                                 */

@@ -27,6 +27,7 @@
 #define NDIEL 8192          /* constants.h:171 */
 #define A_DIV 128           /* constants.h:151 */
 #define SOFTNBC 8.0         /* constants.h:150 */
+#define NBC 8.0             /* constants.h:148 (USE_8A_NBCUTOFF); the tree's Loop-2 radius, main.cpp:352 */
 #define MAXNB 4             /* AG_MAX_NBONDS, autogrid.h:59 */
 
 static double dmin(double a, double b) { return a < b ? a : b; }
@@ -257,6 +258,38 @@ done:
 
 /* --- per-point loop, main.cpp:2805-3431 --------------------------------- */
 
+/* Radius query of a -DUSE_BHTREE build (bhtree.cpp:320-379): the atoms
+ * within `cutoff` of x by the tree's FP32 test -- each component of
+ * x - atom within +-cutoff, then d0*d0 + d1*d1 + d2*d2 <= cutoff*cutoff,
+ * every operation rounded to float -- with the FP32 distance sqrt(D).  The
+ * tree prunes by the same bounds, so this visits the atoms it returns (in
+ * index order; the tree's order only permutes an FP64 sum).  Stores up to
+ * `cap` (index, distance) pairs, returns the count. */
+static int bh_query(const ag_receptor *rec, const float x[3], float cutoff, int *idx, float *dist, int cap) {
+    const float C = cutoff * cutoff;
+    int n = 0;
+    for (int ia = 0; ia < rec->n_atoms; ia++) {
+        const double *a = rec->coord + 3 * (size_t)ia;
+        float d[3];
+        int out = 0;
+        for (int i = 0; i < 3 && !out; i++) {
+            d[i] = x[i] - (float)a[i];
+            out = d[i] > cutoff || d[i] < -cutoff;
+        }
+        if (out) continue;
+        const float D0 = d[0] * d[0];
+        const float D1 = D0 + d[1] * d[1];
+        const float D = D1 + d[2] * d[2];
+        if (D > C) continue;
+        if (n < cap) {
+            if (idx) idx[n] = ia;
+            if (dist) dist[n] = sqrtf(D);
+        }
+        n++;
+    }
+    return n;
+}
+
 int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,
              double *maps, double *fmap, int32_t *cmap, int z0, int z1) {
     const int N = rec->n_atoms, M = ms->n_atom_maps;
@@ -280,6 +313,8 @@ int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, c
 #pragma omp parallel for schedule(dynamic, 1)
     for (int iz = z0; iz < z1; iz++) {
         double c[3];
+        int *bh_idx = ms->bhtree ? malloc(sizeof(int) * ((size_t)N + 1)) : NULL;
+        float *bh_dist = ms->bhtree ? malloc(sizeof(float) * ((size_t)N + 1)) : NULL;
         double hbondmin[AG_MAX_MAPS], hbondmax[AG_MAX_MAPS];
         int hbondflag[AG_MAX_MAPS];
         c[2] = ((double)(iz - grid->ne[2])) * spacing;
@@ -309,8 +344,22 @@ int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, c
                     hbondmax[m] = -999999.;
                     hbondflag[m] = 0;
                 }
+                /* -DUSE_BHTREE build (main.cpp:2893-2928): the constriction count
+                 * by the tree, and unless map_receptor_interior, a collision
+                 * within 1.5 A sets e = 0 and fmap = 1.5 and skips Loop 1 */
+                const int bh = ms->bhtree;
+                int is_collision = 0;
+                if (bh) {
+                    const float fcc[3] = {(float)c[0], (float)c[1], (float)c[2]};
+                    if (cmap) cmap[mapi] = bh_query(rec, fcc, (float)ms->constriction_cutoff, NULL, NULL, 0);
+                    if (bh == 1 && (elecPE >= 0 || fmap) && bh_query(rec, fcc, 1.5f, NULL, NULL, 0) > 0) {
+                        is_collision = 1;
+                        if (elecPE >= 0) maps[(size_t)elecPE * P + mapi] = 0.;
+                        if (fmap) fmap[mapi] = 1.5;
+                    }
+                }
                 /* Loop 1 (main.cpp:2940-2982): all atoms, hypot distance */
-                if (elecPE >= 0 || fmap || cmap) {
+                if (!is_collision && (elecPE >= 0 || fmap || (cmap && !bh))) {
                     double e = 0;
                     for (int ia = 0; ia < N; ia++) {
                         const double *a = rec->coord + 3 * (size_t)ia;
@@ -320,7 +369,7 @@ int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, c
                         int indx_r = (int)(r * A_DIV);
                         if (indx_r > NDIEL - 1) indx_r = NDIEL - 1;
                         if (fmap) fmap[mapi] = dmin(r, fmap[mapi]);
-                        if (cmap && r <= ms->constriction_cutoff) cmap[mapi]++;
+                        if (cmap && !bh && r <= ms->constriction_cutoff) cmap[mapi]++;
                         if (elecPE >= 0) {
                             if (ms->dddiel) e += rec->charge[ia] * inv_rmax * tab->r_eps[indx_r] * ms->coeff_estat;
                             else e += rec->charge[ia] * inv_rmax * ms->inv_diel_cal * ms->coeff_estat;
@@ -348,8 +397,15 @@ int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, c
                         }
                     }
                 if (!(M > 0 || dsolvPE >= 0)) continue;
-                /* Loop 2 (main.cpp:3017-3404) */
-                for (int ia = 0; ia < N; ia++) {
+                /* Loop 2 (main.cpp:3017-3404); with the tree, over the atoms
+                 * its 8 A query returns, at their FP32 distance (main.cpp:3019-3028) */
+                int nbh = N;
+                if (bh) {
+                    const float fcc[3] = {(float)c[0], (float)c[1], (float)c[2]};
+                    nbh = bh_query(rec, fcc, (float)NBC, bh_idx, bh_dist, N);
+                }
+                for (int ib = 0; ib < nbh; ib++) {
+                    const int ia = bh ? bh_idx[ib] : ib;
                     const double *a = rec->coord + 3 * (size_t)ia;
                     const double *rv = rec->rvector + 3 * (size_t)ia;
                     const double *rv2 = rec->rvector2 + 3 * (size_t)ia;
@@ -357,7 +413,7 @@ int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, c
                     const int dis = rec->disorder[ia] != 0;
                     double dnorm[3], racc, rdon, Hramp;
                     for (int i = 0; i < 3; i++) d[i] = a[i] - c[i];
-                    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    double r = bh ? (double)bh_dist[ib] : sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
                     if (r > SOFTNBC) continue;
                     if (ms->disorder_h && dis && at == hydrogen) continue;
                     if (r < APPROX_ZERO) r = APPROX_ZERO;
@@ -499,6 +555,8 @@ int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, c
                     if (hbondflag[m]) maps[(size_t)m * P + mapi] += (hbondmin[m] + dmax(hbondmax[m], 0));
             }
         }
+        free(bh_idx);
+        free(bh_dist);
     }
 #undef EV
     free(h12);
@@ -578,6 +636,7 @@ int orc_vina_tables(const ag_setup *s, const ag_mapset *ms, const double *xs_map
  * zero. */
 int orc_maps_vina(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms, const ag_tables *tab,
                   double *maps, double *fmap, int32_t *cmap, int z0, int z1) {
+    if (ms->bhtree) return -3;   /* the tree build's Vina branch is dead code as well */
     const int N = rec->n_atoms, M = ms->n_atom_maps;
     const int n1x = grid->n1[0], n1y = grid->n1[1];
     const long long P = (long long)grid->n1[0] * grid->n1[1] * grid->n1[2];

@@ -62,6 +62,18 @@ SYN_CASES = {
 
 ALL_CASES = list(REF_CASES) + list(SYN_CASES)
 
+# Cases run with the semantics of a -DUSE_BHTREE build (main.cpp:2893-2928,
+# 3019-3042): name -> (base case, lines appended to its GPF).
+BH_CASES = {
+    "small": ("small", []),
+    "fmap_cmap": ("fmap_cmap", []),
+    "ongrid": ("ongrid", []),
+    "disorder": ("disorder", []),
+    "hsg1_sm": ("hsg1_sm", []),
+    "interior": ("small", ["map_receptor_interior on"]),
+    "interior_off": ("fmap_cmap", ["map_receptor_interior off"]),
+}
+
 
 def case_dir(name: str) -> str:
     return REFTESTS if name in REF_CASES else os.path.join(CASES_DIR, name)
@@ -101,6 +113,17 @@ def materialize(name: str, dest: str) -> str:
         if f.endswith((".gpf", ".pdbqt", ".dat")):
             shutil.copy(os.path.join(src, f), dest)
     return gpf_name(name)
+
+
+def materialize_bh(name: str, dest: str) -> str:
+    """Inputs of a tree-build case in dest; returns the GPF file name."""
+    base, extra = BH_CASES[name]
+    gpf = materialize(base, dest)
+    if extra:
+        with open(os.path.join(dest, gpf), "a") as f:
+            for line in extra:
+                f.write(line + "\n")
+    return gpf
 
 
 def write_synthetic_inputs(name: str, dest: str) -> None:

@@ -10,6 +10,10 @@
        header lines) and the number of values;
      - the .xyz text and the .fld text without its "Created by" line.
 
+3. When oracle/_ref/autogrid4_bhtree is built (make -C oracle ref-bhtree),
+   runs it on the tree-build cases (cases.BH_CASES) and records the same
+   under "bhtree".
+
 Usage:  python tests/golden/make_golden.py [--inputs-only]
 """
 from __future__ import annotations
@@ -29,6 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import cases  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "autogrid4")
+REF_BH = os.path.join(ROOT, "oracle", "_ref", "autogrid4_bhtree")
 
 
 def value_hash(path: str):
@@ -67,6 +72,26 @@ def main():
                 rec["fld"] = fld
             out[name] = rec
             print(name, rec["rc"], rec["success"], len(rec.get("maps", {})))
+    # the tree build (make -C oracle ref-bhtree) on the cases.BH_CASES variants
+    if os.path.exists(REF_BH):
+        bh = {}
+        for name in cases.BH_CASES:
+            with tempfile.TemporaryDirectory() as d:
+                gpf = cases.materialize_bh(name, d)
+                p = subprocess.run([REF_BH, "-p", gpf, "-l", "ref.glg"], cwd=d, capture_output=True)
+                glg = open(os.path.join(d, "ref.glg"), errors="replace").read() if os.path.exists(
+                    os.path.join(d, "ref.glg")) else ""
+                rec = {"rc": p.returncode, "success": "Successful Completion" in glg[-400:],
+                       "tree_line": [ln.strip() for ln in glg.split("\n") if "USE_BHTREE" in ln]}
+                if rec["success"]:
+                    rec["maps"] = {m: dict(zip(("sha256", "n"), value_hash(os.path.join(d, m))))
+                                   for m in cases.map_names(os.path.join(d, gpf))}
+                    for kind, fn in cases.distance_map_names(os.path.join(d, gpf)).items():
+                        h, n = value_hash(os.path.join(d, fn))
+                        rec[kind] = {"file": fn, "sha256": h, "n": n}
+                bh[name] = rec
+                print("bhtree", name, rec["rc"], rec["success"], rec["tree_line"])
+        out["bhtree"] = bh
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 

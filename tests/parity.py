@@ -23,6 +23,19 @@ def within(gpu: np.ndarray, ref: np.ndarray):
     return bad == 0, float(err.max(initial=0.0)), float(np.sqrt(np.mean(err ** 2))) if err.size else 0.0, bad
 
 
+def load_case_bh(name: str, workdir: str):
+    """load_case for a tree-build case (cases.BH_CASES): the front end with
+    the semantics of a -DUSE_BHTREE build."""
+    gpf = cases.materialize_bh(name, workdir)
+    cwd = os.getcwd()
+    os.chdir(workdir)
+    try:
+        st = ag.Job(gpf, bhtree=True).prepare().state()
+    finally:
+        os.chdir(cwd)
+    return st, gpf
+
+
 def load_case(name: str, workdir: str):
     """(PreparedState from the engine's front end, oracle FP64 maps)."""
     gpf = cases.materialize(name, workdir)

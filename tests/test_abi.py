@@ -30,7 +30,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_abi_version():
-    assert _abi.lib().ag_abi_version() == 2
+    assert _abi.lib().ag_abi_version() == 3
 
 
 def test_context_on_a_host_without_gpu_fails_cleanly():

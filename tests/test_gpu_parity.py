@@ -56,6 +56,37 @@ def test_maps_match_oracle(ctx, name, tmp_path):
     assert not failed, f"{name}: maps over tolerance {failed}"
 
 
+BH_SUCCESS = [n for n in cases.BH_CASES if GOLDEN.get("bhtree", {}).get(n, {}).get("success")]
+
+
+@pytest.mark.parametrize("name", BH_SUCCESS)
+def test_tree_build_maps_match_oracle(ctx, name, tmp_path):
+    """The -DUSE_BHTREE semantics (FP32 radius queries for Loop 2 and the
+    constriction count, the 1.5 A collision rule) against the oracle, which
+    reproduces the reference's tree build byte for byte."""
+    print(f"[gpu] tree-build case {name}", flush=True)
+    st, _ = parity.load_case_bh(name, str(tmp_path))
+    assert st.bhtree in (1, 2)
+    want_f, want_c = bool(st.floating_grid), bool(st.constriction_grid)
+    ref, rfm, rcm = ol.maps(ol.oracle_state(st), fmap=True, cmap=True)
+    got = ctx.maps_plain(st)
+    if want_f:
+        assert np.abs(ctx.last_fmap.astype(np.float64) - rfm).max() <= 1e-4
+    if want_c:
+        assert np.array_equal(ctx.last_cmap, rcm)
+    if st.elec_index >= 0:
+        # collision points hold exactly 0 on both sides
+        zero = rfm == 1.5 if st.bhtree == 1 else np.zeros_like(rfm, bool)
+        assert np.all(got[st.elec_index][zero] == 0.0)
+    report = []
+    for k in range(st.n_maps):
+        ok, mx, rms, bad = parity.within(got[k], ref[k])
+        report.append((k, ok, mx, rms, bad))
+    print(name, [(k, f"{mx:.2e}", f"{rms:.2e}", bad) for k, ok, mx, rms, bad in report], flush=True)
+    failed = [r for r in report if not r[1]]
+    assert not failed, f"{name}: maps over tolerance {failed}"
+
+
 def test_repeat_is_bitwise_identical(ctx, tmp_path):
     st, _ = parity.load_case("small", str(tmp_path))
     a = ctx.maps_plain(st)
@@ -124,3 +155,30 @@ def test_cli_dropin(name, tmp_path):
     if "fmap" in g:
         vals = np.array([float(x) for x in open(tmp_path / g["fmap"]["file"]).read().split("\n")[6:] if x])
         assert vals.size == g["fmap"]["n"]
+
+
+@pytest.mark.parametrize("name", ["fmap_cmap", "interior_off"])
+def test_cli_dropin_tree_build(name, tmp_path):
+    """autogrid4 --bhtree: the tree build's outputs.  The constriction counts
+    are byte-identical to the reference tree build's text; the maps hold the
+    oracle's values within the text's rounding."""
+    import hashlib
+    print(f"[gpu] cli --bhtree {name}", flush=True)
+    gpf = cases.materialize_bh(name, str(tmp_path))
+    p = subprocess.run([ag.CLI_PATH, "--bhtree", "-p", gpf, "-l", "out.glg"], cwd=tmp_path, capture_output=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr.decode()
+    glg = open(tmp_path / "out.glg").read()
+    assert "Successful Completion" in "".join(glg.splitlines(True)[-10:])
+    st, _ = parity.load_case_bh(name, str(tmp_path / "oracle"))
+    ref = ol.maps(ol.oracle_state(st))
+    g = GOLDEN["bhtree"][name]
+    for k, m in enumerate(cases.map_names(os.path.join(tmp_path, gpf))):
+        vals = np.array([float(x) for x in open(tmp_path / m).read().split("\n")[6:] if x])
+        assert vals.size == g["maps"][m]["n"]
+        err = np.abs(vals - ref[k].reshape(-1))
+        lim = np.maximum(2e-3, 1e-3 * np.abs(ref[k].reshape(-1)) + 5e-4)
+        assert np.all(err <= lim), f"{m}: max {err.max()}"
+    if "cmap" in g:
+        body = open(tmp_path / g["cmap"]["file"], "rb").read().split(b"\n", 6)[6]
+        assert hashlib.sha256(body).hexdigest() == g["cmap"]["sha256"]

@@ -45,6 +45,56 @@ def test_oracle_text_equals_reference(name, tmp_path):
         assert hashlib.sha256(text).hexdigest() == g["maps"][fn]["sha256"], fn
 
 
+BH_GOLDEN = GOLDEN.get("bhtree", {})
+BH_SUCCESS = [n for n in cases.BH_CASES if BH_GOLDEN.get(n, {}).get("success")]
+
+
+@pytest.mark.parametrize("name", BH_SUCCESS)
+def test_oracle_tree_build_text_equals_reference(name, tmp_path):
+    """The oracle in the tree build's mode (FP32 radius queries, the 1.5 A
+    collision rule) against the reference built with -DUSE_BHTREE."""
+    st, gpf = parity.load_case_bh(name, str(tmp_path))
+    interior = "map_receptor_interior on" in cases.BH_CASES[name][1]
+    assert st.bhtree == (2 if interior else 1)
+    g = BH_GOLDEN[name]
+    m, fm, cm = ol.maps(ol.oracle_state(st), fmap=True, cmap=True)
+    if st.floating_grid:
+        assert hashlib.sha256(ol.format_rmin_map(fm)).hexdigest() == g["fmap"]["sha256"]
+    if st.constriction_grid:
+        assert hashlib.sha256(ol.format_count_map(cm)).hexdigest() == g["cmap"]["sha256"]
+    for k, fn in enumerate(cases.map_names(os.path.join(tmp_path, gpf))):
+        text = ol.format_map(m[k])
+        assert hashlib.sha256(text).hexdigest() == g["maps"][fn]["sha256"], fn
+
+
+def test_tree_build_cases_differ_from_default():
+    """The tree build's maps are not the default build's (collisions zero
+    e, FP32 distances move bins), so the tree-build pins are not vacuous."""
+    assert BH_SUCCESS, "no tree-build fixtures"
+    differ = 0
+    for name in BH_SUCCESS:
+        base, extra = cases.BH_CASES[name]
+        if extra:
+            continue
+        for fn, rec in BH_GOLDEN[name]["maps"].items():
+            differ += rec["sha256"] != GOLDEN[base]["maps"][fn]["sha256"]
+    assert differ > 0
+
+
+def test_map_receptor_interior_off_needs_the_tree(tmp_path):
+    """'map_receptor_interior off' is fatal without the tree build
+    (main.cpp:1655-1656) and accepted with it."""
+    gpf = cases.materialize_bh("interior_off", str(tmp_path))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        with pytest.raises(ag.AgError, match="map_receptor_interior off"):
+            ag.Job(gpf)
+        assert ag.Job(gpf, bhtree=True).state().bhtree == 1
+    finally:
+        os.chdir(cwd)
+
+
 @pytest.mark.skipif(not os.path.exists(ol.REF_BIN), reason="reference autogrid4 not built here")
 @pytest.mark.parametrize("name", ["hsg1_sm", "disorder", "sepdesolv"])
 def test_live_reference_run_matches_golden(name, tmp_path):

@@ -162,6 +162,12 @@ PAIR_ABI_DTYPE = np.dtype([("cA", np.float64), ("cB", np.float64), ("Rij", np.fl
                            ("xA", np.int32), ("xB", np.int32), ("hbonder", np.int32), ("reserved", np.int32)])
 
 
+def read_binary_map(path: str, n1) -> np.ndarray:
+    """A map of the binary side-output: raw little-endian float32, x fastest,
+    as [n1z, n1y, n1x]."""
+    return np.fromfile(path, dtype="<f4").reshape(n1[2], n1[1], n1[0])
+
+
 def _arr(ptr, n, dtype, shape=None):
     if n == 0 or not ptr:
         return np.zeros(shape if shape else (0,), dtype)
@@ -173,9 +179,10 @@ class Job:
     """A GPF read by the engine's front end (ag_job_open)."""
 
     def __init__(self, gpf_path: str, glg_path: str | None = None, program: str = "autogrid4",
-                 files: bool = False, bhtree: bool = False):
+                 files: bool = False, bhtree: bool = False, binary_maps: bool = False):
         h = C.c_void_p()
-        flags = (0 if files else _abi.AG_JOB_NO_FILES) | (_abi.AG_JOB_BHTREE if bhtree else 0)
+        flags = ((0 if files else _abi.AG_JOB_NO_FILES) | (_abi.AG_JOB_BHTREE if bhtree else 0)
+                 | (_abi.AG_JOB_BINARY_MAPS if binary_maps else 0))
         check(lib().ag_job_open(os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
                                 program.encode(), flags, C.byref(h)))
         self._h = h
@@ -208,6 +215,15 @@ class Job:
     @property
     def points(self) -> int:
         return int(lib().ag_job_points(self._h))
+
+    def write_maps(self, maps: np.ndarray, fmap: np.ndarray | None = None, cmap: np.ndarray | None = None,
+                   threads: int = 4) -> None:
+        """Write the job's output files from host maps (ag_job_write_maps)."""
+        maps = np.ascontiguousarray(maps, np.float32)
+        fm = None if fmap is None else np.ascontiguousarray(fmap, np.float32)
+        cm = None if cmap is None else np.ascontiguousarray(cmap, np.int32)
+        check(lib().ag_job_write_maps(self._h, maps.ctypes.data, fm.ctypes.data if fm is not None else None,
+                                      cm.ctypes.data if cm is not None else None, int(threads)))
 
     def state(self) -> PreparedState:
         """Copy of the parsed (and, after prepare(), prepared) state."""
@@ -337,9 +353,10 @@ class Context:
         return {"cell_ms": ms[0], "near_ms": ms[1], "far_ms": ms[2], "total_ms": ms[3]}
 
     def run_gpf(self, gpf_path: str, glg_path: str | None = None, program: str = "autogrid4",
-                bhtree: bool = False) -> None:
+                bhtree: bool = False, binary_maps: bool = False) -> None:
+        flags = (_abi.AG_JOB_BHTREE if bhtree else 0) | (_abi.AG_JOB_BINARY_MAPS if binary_maps else 0)
         check(lib().ag_run_gpf_flags(self._h, os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
-                                     program.encode(), _abi.AG_JOB_BHTREE if bhtree else 0))
+                                     program.encode(), flags))
 
 
 def _tune_struct(tune: dict | None):
@@ -430,10 +447,11 @@ class Plan:
 
 
 def run_gpf(gpf_path: str, glg_path: str | None = None, device: int = 0, program: str = "autogrid4",
-            bhtree: bool = False) -> None:
-    """autogrid4 -p gpf -l glg, on one GPU (bhtree: a -DUSE_BHTREE build's semantics)."""
+            bhtree: bool = False, binary_maps: bool = False) -> None:
+    """autogrid4 -p gpf -l glg, on one GPU (bhtree: a -DUSE_BHTREE build's
+    semantics; binary_maps: also the raw FP32 side-output)."""
     with Context(device) as ctx:
-        ctx.run_gpf(gpf_path, glg_path, program, bhtree)
+        ctx.run_gpf(gpf_path, glg_path, program, bhtree, binary_maps)
 
 
 class Scorer:

@@ -21,6 +21,7 @@ STATUS_NAMES = {0: "AG_OK", 1: "AG_ERR_BAD_INPUT", 2: "AG_ERR_NO_DONOR_H", 3: "A
                 4: "AG_ERR_DEVICE", 5: "AG_ERR_IO", 6: "AG_ERR_UNSUPPORTED"}
 AG_JOB_NO_FILES = 1
 AG_JOB_BHTREE = 2
+AG_JOB_BINARY_MAPS = 4
 AG_NEINT = 1024
 AG_POTENTIAL_AD4 = 0
 AG_POTENTIAL_VINA = 1

@@ -78,6 +78,8 @@ struct Job {
     bool make_files = true;
     bool bhtree = false;                // semantics of a -DUSE_BHTREE build
     bool map_receptor_interior = false; // main.cpp:580
+    bool binary_maps = false;           // raw FP32 side-output next to the ASCII maps
+    bool headers_written = false;       // the 6 header lines of every map file
     FILE *fld_fp = nullptr, *xyz_fp = nullptr, *fmap_fp = nullptr, *cmap_fp = nullptr;
     bool xyz_open = false;       // gridfld seen and .xyz not yet written
     int outlev = 1;

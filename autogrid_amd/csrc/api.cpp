@@ -157,6 +157,7 @@ ag_status ag_job_open(const char *gpf_path, const char *glg_path, const char *pr
     ag::Job &job = j->job;
     job.make_files = !(flags & AG_JOB_NO_FILES);
     job.bhtree = (flags & AG_JOB_BHTREE) != 0;
+    job.binary_maps = (flags & AG_JOB_BINARY_MAPS) != 0;
     if (program_name) job.program = program_name;
     ag_status st = guarded([&] {
         if (glg_path) {
@@ -578,6 +579,10 @@ ag_status ag_job_write_maps(ag_job *j, const float *maps, const float *fmap, con
         return AG_ERR_BAD_INPUT;
     }
     ag_status st = guarded([&] {
+        if (!j->job.prepared) ag::fail(AG_ERR_BAD_INPUT, "ag_job_write_maps needs a prepared job");
+        // the .fld and the map headers, unless already written
+        ag::job_write_fld(j->job);
+        ag::job_write_map_headers(j->job);
         ag::log_map_summary(j->job, ag::write_all_maps(j->job, maps, fmap, cmap, n_threads > 0 ? n_threads : 1));
         j->job.logf("\n%s: Successful Completion.\n", j->job.program.c_str());
         if (j->job.log) fflush(j->job.log);
@@ -593,7 +598,7 @@ ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path, co
 ag_status ag_run_gpf_flags(ag_ctx *ctx, const char *gpf_path, const char *glg_path, const char *program_name,
                            uint32_t flags) {
     ag_job *j = nullptr;
-    ag_status st = ag_job_open(gpf_path, glg_path, program_name, flags & AG_JOB_BHTREE, &j);
+    ag_status st = ag_job_open(gpf_path, glg_path, program_name, flags & (AG_JOB_BHTREE | AG_JOB_BINARY_MAPS), &j);
     if (st != AG_OK) return st;
     ag::Job &job = j->job;
     st = guarded([&] {

@@ -15,7 +15,8 @@ static const char *kHelp =
     "\t\t-d (increment debug level)\n"
     "\t\t-h (display this message)\n"
     "\t\t--version (print version information, copyright, and license)\n"
-    "\t\t--bhtree (the semantics of a -DUSE_BHTREE build)\n";
+    "\t\t--bhtree (the semantics of a -DUSE_BHTREE build)\n"
+    "\t\t--binary-maps (also write raw FP32 maps and a binary .fld)\n";
 
 int main(int argc, char **argv) {
     const char *prog = argv[0];
@@ -28,8 +29,9 @@ int main(int argc, char **argv) {
         switch (a[1]) {
             case 'd': break;
             case 'b':
-                if (strcmp(a, "-bhtree") != 0) { fprintf(stderr, "%s: unknown switch %s\n", prog, argv[i]); return 1; }
-                flags |= AG_JOB_BHTREE;
+                if (strcmp(a, "-bhtree") == 0) flags |= AG_JOB_BHTREE;
+                else if (strcmp(a, "-binary-maps") == 0) flags |= AG_JOB_BINARY_MAPS;
+                else { fprintf(stderr, "%s: unknown switch %s\n", prog, argv[i]); return 1; }
                 break;
             case 'u':
             case 'h': printf("usage: AutoGrid %s\n", kHelp); return 0;

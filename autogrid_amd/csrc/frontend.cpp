@@ -869,10 +869,15 @@ void job_parse_gpf(Job &job, const std::string &gpf_path) {
 }
 
 // AVS .fld header (main.cpp:1897-1956).
-void job_write_fld(Job &job) {
-    FILE *f = job.fld_fp;
-    if (!f) return;
+// The .fld text (main.cpp:1903-1956).  With `binary`, the variables point
+// at the raw FP32 side-output files (<map>.bin, little-endian, x fastest,
+// no header) instead of the ASCII maps.
+static void write_fld_text(Job &job, FILE *f, bool binary) {
     const char *gpf = job.gpf_path.c_str();
+    auto var = [&](int k, const std::string &fn) {
+        if (binary) fprintf(f, "variable %d file=%s.bin filetype=binary skip=0\n", k, fn.c_str());
+        else fprintf(f, "variable %d file=%s filetype=ascii skip=6\n", k, fn.c_str());
+    };
     fprintf(f, "# AVS field file\n#\n");
     fprintf(f, "# AutoDock Atomic Affinity and Electrostatic Grids\n#\n");
     fprintf(f, "# Created by %s %s.\n#\n", job.program.c_str(), "4.2.7.x");
@@ -909,21 +914,32 @@ void job_write_fld(Job &job) {
         fprintf(f, "label=Constriction_Grid\t# component label for variable %d\n",
                 base + (job.floating_grid ? 1 : 0) + 1);
     fprintf(f, "#\n# location of affinity grid files and how to read them\n#\n");
-    for (int i = 0; i < job.num_atom_maps; i++)
-        fprintf(f, "variable %d file=%s filetype=ascii skip=6\n", i + 1, job.maps[i].filename.c_str());
-    if (job.elecPE >= 0)
-        fprintf(f, "variable %d file=%s filetype=ascii skip=6\n", job.num_atom_maps + 1,
-                job.maps[job.elecPE].filename.c_str());
-    if (job.dsolvPE >= 0)
-        fprintf(f, "variable %d file=%s filetype=ascii skip=6\n",
-                job.num_atom_maps + ((job.elecPE >= 0) ? 2 : 1), job.maps[job.dsolvPE].filename.c_str());
-    if (job.floating_grid)
-        fprintf(f, "variable %d file=%s filetype=ascii skip=6\n", job.num_maps + 1, job.fmap_filename.c_str());
-    if (job.constriction_grid)
-        fprintf(f, "variable %d file=%s filetype=ascii skip=6\n",
-                job.num_maps + (job.floating_grid ? 1 : 0) + 1, job.cmap_filename.c_str());
+    for (int i = 0; i < job.num_atom_maps; i++) var(i + 1, job.maps[i].filename);
+    if (job.elecPE >= 0) var(job.num_atom_maps + 1, job.maps[job.elecPE].filename);
+    if (job.dsolvPE >= 0) var(job.num_atom_maps + ((job.elecPE >= 0) ? 2 : 1), job.maps[job.dsolvPE].filename);
+    if (job.floating_grid) var(job.num_maps + 1, job.fmap_filename);
+    if (job.constriction_grid) var(job.num_maps + (job.floating_grid ? 1 : 0) + 1, job.cmap_filename);
+}
+
+static std::string binary_fld_name(const Job &job) {
+    const std::string &fn = job.fld_filename;
+    if (fn.size() > 4 && fn.compare(fn.size() - 4, 4, ".fld") == 0) return fn.substr(0, fn.size() - 4) + ".bin.fld";
+    return fn + ".bin.fld";
+}
+
+void job_write_fld(Job &job) {
+    FILE *f = job.fld_fp;
+    if (!f) return;
+    write_fld_text(job, f, false);
     fclose(f);
     job.fld_fp = nullptr;
+    if (job.binary_maps && job.make_files) {
+        const std::string bfn = binary_fld_name(job);
+        FILE *b = fopen(bfn.c_str(), "w");
+        if (!b) fail(AG_ERR_IO, "can't create the binary grid data file " + bfn + "\n");
+        write_fld_text(job, b, true);
+        if (fclose(b) != 0) fail(AG_ERR_IO, "Problems writing " + bfn + "\n");
+    }
 }
 
 // Six header lines of every map file (main.cpp:2752-2776).
@@ -937,6 +953,8 @@ static void write_header(Job &job, FILE *f) {
 }
 
 void job_write_map_headers(Job &job) {
+    if (job.headers_written) return;
+    job.headers_written = true;
     for (int k = 0; k < job.num_maps; k++)
         if (job.maps[k].fp) write_header(job, job.maps[k].fp);
     if (job.fmap_fp) write_header(job, job.fmap_fp);

@@ -110,6 +110,21 @@ MapExtrema write_all_maps(Job &job, const float *maps, const float *fmap, const 
             }
             const size_t n = (size_t)(o - &text[0]);
             if (f && (fwrite(text.data(), 1, n, f) != n || fflush(f) != 0)) problem = true;
+            // binary side-output: the FP32 values themselves, counts as float
+            if (job.binary_maps && job.make_files) {
+                const std::string fn = (j < nm ? job.maps[j].filename : (j == nm ? job.fmap_filename : job.cmap_filename))
+                                       + ".bin";
+                std::vector<float> cnt;
+                const float *src = j < nm ? maps + (size_t)j * P : (j == nm ? fmap : nullptr);
+                if (j > nm) {
+                    cnt.resize((size_t)P);
+                    for (int64_t i = 0; i < P; i++) cnt[(size_t)i] = (float)cmap[i];
+                    src = cnt.data();
+                }
+                FILE *b = fopen(fn.c_str(), "wb");
+                if (!b || fwrite(src, sizeof(float), (size_t)P, b) != (size_t)P) problem = true;
+                if (b && fclose(b) != 0) problem = true;
+            }
         }
     };
     const int nt = std::max(1, std::min(n_threads, (int)tasks.size()));

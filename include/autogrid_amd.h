@@ -183,6 +183,12 @@ typedef struct ag_job ag_job;
                                  ag_mapset bhtree mode, map_receptor_interior
                                  off accepted (main.cpp:1643-1658), and the
                                  log's USE_BHTREE line reading yes          */
+#define AG_JOB_BINARY_MAPS 4u /* also write each map as raw FP32 (<map>.bin:
+                                 little-endian, x fastest, no header; counts
+                                 as float) and <name>.maps.bin.fld, an AVS
+                                 field over them (filetype=binary); the
+                                 reference writes ASCII only (main.cpp:1917-1920,
+                                 3481-3497)                                 */
 
 /* Read a GPF with the keyword semantics of main.cpp:778-1819 (order
  * dependence included), read the receptor PDBQT (main.cpp:823-1047) and any
@@ -207,7 +213,7 @@ void ag_job_close(ag_job *job);
  * empty gpf_path reads the GPF from standard input (setflags.cpp:78). */
 ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path,
                      const char *program_name);
-/* ag_run_gpf with ag_job_open flags (AG_JOB_BHTREE). */
+/* ag_run_gpf with ag_job_open flags (AG_JOB_BHTREE, AG_JOB_BINARY_MAPS). */
 ag_status ag_run_gpf_flags(ag_ctx *ctx, const char *gpf_path, const char *glg_path,
                            const char *program_name, uint32_t flags);
 
@@ -480,9 +486,12 @@ ag_status ag_scorer_destroy(ag_scorer *s);
 int64_t ag_format_map_values(const float *values, int64_t n, char *buf,
                              int64_t cap);
 
-/* Copy the per-job maps from a computed [n_maps][P] host array into the
- * job's .map files (+ fmap/cmap), then log the min/max summary and the
- * completion line.  Formatting runs on `n_threads` host threads. */
+/* Write a prepared job's outputs (opened with files): the .fld
+ * (main.cpp:1903-1956) and the map headers (main.cpp:2752-2776) unless
+ * already written, then the per-job maps from a computed [n_maps][P] host
+ * array into the .map files (+ fmap/cmap; with AG_JOB_BINARY_MAPS the raw
+ * FP32 side-output too), then the min/max summary and the completion line
+ * of the log.  Formatting runs on `n_threads` host threads. */
 ag_status ag_job_write_maps(ag_job *job, const float *maps, const float *fmap,
                             const int32_t *cmap, int n_threads);
 

@@ -182,3 +182,26 @@ def test_cli_dropin_tree_build(name, tmp_path):
     if "cmap" in g:
         body = open(tmp_path / g["cmap"]["file"], "rb").read().split(b"\n", 6)[6]
         assert hashlib.sha256(body).hexdigest() == g["cmap"]["sha256"]
+
+
+def test_cli_binary_maps(ctx, tmp_path):
+    """autogrid4 --binary-maps: the .bin files hold the very floats the ASCII
+    text was formatted from, and those equal the library's map set bit for
+    bit (same kernels, fixed-order sums)."""
+    print("[gpu] cli --binary-maps", flush=True)
+    name = "fmap_cmap"
+    gpf = cases.materialize(name, str(tmp_path))
+    p = subprocess.run([ag.CLI_PATH, "--binary-maps", "-p", gpf, "-l", "out.glg"], cwd=tmp_path,
+                       capture_output=True, timeout=120)
+    assert p.returncode == 0, p.stderr.decode()
+    st, _ = parity.load_case(name, str(tmp_path / "lib"))
+    want = ctx.maps_plain(st)
+    for k, fn in enumerate(cases.map_names(os.path.join(tmp_path, gpf))):
+        b = ag.read_binary_map(str(tmp_path / (fn + ".bin")), st.n1)
+        assert np.array_equal(b, want[k]), fn
+        text = open(tmp_path / fn, "rb").read().split(b"\n", 6)[6]
+        assert text == ag.format_map_values(b)
+    g = GOLDEN[name]
+    cm = np.fromfile(tmp_path / (g["cmap"]["file"] + ".bin"), "<f4")
+    assert np.array_equal(cm, ctx.last_cmap.reshape(-1).astype(np.float32))
+    assert (tmp_path / f"{name}.maps.bin.fld").exists()

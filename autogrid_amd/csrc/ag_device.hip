@@ -400,7 +400,6 @@ template <int NPLAIN, bool CUBE>
 __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     __shared__ int cand[kCandCap];
     __shared__ int hcand[kHCap];
-    __shared__ NearAtom rec[kChunk];
     __shared__ double hpos[kHCap][3];
     __shared__ int hidx[kHCap];
     __shared__ float hrvs[kHCap][3];
@@ -594,13 +593,12 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     // cos and sin of 1.24791 rad, the disordered-hydroxyl offset angle
     const float cdis = 0.31936680f, sdis = 0.94763168f;
 
-    for (int cbase = 0; cbase < ncand; cbase += kChunk) {
-        const int nc = min(kChunk, ncand - cbase);
-        __syncthreads();
-        for (int j = tid; j < nc; j += kNearThreads) rec[j] = p.atoms[cand[cbase + j]];
-        __syncthreads();
-        for (int j = 0; j < nc; j++) {
-            const NearAtom &a = rec[j];
+    {
+        // candidate records go straight into scalar registers through the
+        // constant address space (the atom array is read-only here)
+        const AG_CONST NearAtom *catoms = as_const(p.atoms);
+        for (int j = 0; j < ncand; j++) {
+            const AG_CONST NearAtom &a = catoms[__builtin_amdgcn_readfirstlane(cand[j])];
             // geometry: FP64 difference as the reference forms it, FP32 after
             const double dx64 = a.x - cx, dy64 = a.y - cy, dz64 = a.z - cz;
             const float dxf = (float)dx64, dyf = (float)dy64, dzf = (float)dz64;

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -35,6 +36,7 @@
 #include <functional>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ag_internal.h"
@@ -1149,7 +1151,8 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
 // Near-kernel layout chosen by the autotuner.
 struct Tune {
     double cell_size = 4.0;     // cell edge of the cell list (A)
-    bool cube = false;          // lanes of a wave on a 4x4x4 cube instead of an 8x8x1 slab
+    bool cube = true;           // lanes of a wave on a 4x4x4 cube instead of an 8x8x1 slab
+                                // (the cube measures faster on the 5,000-atom, 127^3 map set)
 };
 
 
@@ -1460,10 +1463,13 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     int *const type = (int *)(h + o_type), *const hbond = (int *)(h + o_hbond), *const rexp = (int *)(h + o_rexp);
     int *const atom_conf = (int *)(h + o_conf);
     unsigned char *const dis = (unsigned char *)(h + o_dis);
-    for (int i = 0, o = 0; i < n; o += natoms[i], i++) {
+    std::vector<int> first_atom(n + 1, 0);
+    for (int i = 0; i < n; i++) first_atom[i + 1] = first_atom[i] + natoms[i];
+    auto stage_conf = [&](int i) {
+        const int o = first_atom[i];
         const ag_receptor &r = recs[i];
         const size_t k = natoms[i];
-        if (!k) continue;
+        if (!k) return;
         std::copy(r.coord, r.coord + 3 * k, coord + 3 * (size_t)o);
         std::copy(r.charge, r.charge + k, charge + o);
         std::copy(r.vol, r.vol + k, vol + o);
@@ -1475,6 +1481,19 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         std::copy(r.rexp, r.rexp + k, rexp + o);
         std::copy(r.disorder, r.disorder + k, dis + o);
         std::fill(atom_conf + o, atom_conf + o + k, i);
+    };
+    // large batches copy on several host threads, one conformation at a time
+    const int n_thr = N > 200000 ? std::min(8, n) : 1;
+    if (n_thr > 1) {
+        std::atomic<int> next{0};
+        std::vector<std::thread> th;
+        for (int t = 0; t < n_thr; t++)
+            th.emplace_back([&] {
+                for (int i; (i = next.fetch_add(1)) < n;) stage_conf(i);
+            });
+        for (auto &x : th) x.join();
+    } else {
+        for (int i = 0; i < n; i++) stage_conf(i);
     }
     double *d_coord = upload(st, p.coord, coord, n3);
     double *d_charge = upload(st, p.charge, charge, nn);
@@ -2142,7 +2161,7 @@ void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids,
     hipStream_t st = c->stream;
     if (pinned) {
         const HostOut host{maps_out, fmap_out, cmap_out};
-        const auto chunks = plan_chunks(*p, 12);
+        const auto chunks = plan_chunks(*p, kMaxChunks);
         HIP_CHECK(hipEventRecord(c->ev[0], st));
         enqueue(p.get(), d_maps, d_fmap, d_cmap, false, false, chunks, &host);
         HIP_CHECK(hipEventRecord(c->ev[3], st));

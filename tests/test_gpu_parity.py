@@ -205,3 +205,28 @@ def test_cli_binary_maps(ctx, tmp_path):
     cm = np.fromfile(tmp_path / (g["cmap"]["file"] + ".bin"), "<f4")
     assert np.array_equal(cm, ctx.last_cmap.reshape(-1).astype(np.float32))
     assert (tmp_path / f"{name}.maps.bin.fld").exists()
+
+
+@pytest.mark.parametrize("npts", ["2 2 2", "2 4 6", "2 2 40", "10 1 3", "40 2 2"])
+def test_small_and_ragged_grids(ctx, npts, tmp_path):
+    """Grids smaller than a brick or a far-kernel run, and ragged ones (odd
+    npts are rounded as check_size.cpp does): the small receptor's maps on
+    each against the oracle, at every point."""
+    print(f"[gpu] grid npts {npts}", flush=True)
+    gpf = cases.materialize("small", str(tmp_path))
+    path = os.path.join(tmp_path, gpf)
+    lines = open(path).read().split("\n")
+    lines = [f"npts {npts}" if ln.startswith("npts") else ln for ln in lines]
+    open(path, "w").write("\n".join(lines))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        st = ag.Job(gpf).prepare().state()
+    finally:
+        os.chdir(cwd)
+    ref = ol.maps(ol.oracle_state(st))
+    got = ctx.maps_plain(st)
+    assert got.shape == ref.shape
+    for k in range(st.n_maps):
+        ok, mx, rms, bad = parity.within(got[k], ref[k])
+        assert ok, f"npts {npts} map {k}: max {mx} rms {rms} bad {bad}"

@@ -58,7 +58,7 @@ constexpr int kFarK = 16;                         // points per lane, far kernel
 constexpr int kFarRuns = 256;                     // runs (lanes of kFarK points) per far-kernel item
 constexpr int kFarThreads = 2 * kFarRuns;         // two halves of the atom list per run
 constexpr int kFarTabMax = 16380;                 // LDS table entries without the clamp (two workgroups per CU)
-constexpr int kFarAtomChunk = 64;                 // Kahan block
+constexpr int kFarAtomChunk = 128;                // Kahan block
 constexpr int kFarBinMin = 1025;                  // far kernel takes bins >= this
 constexpr float kGuardBins = 1.0e-3f;             // FP64 recompute band around bin edges
 
@@ -1022,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
 //
 // An item is kFarRuns runs; the workgroup's two halves (four waves each)
 // take the same runs, the first half the atoms [0, a_mid), the second
-// [a_mid, n), a_mid rounded up to a Kahan block.  The second half leaves its
+// [a_mid, n), a_mid = ceil(n / 2).  The second half leaves its
 // compensated sums in LDS and the first adds them in a fixed order, so the
 // result does not depend on scheduling.  Splitting the atoms keeps four
 // waves per SIMD on grids that give only two per SIMD at one wave per run.
@@ -1075,7 +1075,7 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
             // atoms are read-only here: scalar loads, one atom ahead
             const AG_CONST PlainF4 *atoms = as_const(reinterpret_cast<const PlainF4 *>(p.atoms + cf.atom_base));
             const int n_atoms = cf.n_atoms;
-            const int a_mid = min(n_atoms, ((n_atoms + 1) / 2 + kFarAtomChunk - 1) / kFarAtomChunk * kFarAtomChunk);
+            const int a_mid = (n_atoms + 1) / 2;   // Kahan blocks of each half start at its first atom
             const int a_beg = half ? a_mid : 0;
             const int a_end = half ? n_atoms : a_mid;
             PlainF4 nxt = ld4(atoms, min(a_beg, max(a_end - 1, 0)));

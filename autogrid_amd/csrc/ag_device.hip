@@ -151,7 +151,6 @@ struct NearParams {
     const float *reps;            // [8192] electrostatic factor (r_eps or 332/diel)
     // map slots of this pass: [0, kPlainSlots) maps without H-bond terms,
     // [kPlainSlots, kMapSlots) H-bonding probes
-    float kE[kMapSlots];          // 1: add the vdW/H-bond table value; 0: desolvation only or unused
     float sp[kMapSlots];          // solpar_probe, 0 when the slot carries no desolvation
     float volp[kMapSlots];        // vol_probe, 0 when the slot carries no desolvation
     int hcode[kHbSlots][7];       // per atom H-bond class: bits 0-3 ordered atom, 4-7 disordered
@@ -399,7 +398,7 @@ __device__ __forceinline__ float unif(float v) {
 // CUBE selects the lane layout inside the 8x8x4 brick: each wave takes an
 // 8x8x1 slab (false) or a 4x4x4 cube (true) of points.
 template <int NPLAIN, bool CUBE>
-__global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
+__global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_eu(5))) k_near(NearParams p) {
     __shared__ int cand[kCandCap];
     __shared__ int hcand[kHCap];
     __shared__ double hpos[kHCap][3];
@@ -595,14 +594,46 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
     // cos and sin of 1.24791 rad, the disordered-hydroxyl offset angle
     const float cdis = 0.31936680f, sdis = 0.94763168f;
 
+    // H-bond codes of the pass, one word per atom class (lane hb holds
+    // hcode[k][hb] in byte k), read with readlane instead of a load per atom.
+    // Cross-lane reads (this one and the candidate index) happen only at the
+    // top of a candidate iteration, where every lane of the wave is active:
+    // a register reloaded from a spill under a partial exec mask would hold
+    // stale values in the inactive lanes.
+    const int wlane = tid & 63;
+    const int hcw = wlane < 7 ? (p.hcode[0][wlane] | p.hcode[1][wlane] << 8 | p.hcode[2][wlane] << 16 |
+                                 p.hcode[3][wlane] << 24)
+                              : 0;
     {
         // candidate records go straight into scalar registers through the
-        // constant address space (the atom array is read-only here)
+        // constant address space (the atom array is read-only here).  The
+        // wave holds 64 candidate indices in one register and loads the
+        // next record's coordinates, class and index one candidate ahead,
+        // which also brings both of its cache lines into the scalar cache.
         const AG_CONST NearAtom *catoms = as_const(p.atoms);
-        for (int j = 0; j < ncand; j++) {
-            const AG_CONST NearAtom &a = catoms[__builtin_amdgcn_readfirstlane(cand[j])];
+        for (int j0 = 0; j0 < ncand; j0 += 64) {
+        const int vcand = cand[min(j0 + wlane, ncand - 1)];
+        const int jn = min(64, ncand - j0);
+        int idx = __builtin_amdgcn_readlane(vcand, 0);
+        double nx = catoms[idx].x, ny = catoms[idx].y, nz = catoms[idx].z;
+        int nmeta = catoms[idx].meta, nindex = catoms[idx].index;
+        for (int jj = 0; jj < jn; jj++) {
+            const AG_CONST NearAtom &a = catoms[idx];
+            const double ax = nx, ay = ny, az = nz;
+            const int ameta = nmeta, aindex = nindex;
+            const unsigned hw = (unsigned)__builtin_amdgcn_readlane(hcw, (ameta >> 8) & 63);
+            idx = __builtin_amdgcn_readlane(vcand, min(jj + 1, jn - 1));
+            // the next loads may issue only once this record has arrived:
+            // scalar loads complete out of order, so the wait for this
+            // record would otherwise also wait for the next one
+            asm volatile("" : "+s"(idx) : "s"(ameta), "s"(aindex), "s"(ax), "s"(ay), "s"(az));
+            nx = catoms[idx].x;
+            ny = catoms[idx].y;
+            nz = catoms[idx].z;
+            nmeta = catoms[idx].meta;
+            nindex = catoms[idx].index;
             // geometry: FP64 difference as the reference forms it, FP32 after
-            const double dx64 = a.x - cx, dy64 = a.y - cy, dz64 = a.z - cz;
+            const double dx64 = ax - cx, dy64 = ay - cy, dz64 = az - cz;
             const float dxf = (float)dx64, dyf = (float)dy64, dzf = (float)dz64;
             const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
             if (r2 > 64.33f) continue;  // beyond 8.02 A: neither cutoff pass nor near e term
@@ -614,7 +645,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             bool cut;
             if (fabsf(g - rintf(g)) < kGuardBins) {
                 double rdx, rdy, rdz;
-                const double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                const double r64 = ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
                 bin = (int)(r64 * 128.0);
                 cut = r64 > 8.0;
                 r = (float)r64;
@@ -628,7 +659,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             bool in8 = false;
             float rB = 0.f;
             if (bh) {
-                const float e0 = fcx - (float)a.x, e1 = fcy - (float)a.y, e2 = fcz - (float)a.z;
+                const float e0 = fcx - (float)ax, e1 = fcy - (float)ay, e2 = fcz - (float)az;
                 const float a0 = fabsf(e0), a1 = fabsf(e1), a2 = fabsf(e2);
                 const float D = (e0 * e0 + e1 * e1) + e2 * e2;
                 in8 = a0 <= 8.f && a1 <= 8.f && a2 <= 8.f && D <= 64.f;
@@ -640,7 +671,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             if (cmap && !bh) {
                 if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
                     double rdx, rdy, rdz;
-                    ccount += ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz) <= p.ccut;
+                    ccount += ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz) <= p.ccut;
                 } else {
                     ccount += r <= (float)p.ccut;
                 }
@@ -665,14 +696,14 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 bin = (int)((double)rB * 128.0);
             }
             if (cut || !p.has_affinity) continue;
-            const int meta = uni(a.meta);
+            const int meta = ameta;
             const int type = meta & 0xff, hb = (meta >> 8) & 0xff, rexp = (meta >> 16) & 0xff;
             const bool dis = (meta >> 24) & 1;
             if (p.disorder_h && dis && type == p.hydrogen_type) continue;
             const float inv_r = __builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f));
             const float dn0 = dxf * inv_r, dn1 = dyf * inv_r, dn2 = dzf * inv_r;
             const float rv0 = unif(a.rv[0]), rv1 = unif(a.rv[1]), rv2 = unif(a.rv[2]);
-            const int aidx = uni(a.index);
+            const int aidx = aindex;
             float racc = 1.f, rdon = 1.f, Hramp = 1.f;
             if (hb == AG_HB_D1 || (hb == AG_HB_A2 && !dis)) {
                 // cos(theta) of the D1 / ordered-A2 branches; FP64 near 0
@@ -680,7 +711,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 bool pos;
                 if (fabsf(cos_t) < 1.0e-5f) {
                     double rdx, rdy, rdz;
-                    double r64 = ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                    double r64 = ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
                     if (bh) r64 = (double)rB;   // the tree build's r (main.cpp:3026)
                     if (r64 < 1.0e-6) r64 = 1.0e-6;
                     const double ir = 1. / r64;
@@ -771,7 +802,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             }
             // plain slots: the table value
 #pragma unroll
-            for (int m = 0; m < NPLAIN; m++) acc[m] = __builtin_fmaf(p.kE[m], E[m], acc[m]);
+            for (int m = 0; m < NPLAIN; m++) acc[m] += E[m];
             // H-bonding slots (main.cpp:3286-3374): the (probe, atom) case is a
             // set of 0/1 coefficients chosen per atom class.  The class is the
             // same for every lane, so the case is a scalar branch: a plain
@@ -782,20 +813,20 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
             bool any_mix = false;
 #pragma unroll
             for (int k = 0; k < kHbSlots; k++) {
-                const int code = (p.hcode[k][hb] >> dsh) & 15;
+                const int code = (hw >> (8 * k + dsh)) & 15;
                 any_mix |= (code & ~HC_PLAIN) != 0;
             }
             if (!any_mix) {
 #pragma unroll
                 for (int k = 0; k < kHbSlots; k++) {
-                    const int code = (p.hcode[k][hb] >> dsh) & 15;
+                    const int code = (hw >> (8 * k + dsh)) & 15;
                     if (code & HC_PLAIN) acc[kPlainSlots + k] += E[kPlainSlots + k];
                 }
             } else {
                 float Eh[kHbSlots] = {0.f, 0.f, 0.f, 0.f};
                 if (dis && p.hydrogen_type >= 0) {
                     double rdx, rdy, rdz;
-                    const double r64 = bh ? (double)rB : ref_dist(a.x, a.y, a.z, cx, cy, cz, rdx, rdy, rdz);
+                    const double r64 = bh ? (double)rB : ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
                     const int hb_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
                     const float4 v = reinterpret_cast<const float4 *>(
                         tab + ((size_t)p.hydrogen_type * AG_NEINT + hb_bin) * kMapSlots)[kPlainSlots / 4];
@@ -805,7 +836,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
 #pragma unroll
                 for (int k = 0; k < kHbSlots; k++) {
                     const int m = kPlainSlots + k;
-                    const int code = (p.hcode[k][hb] >> dsh) & 15;
+                    const int code = (hw >> (8 * k + dsh)) & 15;
                     const float e = E[m];
                     if (code & HC_PLAIN) acc[m] += e;
                     if ((code & ~HC_PLAIN) == 0) continue;
@@ -823,6 +854,7 @@ __global__ void __launch_bounds__(kNearThreads) k_near(NearParams p) {
                 }
             }
             if (p.d_slot >= 0) d_acc += unif(a.dvol) * S;
+        }
         }
     }
 
@@ -1555,7 +1587,9 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         for (int t = 0; t < T; t++)
             for (int j = 0; j < kMapSlots; j++) {
                 const int m = slot_map[g][j];
-                if (m < 0) continue;
+                // a slot without an affinity term (desolvation only) keeps
+                // zeros, so the kernel adds every slot's table value
+                if (m < 0 || !(map_role(ms, m) & ROLE_AFF)) continue;
                 const float *srcrow = tab->e_vdw_hb + ((size_t)t * M + m) * AG_NEINT;
                 float *dst = gtab.data() + g * gstride + (size_t)t * AG_NEINT * kMapSlots + j;
                 for (int k = 0; k < AG_NEINT; k++) dst[(size_t)k * kMapSlots] = srcrow[k];
@@ -1582,14 +1616,12 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         for (int j = 0; j < kMapSlots; j++) {
             const int m = slot_map[g][j];
             q.out_slot[j] = m;
-            q.kE[j] = 0.f;
             q.sp[j] = 0.f;
             q.volp[j] = 0.f;
             if (m < 0) continue;
             used = true;
             const int role = map_role(ms, m);
             const ag_map_desc &md = ms->maps[m];
-            q.kE[j] = (role & ROLE_AFF) ? 1.f : 0.f;
             if ((role & ROLE_DES) && !vina) {
                 q.sp[j] = (float)md.solpar_probe;
                 q.volp[j] = (float)md.vol_probe;

@@ -395,8 +395,23 @@ __device__ __forceinline__ float unif(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-// CUBE selects the lane layout inside the 8x8x4 brick: each wave takes an
-// 8x8x1 slab (false) or a 4x4x4 cube (true) of points.
+// A thread's point inside the 8x8x4 brick: each wave takes an 8x8x1 slab
+// (CUBE false) or a 4x4x4 cube (CUBE true) of points.
+template <bool CUBE>
+__device__ __forceinline__ void brick_lane(int tid, int &lx, int &ly, int &lz) {
+    if (CUBE) {
+        const int wave = tid >> 6, lane = tid & 63;
+        lx = (wave & 1) * 4 + (lane & 3);
+        ly = (wave >> 1) * 4 + ((lane >> 2) & 3);
+        lz = lane >> 4;
+    } else {
+        lx = tid % kBX;
+        ly = (tid / kBX) % kBY;
+        lz = tid / (kBX * kBY);
+    }
+}
+
+// CUBE selects the lane layout inside the 8x8x4 brick (brick_lane).
 template <int NPLAIN, bool CUBE>
 __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_eu(5))) k_near(NearParams p) {
     __shared__ int cand[kCandCap];
@@ -415,16 +430,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     for (int k = tid; k < kNearRepsLen; k += kNearThreads) reps_s[k] = p.reps[k];
     for (int k = tid; k <= AG_NEINT; k += kNearThreads) sol_s[k] = p.sol[k];
     int lx, ly, lz;
-    if (CUBE) {
-        const int wave = tid >> 6, lane = tid & 63;
-        lx = (wave & 1) * 4 + (lane & 3);
-        ly = (wave >> 1) * 4 + ((lane >> 2) & 3);
-        lz = lane >> 4;
-    } else {
-        lx = tid % kBX;
-        ly = (tid / kBX) % kBY;
-        lz = tid / (kBX * kBY);
-    }
+    brick_lane<CUBE>(tid, lx, ly, lz);
 
     // one workgroup per brick of all conformations.  (A persistent loop
     // over a work counter keeps more uniform state live than the scalar
@@ -445,7 +451,6 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     const int ix = bx * kBX + lx;
     const int iy = by * kBY + ly;
     const int iz = bz * kBZ + lz;
-    const bool active = ix < n1x && iy < n1y && iz < n1z;
 
     // brick box in the grid frame
     const int ix0 = bx * kBX, iy0 = by * kBY, iz0 = bz * kBZ;
@@ -457,16 +462,10 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     const double cx = (double)(ix - nex) * spacing;
     const double cy = (double)(iy - ney) * spacing;
     const double cz = (double)(iz - nez) * spacing;
-    // the far kernel's point coordinates: run centre, offset in the run, y, z
-    const int kxE = ix % kFarK;
-    const float cxE = far_cx(ix - kxE, nex, s128);
-    const float dkE = far_dk(kxE, s128);
-    const float pyE = (float)(iy - ney) * s128;
-    const float pzE = (float)(iz - nez) * s128;
-    const float biasE = far_bias(s128);
-    // the tree build's query point: the grid point as float (main.cpp:2860)
+    // (the far kernel's coordinates of the point and the tree build's FP32
+    // query point are formed where they are used, in rare branches of the
+    // candidate loop, so they hold no registers across it)
     const int bh = p.bhtree;
-    const float fcx = (float)cx, fcy = (float)cy, fcz = (float)cz;
     const float ccf = (float)p.ccut, ccf2 = ccf * ccf;
     bool collide = false;
 
@@ -584,11 +583,21 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
         hrv[0] = hrvs[best_h][0]; hrv[1] = hrvs[best_h][1]; hrv[2] = hrvs[best_h][2];
     }
 
+    // The point indices are formed again from the thread index where they
+    // are needed after this point (a rare branch of the candidate loop and
+    // the stores), so three indices do not stay live across the loop: one
+    // register instead of three, and no spill.
+    const auto point_index = [&](int &qx, int &qy, int &qz) {
+        int t = tid;
+        asm volatile("" : "+v"(t));   // opaque at each use: not hoisted out of the loop
+        brick_lane<CUBE>(t, qx, qy, qz);
+        qx += bx * kBX;
+        qy += by * kBY;
+        qz += bz * kBZ;
+    };
     float e_near = 0.f, d_acc = 0.f, dv_acc = 0.f, dq_acc = 0.f;
     int ccount = 0;
-    int *const cmap = p.cmap ? p.cmap + cf.pt_off : nullptr;
-    float *const maps = p.maps + cf.map_off;
-    const long long P = cf.P;
+    const bool want_cmap = p.cmap != nullptr;
     const float *tab = p.tab;
     const bool want_e = p.e_slot >= 0;
     // cos and sin of 1.24791 rad, the disordered-hydroxyl offset angle
@@ -659,16 +668,17 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
             bool in8 = false;
             float rB = 0.f;
             if (bh) {
-                const float e0 = fcx - (float)ax, e1 = fcy - (float)ay, e2 = fcz - (float)az;
+                // the tree build's query point: the grid point as float (main.cpp:2860)
+                const float e0 = (float)cx - (float)ax, e1 = (float)cy - (float)ay, e2 = (float)cz - (float)az;
                 const float a0 = fabsf(e0), a1 = fabsf(e1), a2 = fabsf(e2);
                 const float D = (e0 * e0 + e1 * e1) + e2 * e2;
                 in8 = a0 <= 8.f && a1 <= 8.f && a2 <= 8.f && D <= 64.f;
                 if (a0 <= 1.5f && a1 <= 1.5f && a2 <= 1.5f && D <= 2.25f) collide = true;
-                if (cmap) ccount += a0 <= ccf && a1 <= ccf && a2 <= ccf && D <= ccf2;
+                if (want_cmap) ccount += a0 <= ccf && a1 <= ccf && a2 <= ccf && D <= ccf2;
                 rB = sqrtf(D);   // correctly rounded, as the tree's sqrt of a float
             }
             // constriction grid: atoms within the cutoff (main.cpp:2969-2973)
-            if (cmap && !bh) {
+            if (want_cmap && !bh) {
                 if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
                     double rdx, rdy, rdz;
                     ccount += ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz) <= p.ccut;
@@ -681,7 +691,15 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 bool near_e;
                 if (g < 1022.f) near_e = true;
                 else if (g > 1027.f) near_e = false;
-                else near_e = far_bin4(a.xE, a.yE, a.zE, cxE, dkE, pyE, pzE, biasE) < 4u * kFarBinMin;
+                else {
+                    // the far kernel's point coordinates: run centre, offset in the run, y, z
+                    int qx, qy, qz;
+                    point_index(qx, qy, qz);
+                    const int kxE = qx % kFarK;
+                    near_e = far_bin4(a.xE, a.yE, a.zE, far_cx(qx - kxE, nex, s128), far_dk(kxE, s128),
+                                      (float)(qy - ney) * s128, (float)(qz - nez) * s128, far_bias(s128)) <
+                             4u * kFarBinMin;
+                }
                 if (near_e) {
                     // 1/max(r, 0.5) with r clamped at APPROX_ZERO
                     const float inv_rmax = fminf(__builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f)), 2.f);
@@ -858,8 +876,15 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
         }
     }
 
-    if (active) {
-        const long long pt = (long long)ix + (long long)n1x * ((long long)iy + (long long)n1y * iz);
+    int qx, qy, qz;
+    point_index(qx, qy, qz);
+    if (qx < n1x && qy < n1y && qz < n1z) {
+        // output pointers are formed here, after the candidate loop, so they
+        // hold no registers during it
+        int *const cmap = p.cmap ? p.cmap + cf.pt_off : nullptr;
+        float *const maps = p.maps + cf.map_off;
+        const long long P = cf.P;
+        const long long pt = (long long)qx + (long long)n1x * ((long long)qy + (long long)n1y * qz);
 #pragma unroll
         for (int m = 0; m < kMapSlots; m++) {
             if ((m >= NPLAIN && m < kPlainSlots) || p.out_slot[m] < 0) continue;

@@ -230,3 +230,49 @@ def test_small_and_ragged_grids(ctx, npts, tmp_path):
     for k in range(st.n_maps):
         ok, mx, rms, bad = parity.within(got[k], ref[k])
         assert ok, f"npts {npts} map {k}: max {mx} rms {rms} bad {bad}"
+
+
+@pytest.mark.parametrize("n_atoms", [1, 63, 64, 65, 129])
+def test_candidate_counts_around_index_blocks(ctx, n_atoms, tmp_path):
+    """Every brick of an 11^3 grid sees the same n_atoms candidates: the
+    atoms are packed into a 3 A ball at the grid centre, within 8 A of every
+    point.  The near kernel reads candidate indices 64 at a time and loads
+    each record one candidate ahead, so these counts put the last candidate
+    of a block, the first of the next and a lone candidate through the loop.
+    Maps against the oracle at every point."""
+    import dataclasses
+    print(f"[gpu] {n_atoms} candidates per brick", flush=True)
+    gpf = cases.materialize("small", str(tmp_path))
+    path = os.path.join(tmp_path, gpf)
+    lines = open(path).read().split("\n")
+    lines = ["npts 10 10 10" if ln.startswith("npts") else ln for ln in lines]
+    open(path, "w").write("\n".join(lines))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        st = ag.Job(gpf).prepare().state()
+    finally:
+        os.chdir(cwd)
+    # the reference needs a donor H among the atoms (its closest-H search)
+    donor = int(np.flatnonzero(st.hbond == 2)[0])
+    pick = np.concatenate([[donor], np.setdiff1d(np.arange(n_atoms + 1), [donor])[:n_atoms - 1]])
+    assert len(pick) == n_atoms and len(set(pick.tolist())) == n_atoms
+    c = st.coord[pick] - st.coord[pick].mean(axis=0)
+    scale = 3.0 / max(float(np.linalg.norm(c, axis=1).max()), 1e-9)
+    st = dataclasses.replace(
+        st, coord=np.ascontiguousarray(c * scale), charge=st.charge[pick].copy(), vol=st.vol[pick].copy(),
+        solpar=st.solpar[pick].copy(), type=st.type[pick].copy(), hbond=st.hbond[pick].copy(),
+        rexp=st.rexp[pick].copy(), rvector=st.rvector[pick].copy(), rvector2=st.rvector2[pick].copy(),
+        disorder=st.disorder[pick].copy(), _keep=[])
+    # bond vectors for the packed geometry (the oracle derives its own from it)
+    rexp, rv, rv2, dis = ag.bond_vectors(st, st.coord, st.type, st.hbond)
+    st = dataclasses.replace(st, rexp=rexp, rvector=rv, rvector2=rv2, disorder=dis, _keep=[])
+    # every point lies within 8 A of every atom, so each brick stages them all
+    half = 5 * st.spacing * np.sqrt(3.0)
+    assert half + 3.0 < 8.0
+    ref = ol.maps(ol.oracle_state(st))
+    got = ctx.maps_plain(st)
+    assert got.shape == ref.shape
+    for k in range(st.n_maps):
+        ok, mx, rms, bad = parity.within(got[k], ref[k])
+        assert ok, f"{n_atoms} atoms map {k}: max {mx} rms {rms} bad {bad}"

@@ -1277,6 +1277,20 @@ ag_status context_create(ag_ctx **out, int device) {
     return AG_OK;
 }
 
+void *host_alloc_pinned(ag_ctx *c, size_t bytes) {
+    HIP_CHECK(hipSetDevice(c->device));
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+void host_free_pinned(void *p) {
+    if (p) (void)hipHostFree(p);
+}
+
 void context_destroy(ag_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);

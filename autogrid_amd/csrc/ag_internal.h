@@ -194,6 +194,9 @@ namespace agdev {
 ag_status context_create(ag_ctx **out, int device);
 void context_destroy(ag_ctx *ctx);
 void context_timing(ag_ctx *ctx, float *ms);
+// Pinned host memory on ctx's device; null when the allocation fails.
+void *host_alloc_pinned(ag_ctx *ctx, size_t bytes);
+void host_free_pinned(void *p);
 ag_plan *plan_create(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                      const ag_tables *tab, int potential = AG_POTENTIAL_AD4);
 void plan_destroy(ag_plan *p);

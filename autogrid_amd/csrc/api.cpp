@@ -3,6 +3,7 @@
 // Every entry point catches the library's exceptions and turns them into an
 // ag_status plus a thread-local message; none of them calls exit().
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -597,6 +598,34 @@ ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path, co
 
 ag_status ag_run_gpf_flags(ag_ctx *ctx, const char *gpf_path, const char *glg_path, const char *program_name,
                            uint32_t flags) {
+    // With no context given, device 0 is opened on a thread of its own while
+    // the inputs are read and the tables built.  Its outcome is looked at
+    // only after every input error has had its chance to surface, so the
+    // errors reported are the same as when opening it afterwards.
+    ag_ctx *own = nullptr;
+    ag_status dev_status = AG_OK;
+    std::string dev_error;
+    struct Own {
+        ag_ctx *&c;
+        ~Own() { if (c) agdev::context_destroy(c); }
+    } own_guard{own};
+    std::thread opener;
+    struct Join {   // declared after own_guard: the thread ends before the context goes
+        std::thread &t;
+        ~Join() { if (t.joinable()) t.join(); }
+    } opener_join{opener};
+    if (!ctx)
+        opener = std::thread([&] {
+            try {
+                agdev::context_create(&own, 0);
+            } catch (const ag::Error &e) {
+                dev_status = e.status;
+                dev_error = e.what();
+            } catch (const std::exception &e) {
+                dev_status = AG_ERR_DEVICE;
+                dev_error = e.what();
+            }
+        });
     ag_job *j = nullptr;
     ag_status st = ag_job_open(gpf_path, glg_path, program_name, flags & (AG_JOB_BHTREE | AG_JOB_BINARY_MAPS), &j);
     if (st != AG_OK) return st;
@@ -615,17 +644,24 @@ ag_status ag_run_gpf_flags(ag_ctx *ctx, const char *gpf_path, const char *glg_pa
         ag_mapset ms;
         ag_tables tab;
         ag::job_views(job, &rec, &grid, &ms, &tab, nullptr);
-        std::vector<float> maps((size_t)job.num_maps * job.points());
-        std::vector<float> fmap(job.floating_grid ? job.points() : 0);
-        std::vector<int32_t> cmap(job.constriction_grid ? job.points() : 0);
-        // With no context given, the device is opened only now, after every
-        // input error has had its chance to surface.
-        ag_ctx *own = nullptr;
         if (!ctx) {
-            agdev::context_create(&own, 0);
+            opener.join();
+            if (dev_status != AG_OK) ag::fail(dev_status, dev_error);
             ctx = own;
         }
-        struct Own { ag_ctx *c; ~Own() { if (c) agdev::context_destroy(c); } } own_guard{own};
+        // The maps come back into pinned host memory (device-to-host copies
+        // at full link speed, no zero fill); pageable memory if that fails.
+        const size_t n_values = (size_t)job.num_maps * job.points();
+        struct HostMaps {
+            float *pinned = nullptr;
+            std::vector<float> pageable;
+            ~HostMaps() { agdev::host_free_pinned(pinned); }
+        } host;
+        host.pinned = (float *)agdev::host_alloc_pinned(ctx, sizeof(float) * std::max<size_t>(n_values, 1));
+        if (!host.pinned) host.pageable.resize(n_values);
+        float *const maps = host.pinned ? host.pinned : host.pageable.data();
+        std::vector<float> fmap(job.floating_grid ? job.points() : 0);
+        std::vector<int32_t> cmap(job.constriction_grid ? job.points() : 0);
         // The device computes the map set; a copy thread brings the maps to
         // the host one by one while writer threads format and write those
         // already there.
@@ -643,7 +679,7 @@ ag_status ag_run_gpf_flags(ag_ctx *ctx, const char *gpf_path, const char *glg_pa
                 cv.notify_all();
             };
             try {
-                agdev::maps_plain_streamed(ctx, &rec, &grid, &ms, &tab, maps.data(),
+                agdev::maps_plain_streamed(ctx, &rec, &grid, &ms, &tab, maps,
                                            job.floating_grid ? fmap.data() : nullptr,
                                            job.constriction_grid ? cmap.data() : nullptr, mark);
             } catch (const ag::Error &e) {
@@ -665,7 +701,7 @@ ag_status ag_run_gpf_flags(ag_ctx *ctx, const char *gpf_path, const char *glg_pa
         unsigned hw = std::thread::hardware_concurrency();
         ag::MapExtrema ext;
         try {
-            ext = ag::write_all_maps(job, maps.data(), fmap.data(), cmap.data(), (int)std::min(16u, hw ? hw : 1u),
+            ext = ag::write_all_maps(job, maps, fmap.data(), cmap.data(), (int)std::min(16u, hw ? hw : 1u),
                                      wait_ready);
         } catch (...) {
             copier.join();

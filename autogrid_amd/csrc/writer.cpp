@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <thread>
 
 #include "ag_internal.h"
@@ -44,8 +45,45 @@ static int format_fixed3(float v, char *out) {
     return (int)(p - out);
 }
 
+// "00".."99"
+static const char kDigitPairs[201] =
+    "0001020304050607080910111213141516171819202122232425262728293031323334353637383940414243444546474849"
+    "5051525354555657585960616263646566676869707172737475767778798081828384858687888990919293949596979899";
+
+// Writes n / 1000 with three decimals and a newline, for |n| < 16,384,000,
+// without data-dependent branches: the integer part is formed as five
+// zero-padded digits and the text is copied from its first significant one.
+// Copies 11 bytes after the sign, so `out` needs 12 bytes of room.
+static int format_milli(int n, char *out) {
+    const int neg = n < 0;
+    const unsigned k = neg ? 0u - (unsigned)n : (unsigned)n;
+    const unsigned ip = k / 1000u, fp = k - ip * 1000u;   // ip < 16384
+    const unsigned hi = ip / 100u, lo = ip - hi * 100u;   // hi < 164
+    const unsigned top = hi / 100u, mid = hi - top * 100u;
+    char t[16] = {};
+    t[0] = (char)('0' + top);
+    memcpy(t + 1, kDigitPairs + 2 * mid, 2);
+    memcpy(t + 3, kDigitPairs + 2 * lo, 2);
+    t[5] = '.';
+    t[6] = (char)('0' + fp / 100u);
+    memcpy(t + 7, kDigitPairs + 2 * (fp % 100u), 2);
+    t[9] = '\n';
+    const int nd = 1 + (ip >= 10u) + (ip >= 100u) + (ip >= 1000u) + (ip >= 10000u);
+    out[0] = '-';
+    memcpy(out + neg, t + 5 - nd, 16 - 5);
+    return neg + nd + 5;
+}
+
 // One map value: "0" when |e| < PRECISION, else "%.3f" of
 // (float)(floor(e*1000 + 0.5)/1000)  (main.cpp:3487-3490, HAVE_ROUND unset).
+//
+// With n = floor(e*1000 + 0.5) and |n| < 16,384,000 the text is n/1000 with
+// three decimals: v = (float)(n/1000) is below 16384 in magnitude, so it is
+// within half a float ulp (<= 2^-11) of n/1000 plus a double rounding of
+// order 2^-40, and 1000*v (exact in double) is within 0.489 of n, which
+// printf's "%.3f" therefore rounds back to n.  Only larger values take the
+// float conversion.  n is never -0 (x + 0.5 with x = -0.5 is +0).  The
+// floor of x there is its truncation, less one when that lies above x.
 int format_map_value(float e32, char *out) {
     const double e = (double)e32;
     if (std::fabs(e) < kPrecision) {
@@ -53,7 +91,13 @@ int format_map_value(float e32, char *out) {
         out[1] = '\n';
         return 2;
     }
-    const float v = (float)(std::floor(e * 1000.0 + 0.5) / 1000.0);
+    const double x = e * 1000.0 + 0.5;
+    if (x >= -16383999.0 && x < 16384000.0) {
+        int n = (int)x;
+        n -= (double)n > x;
+        return format_milli(n, out);
+    }
+    const float v = (float)(std::floor(x) / 1000.0);
     return format_fixed3(v, out);
 }
 
@@ -80,36 +124,42 @@ MapExtrema write_all_maps(Job &job, const float *maps, const float *fmap, const 
     if (job.constriction_grid && cmap && job.cmap_fp) tasks.push_back(nm + 1);
     std::atomic<int> next{0};
     std::atomic<bool> problem{false};
+    // values are formatted in blocks into a per-thread buffer with room for
+    // the longest text of a value (64 bytes), and each block is written out
+    constexpr int64_t kBlock = 65536;
+    constexpr int64_t kRoom = 64;
     auto worker = [&]() {
-        std::string text;
+        std::unique_ptr<char[]> buf(new char[(size_t)(kBlock * kRoom + kRoom)]);
         for (;;) {
             const int t = next.fetch_add(1);
             if (t >= (int)tasks.size()) return;
             const int j = tasks[t];
             if (wait_ready) wait_ready(j);
-            text.resize((size_t)P * 16 + 64);
-            char *o = &text[0];
-            FILE *f = nullptr;
-            if (j < nm) {
-                const float *v = maps + (size_t)j * P;
-                double lo = kBig, hi = -kBig;
-                for (int64_t i = 0; i < P; i++) {
-                    o += format_map_value(v[i], o);
-                    lo = std::min(lo, (double)v[i]);
-                    hi = std::max(hi, (double)v[i]);
+            FILE *f = j < nm ? job.maps[j].fp : (j == nm ? job.fmap_fp : job.cmap_fp);
+            double lo = kBig, hi = -kBig;
+            for (int64_t i0 = 0; i0 < P; i0 += kBlock) {
+                const int64_t i1 = std::min(P, i0 + kBlock);
+                char *o = buf.get();
+                if (j < nm) {
+                    const float *v = maps + (size_t)j * P;
+                    for (int64_t i = i0; i < i1; i++) {
+                        o += format_map_value(v[i], o);
+                        lo = std::min(lo, (double)v[i]);
+                        hi = std::max(hi, (double)v[i]);
+                    }
+                } else if (j == nm) {
+                    for (int64_t i = i0; i < i1; i++) o += format_rmin(fmap[i], o);
+                } else {
+                    for (int64_t i = i0; i < i1; i++) o += snprintf(o, 16, "%d\n", (int)cmap[i]);
                 }
+                const size_t n = (size_t)(o - buf.get());
+                if (f && fwrite(buf.get(), 1, n, f) != n) problem = true;
+            }
+            if (j < nm) {
                 emin[j] = lo;
                 emax[j] = hi;
-                f = job.maps[j].fp;
-            } else if (j == nm) {
-                for (int64_t i = 0; i < P; i++) o += format_rmin(fmap[i], o);
-                f = job.fmap_fp;
-            } else {
-                for (int64_t i = 0; i < P; i++) o += snprintf(o, 16, "%d\n", (int)cmap[i]);
-                f = job.cmap_fp;
             }
-            const size_t n = (size_t)(o - &text[0]);
-            if (f && (fwrite(text.data(), 1, n, f) != n || fflush(f) != 0)) problem = true;
+            if (f && fflush(f) != 0) problem = true;
             // binary side-output: the FP32 values themselves, counts as float
             if (job.binary_maps && job.make_files) {
                 const std::string fn = (j < nm ? job.maps[j].filename : (j == nm ? job.fmap_filename : job.cmap_filename))

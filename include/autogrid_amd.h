@@ -209,7 +209,8 @@ void ag_job_close(ag_job *job);
 /* Whole drop-in: open + prepare + compute on ctx + write every output file +
  * log "Successful Completion" (main.cpp:3565).  On failure the log ends with
  * the reference's "Unsuccessful Completion" line (main.cpp:173-176).  With
- * ctx NULL a context on device 0 is created after the inputs are read.  An
+ * ctx NULL a context on device 0 is created while the inputs are read; its
+ * failure is reported only after the inputs have been checked.  An
  * empty gpf_path reads the GPF from standard input (setflags.cpp:78). */
 ag_status ag_run_gpf(ag_ctx *ctx, const char *gpf_path, const char *glg_path,
                      const char *program_name);

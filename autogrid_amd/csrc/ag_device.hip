@@ -566,14 +566,29 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     }
 
     // ---- closest donor H (main.cpp:2985-3015), FP64, lowest index on ties ----
-    double best_r = 999999.0;
+    // The reference compares r = sqrt(d2), which is correctly rounded and so
+    // monotone in d2: squared distances more than 2^-49 apart (relative)
+    // order the same way, strictly.  Within that band the two rounded
+    // distances may tie, so there they are formed and compared as the
+    // reference does.  The start, 999999^2, is exact and its root is 999999.
+    double best_d2 = 999999.0 * 999999.0;
     int best_idx = 0x7fffffff, best_h = -1;
     for (int h = 0; h < nh; h++) {
-        double dx, dy, dz;
-        const double r = ref_dist(hpos[h][0], hpos[h][1], hpos[h][2], cx, cy, cz, dx, dy, dz);
+#pragma clang fp contract(off)
+        const double dx = hpos[h][0] - cx, dy = hpos[h][1] - cy, dz = hpos[h][2] - cz;
+        const double d2 = dx * dx + dy * dy + dz * dz;   // as ref_dist forms it
         const int ix_h = hidx[h];
-        if (r < best_r || (r == best_r && ix_h < best_idx)) {
-            best_r = r;
+        bool take;
+        if (d2 < best_d2 * (1.0 - 0x1p-49)) {
+            take = true;
+        } else if (d2 > best_d2 * (1.0 + 0x1p-49)) {
+            take = false;
+        } else {
+            const double r = sqrt(d2), rb = sqrt(best_d2);
+            take = r < rb || (r == rb && ix_h < best_idx);
+        }
+        if (take) {
+            best_d2 = d2;
             best_idx = ix_h;
             best_h = h;
         }
@@ -718,11 +733,13 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
             const int type = meta & 0xff, hb = (meta >> 8) & 0xff, rexp = (meta >> 16) & 0xff;
             const bool dis = (meta >> 24) & 1;
             if (p.disorder_h && dis && type == p.hydrogen_type) continue;
+            float racc = 1.f, rdon = 1.f, Hramp = 1.f;
+            // directional classes only: the unit vector to the point
+            if (hb == AG_HB_D1 || hb == AG_HB_A2 || hb == AG_HB_A1) {
             const float inv_r = __builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f));
             const float dn0 = dxf * inv_r, dn1 = dyf * inv_r, dn2 = dzf * inv_r;
             const float rv0 = unif(a.rv[0]), rv1 = unif(a.rv[1]), rv2 = unif(a.rv[2]);
             const int aidx = aindex;
-            float racc = 1.f, rdon = 1.f, Hramp = 1.f;
             if (hb == AG_HB_D1 || (hb == AG_HB_A2 && !dis)) {
                 // cos(theta) of the D1 / ordered-A2 branches; FP64 near 0
                 float cos_t = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
@@ -797,6 +814,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                     racc = rdon;
                 }
             }
+            }  // directional classes
             const int indx_n = min(bin, AG_NEINT - 1);
             const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
             const float S = sol_s[indx_r];

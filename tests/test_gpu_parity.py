@@ -276,3 +276,42 @@ def test_candidate_counts_around_index_blocks(ctx, n_atoms, tmp_path):
     for k in range(st.n_maps):
         ok, mx, rms, bad = parity.within(got[k], ref[k])
         assert ok, f"{n_atoms} atoms map {k}: max {mx} rms {rms} bad {bad}"
+
+
+def test_closest_donor_h_ties_take_the_lower_index(ctx, tmp_path):
+    """Two C-H donor groups whose H sit at mirror positions about the plane x = 0:
+    every grid point in that plane is exactly equidistant from both H, and
+    the reference keeps the lower atom index as the closest H.  The parents
+    point in different directions, so the choice changes the H-bonding maps
+    there (the other H gets the angular ramp).  Maps against the oracle."""
+    import dataclasses
+    gpf = cases.materialize("small", str(tmp_path))
+    path = os.path.join(tmp_path, gpf)
+    lines = open(path).read().split("\n")
+    lines = ["npts 10 10 10" if ln.startswith("npts") else ln for ln in lines]
+    open(path, "w").write("\n".join(lines))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        st = ag.Job(gpf).prepare().state()
+    finally:
+        os.chdir(cwd)
+    tn = st.t_types
+    donor = int(np.flatnonzero(st.hbond == 2)[0])
+    parent = int(np.flatnonzero(st.type == tn["t_carbon"])[0])
+    pick = np.array([parent, donor, parent, donor])
+    coord = np.array([[1.75, 0.2, 0.1], [0.75, 0.2, 0.1],       # C1 - H1, bond along -x
+                      [-0.75, 1.2, 0.1], [-0.75, 0.2, 0.1]])    # C2 - H2, bond along -y
+    st = dataclasses.replace(
+        st, coord=coord, charge=st.charge[pick].copy(), vol=st.vol[pick].copy(), solpar=st.solpar[pick].copy(),
+        type=st.type[pick].copy(), hbond=st.hbond[pick].copy(), rexp=st.rexp[pick].copy(),
+        rvector=st.rvector[pick].copy(), rvector2=st.rvector2[pick].copy(), disorder=st.disorder[pick].copy(),
+        _keep=[])
+    rexp, rv, rv2, dis = ag.bond_vectors(st, st.coord, st.type, st.hbond)
+    st = dataclasses.replace(st, rexp=rexp, rvector=rv, rvector2=rv2, disorder=dis, _keep=[])
+    assert np.allclose(rv[1], [-1, 0, 0]) and np.allclose(rv[3], [0, -1, 0])
+    ref = ol.maps(ol.oracle_state(st))
+    got = ctx.maps_plain(st)
+    for k in range(st.n_maps):
+        ok, mx, rms, bad = parity.within(got[k], ref[k])
+        assert ok, f"map {k}: max {mx} rms {rms} bad {bad}"

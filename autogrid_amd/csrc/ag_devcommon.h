@@ -21,7 +21,9 @@
                                         " at " #x);                                     \
     } while (0)
 
-constexpr int kMaxChunks = 16;                    // pipeline stages of a batch with host outputs
+constexpr int kMaxChunks = 16;                    // pipeline stages of a batch with host outputs (at most)
+constexpr int kBatchChunks = 10;                  // chunks a batch with host outputs is cut into
+constexpr int kBatchTaper = 3;                    // of which the last ones halve in size
 
 // Device blocks that destroyed plans released, kept by their context for
 // the next plan: a run of plans (one per receptor, or one per batch) then

@@ -1780,10 +1780,26 @@ struct HostOut {
 };
 
 // Contiguous conformation ranges of about equal work (points x atoms).
-static std::vector<std::pair<int, int>> plan_chunks(const ag_plan &p, int want) {
+// Chunks of consecutive conformations for a batch with host outputs: `want`
+// chunks whose work (points x atoms) follows the weights 1, ..., 1, 1/2,
+// 1/4, ..., 1/2^taper.  The last chunks shrink so that the copy left after
+// the final kernel is short; fewer, larger chunks before them keep the
+// number of chunk boundaries low.
+static std::vector<std::pair<int, int>> plan_chunks(const ag_plan &p, int want, int taper) {
     std::vector<std::pair<int, int>> out;
     const int n = p.nconf;
     want = std::max(1, std::min({want, n, kMaxChunks}));
+    taper = std::max(0, std::min(taper, want - 1));
+    std::vector<double> bound(want);   // cumulative work fraction at the end of each chunk
+    {
+        double sum = 0.0;
+        for (int k = 0; k < want; k++) sum += k < want - taper ? 1.0 : std::ldexp(1.0, -(k - (want - taper) + 1));
+        double acc = 0.0;
+        for (int k = 0; k < want; k++) {
+            acc += k < want - taper ? 1.0 : std::ldexp(1.0, -(k - (want - taper) + 1));
+            bound[k] = acc / sum;
+        }
+    }
     double total = 0.0;
     for (const ConfDev &cf : p.confs) total += (double)cf.P * (cf.n_atoms + 64);
     int c0 = 0;
@@ -1791,7 +1807,7 @@ static std::vector<std::pair<int, int>> plan_chunks(const ag_plan &p, int want) 
     for (int c = 0; c < n; c++) {
         acc += (double)p.confs[c].P * (p.confs[c].n_atoms + 64);
         const int k = (int)out.size();
-        if (c == n - 1 || (k < want - 1 && acc >= total * (k + 1) / want)) {
+        if (c == n - 1 || (k < want - 1 && acc >= total * bound[k])) {
             out.emplace_back(c0, c + 1);
             c0 = c + 1;
         }
@@ -2250,7 +2266,9 @@ void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids,
     hipStream_t st = c->stream;
     if (pinned) {
         const HostOut host{maps_out, fmap_out, cmap_out};
-        const auto chunks = plan_chunks(*p, kMaxChunks);
+        // 10 chunks, the last 3 halving: on the 64-conformation ensemble this
+        // measured 112-114 ms of GPU time against 116-117 ms for 16 equal chunks
+        const auto chunks = plan_chunks(*p, kBatchChunks, kBatchTaper);
         HIP_CHECK(hipEventRecord(c->ev[0], st));
         enqueue(p.get(), d_maps, d_fmap, d_cmap, false, false, chunks, &host);
         HIP_CHECK(hipEventRecord(c->ev[3], st));

@@ -122,3 +122,44 @@ def test_batch_distance_grids_and_vina(ens, ensemble):
     for k in range(vs[0].n_maps):
         assert parity.within(got[0][k, z], ref[k])[0]
 
+
+
+@pytest.fixture(scope="module")
+def ensemble12(tmp_path_factory):
+    d = tmp_path_factory.mktemp("ens12")
+    e = synth.make_ensemble(str(d), n_conf=12, seed=3002, base_atoms=2000, n_range=(400, 2000),
+                            npts_range=(16, 48))
+    return [synth.conformation_state(e, i) for i in range(len(e.confs))]
+
+
+@pytest.mark.parametrize("which", ["ensemble", "ensemble12"])
+def test_batch_pinned_outputs_pipeline(request, which):
+    """With pinned host outputs ag_maps_batch runs its chunked pipeline
+    (chunks alternating between two compute streams, each copied to the host
+    while later chunks compute).  Twelve conformations give the full chunk
+    pattern; six give fewer chunks than it asks for.  The maps must equal
+    the pageable-output path's bit for bit, conformation by conformation."""
+    import ctypes as C
+    states = request.getfixturevalue(which)
+    print(f"[gpu] pinned batch outputs, {len(states)} conformations", flush=True)
+    hip = _hip()
+    sizes = [st.n_maps * st.points for st in states]
+    buf = C.c_void_p()
+    assert hip.hipHostMalloc(C.byref(buf), C.c_size_t(4 * sum(sizes)), 0) == 0
+    try:
+        flat = np.ctypeslib.as_array(C.cast(buf, C.POINTER(C.c_float)), shape=(sum(sizes),))
+        flat[:] = np.nan
+        outs, o = [], 0
+        for st, n in zip(states, sizes):
+            outs.append(flat[o:o + n].reshape(st.n_maps, st.n1[2], st.n1[1], st.n1[0]))
+            o += n
+        tune = {"cell_size": 4.0, "cube": True}
+        with ag.Context(0) as ctx:
+            ref = ctx.maps_batch(states, tune=tune)
+            for rep in range(2):
+                ctx.maps_batch(states, tune=tune, outs=outs)
+                for i, (g, r) in enumerate(zip(outs, ref)):
+                    assert np.array_equal(g.view(np.uint32), r.view(np.uint32)), f"rep {rep} conformation {i}"
+                flat[:] = np.nan
+    finally:
+        hip.hipHostFree(buf)

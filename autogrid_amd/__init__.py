@@ -365,24 +365,35 @@ def _tune_struct(tune: dict | None):
     return _abi.Tune(float(tune.get("cell_size", 4.0)), int(bool(tune.get("cube", False))), 0, 0.0)
 
 
+_RECEPTOR_FIELDS = (("coord", np.float64), ("charge", np.float64), ("vol", np.float64), ("solpar", np.float64),
+                    ("type", np.int32), ("hbond", np.int32), ("rexp", np.int32), ("rvector", np.float64),
+                    ("rvector2", np.float64), ("disorder", np.uint8))
+
+
 def _batch_structs(states: list):
     """ctypes arrays of Receptor and Grid for a list of states, the first
-    state's MapSet and Tables, and the list that keeps their arrays alive."""
+    state's MapSet and Tables, and the list that keeps their arrays alive.
+    The receptors are filled with plain addresses (ReceptorAddr, the same
+    layout), one integer per field instead of a typed-pointer cast."""
     n = len(states)
-    recs = (_abi.Receptor * n)()
+    recs = (_abi.ReceptorAddr * n)()
     grids = (_abi.Grid * n)()
-    keep = []
-    ms = tab = None
+    _, _, ms, tab, _ = states[0].structs()
+    keep = [recs, list(states[0]._keep)]
     for i, st in enumerate(states):
-        if i == 0:
-            r, g, ms, tab, _ = st.structs()
-            keep.append(list(st._keep))
-        else:
-            r, g, arrs = st.receptor_grid()
-            keep.append(arrs)
-        recs[i] = r
-        grids[i] = g
-    return recs, grids, ms, tab, keep
+        r = recs[i]
+        r.n_atoms = st.n_atoms
+        arrs = []
+        for name, dt in _RECEPTOR_FIELDS:
+            a = np.ascontiguousarray(getattr(st, name), dt)
+            arrs.append(a)
+            setattr(r, name, a.ctypes.data)
+        keep.append(arrs)
+        g = grids[i]
+        g.n1[:] = st.n1
+        g.ne[:] = st.ne
+        g.spacing = float(st.spacing)
+    return C.cast(recs, C.POINTER(_abi.Receptor)), grids, ms, tab, keep
 
 
 class Plan:
@@ -616,36 +627,50 @@ def bond_vectors_vjp(st: PreparedState, coord, g_rv, g_rv2) -> np.ndarray:
     return out
 
 
+def _with(st: PreparedState, **fields) -> PreparedState:
+    """A copy of a state with some fields replaced (dataclasses.replace
+    without re-running __init__); its keep list starts empty."""
+    new = object.__new__(type(st))
+    new.__dict__.update(st.__dict__)
+    new.__dict__.update(fields)
+    new._keep = []
+    return new
+
+
 def bond_vectors_batch(states: list, n_threads: int = 0) -> list:
     """Copies of the states with bond vectors from ag_bond_vectors_batch
-    (host threads); the first state's setup (types, options) applies to all."""
-    import dataclasses
+    (host threads); the first state's setup (types, options) applies to all.
+    The outputs of all states share one array per field; each state holds
+    its slice."""
     n = len(states)
     _, _, _, _, setup = states[0].structs()
-    keep0 = list(states[0]._keep)
-    recs = (_abi.Receptor * n)()
-    outs = (_abi.BondOut * n)()
-    new, keep = [], [keep0]
+    counts = np.array([st.n_atoms for st in states], np.int64)
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    total = int(offs[-1])
+    rexp_all = np.zeros(total, np.int32)
+    rv_all = np.zeros((total, 3))
+    rv2_all = np.zeros((total, 3))
+    dis_all = np.zeros(total, np.uint8)
+    b_rexp, b_rv, b_rv2, b_dis = (a.ctypes.data for a in (rexp_all, rv_all, rv2_all, dis_all))
+    recs = (_abi.ReceptorAddr * n)()
+    outs = (_abi.BondOutAddr * n)()
+    keep = [list(states[0]._keep)]
+    new = []
     for i, st in enumerate(states):
-        na = st.n_atoms
+        o0, o1 = int(offs[i]), int(offs[i + 1])
         coord = np.ascontiguousarray(st.coord, np.float64)
         typ = np.ascontiguousarray(st.type, np.int32)
         hb = np.ascontiguousarray(st.hbond, np.int32)
-        rexp = np.zeros(na, np.int32)
-        rv = np.zeros((na, 3))
-        rv2 = np.zeros((na, 3))
-        dis = np.zeros(na, np.uint8)
         keep.append((coord, typ, hb))
-        r = _abi.Receptor()
-        r.n_atoms = na
-        r.coord = _abi._ptr(coord, C.c_double)
-        r.type = _abi._ptr(typ, C.c_int32)
-        r.hbond = _abi._ptr(hb, C.c_int32)
-        recs[i] = r
-        outs[i] = _abi.BondOut(_abi._ptr(rexp, C.c_int32), _abi._ptr(rv, C.c_double), _abi._ptr(rv2, C.c_double),
-                               _abi._ptr(dis, C.c_uint8))
-        new.append(dataclasses.replace(st, rexp=rexp, rvector=rv, rvector2=rv2, disorder=dis, _keep=[]))
-    check(lib().ag_bond_vectors_batch(C.byref(setup), n, recs, outs, int(n_threads)))
+        r = recs[i]
+        r.n_atoms = st.n_atoms
+        r.coord, r.type, r.hbond = coord.ctypes.data, typ.ctypes.data, hb.ctypes.data
+        ob = outs[i]
+        ob.rexp, ob.rvector, ob.rvector2, ob.disorder = b_rexp + 4 * o0, b_rv + 24 * o0, b_rv2 + 24 * o0, b_dis + o0
+        new.append(_with(st, rexp=rexp_all[o0:o1], rvector=rv_all[o0:o1], rvector2=rv2_all[o0:o1],
+                         disorder=dis_all[o0:o1]))
+    check(lib().ag_bond_vectors_batch(C.byref(setup), n, C.cast(recs, C.POINTER(_abi.Receptor)),
+                                      C.cast(outs, C.POINTER(_abi.BondOut)), int(n_threads)))
     return new
 
 

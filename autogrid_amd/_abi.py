@@ -45,6 +45,15 @@ class Receptor(C.Structure):
                 ("disorder", C.POINTER(C.c_uint8))]
 
 
+class ReceptorAddr(C.Structure):
+    """Receptor with its pointers as plain addresses (same layout), filled
+    from ndarray.ctypes.data for many conformations at once."""
+    _fields_ = [("n_atoms", C.c_int32)] + [(name, C.c_void_p) for name, _ in Receptor._fields_[1:]]
+
+
+assert C.sizeof(ReceptorAddr) == C.sizeof(Receptor)
+
+
 class Grid(C.Structure):
     _fields_ = [("n1", C.c_int32 * 3), ("ne", C.c_int32 * 3), ("spacing", C.c_double)]
 
@@ -87,6 +96,14 @@ class Weights(C.Structure):
 class BondOut(C.Structure):
     _fields_ = [("rexp", C.POINTER(C.c_int32)), ("rvector", C.POINTER(C.c_double)),
                 ("rvector2", C.POINTER(C.c_double)), ("disorder", C.POINTER(C.c_uint8))]
+
+
+class BondOutAddr(C.Structure):
+    """BondOut with its pointers as plain addresses (same layout)."""
+    _fields_ = [(name, C.c_void_p) for name, _ in BondOut._fields_]
+
+
+assert C.sizeof(BondOutAddr) == C.sizeof(BondOut)
 
 
 class Setup(C.Structure):

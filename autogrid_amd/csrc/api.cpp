@@ -318,13 +318,18 @@ ag_status ag_bond_vectors_batch(const ag_setup *setup, int n, const ag_receptor 
     return guarded([&] {
         int nt = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
         nt = std::max(1, std::min(nt, n));
+        // largest conformations first, so the threads finish together
+        std::vector<int> order(n);
+        for (int i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return recs[a].n_atoms > recs[b].n_atoms; });
         std::atomic<int> next{0};
         std::vector<std::string> errors(nt);
         std::vector<ag_status> codes(nt, AG_OK);
         auto work = [&](int t) {
             for (;;) {
-                const int i = next.fetch_add(1);
-                if (i >= n) return;
+                const int k = next.fetch_add(1);
+                if (k >= n) return;
+                const int i = order[k];
                 try {
                     ag::build_bond_vectors(*setup, recs[i].n_atoms, recs[i].coord, recs[i].type, recs[i].hbond,
                                            outs[i].rexp, outs[i].rvector, outs[i].rvector2, outs[i].disorder,

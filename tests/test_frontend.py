@@ -112,3 +112,24 @@ def test_npts_rounds_down_to_even(tmp_path):
     finally:
         os.chdir(cwd)
     assert st.n1 == (5, 7, 3) and st.ne == (2, 3, 1)
+
+
+def test_cli_reports_the_device_failure_after_valid_inputs(tmp_path):
+    """The drop-in opens the device on a thread while it reads the inputs.
+    On a host without a GPU and with valid inputs, the device's failure is
+    what ends the run: exit 1, the HIP error and 'Unsuccessful Completion'
+    in the log, after the headers of every map were written."""
+    import ctypes as C
+    from autogrid_amd import _abi
+    h = C.c_void_p()
+    if _abi.lib().ag_context_create(C.byref(h), 0) == 0:
+        _abi.lib().ag_context_destroy(h)
+        pytest.skip("a GPU is present")
+    gpf = cases.materialize("small", str(tmp_path))
+    p = subprocess.run([ag.CLI_PATH, "-p", gpf, "-l", "out.glg"], cwd=tmp_path, capture_output=True, timeout=60)
+    assert p.returncode == 1
+    log = open(tmp_path / "out.glg").read()
+    tail = "".join(log.splitlines(True)[-10:])
+    assert "Unsuccessful Completion" in tail and "HIP" in tail
+    for m in cases.map_names(os.path.join(tmp_path, gpf)):
+        assert len(open(tmp_path / m).read().split("\n")) >= 6

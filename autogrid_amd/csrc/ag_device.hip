@@ -264,28 +264,44 @@ __global__ void k_zero_int(int *p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Exclusive scan of count[0..n) into start[0..n], one workgroup of 1024.
+// Clears the cell counts and cursors [0, n), the status word and the nq
+// work counters of one map set, in one launch.
+__global__ void k_zero_cells(int *count, int *cursor, int n, int *status, int *queue, int nq) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        count[i] = 0;
+        cursor[i] = 0;
+    }
+    if (i < nq) queue[i] = 0;
+    if (i == 0) *status = 0;
+}
+
+// Exclusive scan of count[0..n) into start[0..n], one workgroup of 1024:
+// each thread sums its own slice, the slice sums are scanned within each
+// wave by shuffles and across the 16 waves through LDS, and each thread
+// then writes its slice's prefixes.
 __global__ void __launch_bounds__(1024) k_scan(const int *count, int *start, int n) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
+    __shared__ int wave_sum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n + 1023) / 1024;
-    const int b = t * per, e = min(n, b + per);
+    const int b = min(n, t * per), e = min(n, b + per);
     int s = 0;
     for (int i = b; i < e; i++) s += count[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    int x = s;   // inclusive scan over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
     }
-    int run = part[t] - s;
+    if (lane == 63) wave_sum[w] = x;
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < w; k++) before += wave_sum[k];
+    int run = before + x - s;
     for (int i = b; i < e; i++) {
         start[i] = run;
         run += count[i];
     }
-    if (t == 1023) start[n] = part[1023];
+    if (t == 1023) start[n] = before + x;
 }
 
 __global__ void k_cell_scatter(int n, const int *atom_cell, const int *start, int *cursor, int *slot_atom) {
@@ -1835,10 +1851,8 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
     if (nchunks < 1 || nchunks > kMaxChunks) ag::fail(AG_ERR_BAD_INPUT, "bad chunking of the batch");
     const unsigned cgrid = (unsigned)((p->total_cells + tpb - 1) / tpb);
     if (events) HIP_CHECK(hipEventRecord(c->ev[0], st));
-    k_zero_int<<<cgrid, tpb, 0, st>>>(d_count, p->total_cells);
-    k_zero_int<<<cgrid, tpb, 0, st>>>(d_cursor, p->total_cells);
-    k_zero_int<<<1, 64, 0, st>>>(d_status, 1);
-    k_zero_int<<<1, 64, 0, st>>>(d_queue, kMaxChunks);
+    k_zero_cells<<<std::max(cgrid, (unsigned)((kMaxChunks + tpb - 1) / tpb)), tpb, 0, st>>>(
+        d_count, d_cursor, p->total_cells, d_status, d_queue, kMaxChunks);
     if (N > 0)
         k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>((const ConfDev *)p->confs_d.ptr,
                                                           p->nconf > 1 ? (const int *)p->atom_conf_d.ptr : nullptr, N,

@@ -143,6 +143,7 @@ struct ag_ctx {
     std::shared_ptr<DevPool> pool = std::make_shared<DevPool>();   // blocks of destroyed plans
     DevBuf maps;                       // output staging of ag_maps_plain
     HostStage stage;                   // receptor arrays of a plan on their way up
+    HostStage status_word;             // pinned target of a plan's status flag after each map set
     hipEvent_t ev[4] = {};
     float last_ms[4] = {0, 0, 0, 0};   // cell list, near, far, total of the last execute
     int n_cu = 0;

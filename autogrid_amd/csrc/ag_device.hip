@@ -1977,10 +1977,11 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
 }
 
 static void check_status(ag_plan *p) {
-    int status = 0;
-    HIP_CHECK(hipMemcpyAsync(&status, p->status.ptr, sizeof(int), hipMemcpyDeviceToHost, p->ctx->stream));
+    // into pinned memory: a direct copy, not a staged one
+    int *status = (int *)p->ctx->status_word.get(sizeof(int));
+    HIP_CHECK(hipMemcpyAsync(status, p->status.ptr, sizeof(int), hipMemcpyDeviceToHost, p->ctx->stream));
     HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
-    if (status & 1) ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
+    if (*status & 1) ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
 }
 
 // Autotuner: times the near-field pass of every candidate layout on a

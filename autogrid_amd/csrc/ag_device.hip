@@ -652,261 +652,262 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
         // which also brings both of its cache lines into the scalar cache.
         const AG_CONST NearAtom *catoms = as_const(p.atoms);
         for (int j0 = 0; j0 < ncand; j0 += 64) {
-        const int vcand = cand[min(j0 + wlane, ncand - 1)];
-        const int jn = min(64, ncand - j0);
-        int idx = __builtin_amdgcn_readlane(vcand, 0);
-        double nx = catoms[idx].x, ny = catoms[idx].y, nz = catoms[idx].z;
-        int nmeta = catoms[idx].meta, nindex = catoms[idx].index;
-        for (int jj = 0; jj < jn; jj++) {
-            const AG_CONST NearAtom &a = catoms[idx];
-            const double ax = nx, ay = ny, az = nz;
-            const int ameta = nmeta, aindex = nindex;
-            const unsigned hw = (unsigned)__builtin_amdgcn_readlane(hcw, (ameta >> 8) & 63);
-            idx = __builtin_amdgcn_readlane(vcand, min(jj + 1, jn - 1));
-            // the next loads may issue only once this record has arrived:
-            // scalar loads complete out of order, so the wait for this
-            // record would otherwise also wait for the next one
-            asm volatile("" : "+s"(idx) : "s"(ameta), "s"(aindex), "s"(ax), "s"(ay), "s"(az));
-            nx = catoms[idx].x;
-            ny = catoms[idx].y;
-            nz = catoms[idx].z;
-            nmeta = catoms[idx].meta;
-            nindex = catoms[idx].index;
-            // geometry: FP64 difference as the reference forms it, FP32 after
-            const double dx64 = ax - cx, dy64 = ay - cy, dz64 = az - cz;
-            const float dxf = (float)dx64, dyf = (float)dy64, dzf = (float)dz64;
-            const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
-            if (r2 > 64.33f) continue;  // beyond 8.02 A: neither cutoff pass nor near e term
-            // hardware rsq: 128 r is good to ~3e-4 bins here, inside the guard band
-            const float r2c = fmaxf(r2, 1.0e-12f);
-            float r = r2c * __builtin_amdgcn_rsqf(r2c);
-            const float g = r * 128.f;
-            int bin;
-            bool cut;
-            if (fabsf(g - rintf(g)) < kGuardBins) {
-                double rdx, rdy, rdz;
-                const double r64 = ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
-                bin = (int)(r64 * 128.0);
-                cut = r64 > 8.0;
-                r = (float)r64;
-            } else {
-                bin = (int)g;
-                cut = r > 8.0f;
-            }
-            // the tree build's radius queries (bhtree.cpp:341-362): FP32 point
-            // and atom, each component within the radius, then the rounded
-            // squared distance against the rounded squared radius
-            bool in8 = false;
-            float rB = 0.f;
-            if (bh) {
-                // the tree build's query point: the grid point as float (main.cpp:2860)
-                const float e0 = (float)cx - (float)ax, e1 = (float)cy - (float)ay, e2 = (float)cz - (float)az;
-                const float a0 = fabsf(e0), a1 = fabsf(e1), a2 = fabsf(e2);
-                const float D = (e0 * e0 + e1 * e1) + e2 * e2;
-                in8 = a0 <= 8.f && a1 <= 8.f && a2 <= 8.f && D <= 64.f;
-                if (a0 <= 1.5f && a1 <= 1.5f && a2 <= 1.5f && D <= 2.25f) collide = true;
-                if (want_cmap) ccount += a0 <= ccf && a1 <= ccf && a2 <= ccf && D <= ccf2;
-                rB = sqrtf(D);   // correctly rounded, as the tree's sqrt of a float
-            }
-            // constriction grid: atoms within the cutoff (main.cpp:2969-2973)
-            if (want_cmap && !bh) {
-                if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
+            const int vcand = cand[min(j0 + wlane, ncand - 1)];
+            const int jn = min(64, ncand - j0);
+            int idx = __builtin_amdgcn_readlane(vcand, 0);
+            double nx = catoms[idx].x, ny = catoms[idx].y, nz = catoms[idx].z;
+            int nmeta = catoms[idx].meta, nindex = catoms[idx].index;
+            for (int jj = 0; jj < jn; jj++) {
+                const AG_CONST NearAtom &a = catoms[idx];
+                const double ax = nx, ay = ny, az = nz;
+                const int ameta = nmeta, aindex = nindex;
+                const unsigned hw = (unsigned)__builtin_amdgcn_readlane(hcw, (ameta >> 8) & 63);
+                idx = __builtin_amdgcn_readlane(vcand, min(jj + 1, jn - 1));
+                // the next loads may issue only once this record has arrived:
+                // scalar loads complete out of order, so the wait for this
+                // record would otherwise also wait for the next one
+                asm volatile("" : "+s"(idx) : "s"(ameta), "s"(aindex), "s"(ax), "s"(ay), "s"(az));
+                nx = catoms[idx].x;
+                ny = catoms[idx].y;
+                nz = catoms[idx].z;
+                nmeta = catoms[idx].meta;
+                nindex = catoms[idx].index;
+                // geometry: FP64 difference as the reference forms it, FP32 after
+                const double dx64 = ax - cx, dy64 = ay - cy, dz64 = az - cz;
+                const float dxf = (float)dx64, dyf = (float)dy64, dzf = (float)dz64;
+                const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
+                if (r2 > 64.33f) continue;  // beyond 8.02 A: neither cutoff pass nor near e term
+                // hardware rsq: 128 r is good to ~3e-4 bins here, inside the guard band
+                const float r2c = fmaxf(r2, 1.0e-12f);
+                float r = r2c * __builtin_amdgcn_rsqf(r2c);
+                const float g = r * 128.f;
+                int bin;
+                bool cut;
+                if (fabsf(g - rintf(g)) < kGuardBins) {
                     double rdx, rdy, rdz;
-                    ccount += ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz) <= p.ccut;
+                    const double r64 = ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
+                    bin = (int)(r64 * 128.0);
+                    cut = r64 > 8.0;
+                    r = (float)r64;
                 } else {
-                    ccount += r <= (float)p.ccut;
+                    bin = (int)g;
+                    cut = r > 8.0f;
                 }
-            }
-            // electrostatics of pairs the far kernel leaves out
-            if (want_e) {
-                bool near_e;
-                if (g < 1022.f) near_e = true;
-                else if (g > 1027.f) near_e = false;
-                else {
-                    // the far kernel's point coordinates: run centre, offset in the run, y, z
-                    int qx, qy, qz;
-                    point_index(qx, qy, qz);
-                    const int kxE = qx % kFarK;
-                    near_e = far_bin4(a.xE, a.yE, a.zE, far_cx(qx - kxE, nex, s128), far_dk(kxE, s128),
-                                      (float)(qy - ney) * s128, (float)(qz - nez) * s128, far_bias(s128)) <
-                             4u * kFarBinMin;
+                // the tree build's radius queries (bhtree.cpp:341-362): FP32 point
+                // and atom, each component within the radius, then the rounded
+                // squared distance against the rounded squared radius
+                bool in8 = false;
+                float rB = 0.f;
+                if (bh) {
+                    // the tree build's query point: the grid point as float (main.cpp:2860)
+                    const float e0 = (float)cx - (float)ax, e1 = (float)cy - (float)ay, e2 = (float)cz - (float)az;
+                    const float a0 = fabsf(e0), a1 = fabsf(e1), a2 = fabsf(e2);
+                    const float D = (e0 * e0 + e1 * e1) + e2 * e2;
+                    in8 = a0 <= 8.f && a1 <= 8.f && a2 <= 8.f && D <= 64.f;
+                    if (a0 <= 1.5f && a1 <= 1.5f && a2 <= 1.5f && D <= 2.25f) collide = true;
+                    if (want_cmap) ccount += a0 <= ccf && a1 <= ccf && a2 <= ccf && D <= ccf2;
+                    rB = sqrtf(D);   // correctly rounded, as the tree's sqrt of a float
                 }
-                if (near_e) {
-                    // 1/max(r, 0.5) with r clamped at APPROX_ZERO
-                    const float inv_rmax = fminf(__builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f)), 2.f);
-                    e_near += unif(a.qe) * inv_rmax * reps_s[min(bin, kNearRepsLen - 1)];
-                }
-            }
-            // Loop 2 of the tree build takes the atoms its 8 A query returns,
-            // at their FP32 distance (main.cpp:3019-3028)
-            if (bh) {
-                cut = !in8;
-                r = rB;
-                bin = (int)((double)rB * 128.0);
-            }
-            if (cut || !p.has_affinity) continue;
-            const int meta = ameta;
-            const int type = meta & 0xff, hb = (meta >> 8) & 0xff, rexp = (meta >> 16) & 0xff;
-            const bool dis = (meta >> 24) & 1;
-            if (p.disorder_h && dis && type == p.hydrogen_type) continue;
-            float racc = 1.f, rdon = 1.f, Hramp = 1.f;
-            // directional classes only: the unit vector to the point
-            if (hb == AG_HB_D1 || hb == AG_HB_A2 || hb == AG_HB_A1) {
-            const float inv_r = __builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f));
-            const float dn0 = dxf * inv_r, dn1 = dyf * inv_r, dn2 = dzf * inv_r;
-            const float rv0 = unif(a.rv[0]), rv1 = unif(a.rv[1]), rv2 = unif(a.rv[2]);
-            const int aidx = aindex;
-            if (hb == AG_HB_D1 || (hb == AG_HB_A2 && !dis)) {
-                // cos(theta) of the D1 / ordered-A2 branches; FP64 near 0
-                float cos_t = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
-                bool pos;
-                if (fabsf(cos_t) < 1.0e-5f) {
-                    double rdx, rdy, rdz;
-                    double r64 = ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
-                    if (bh) r64 = (double)rB;   // the tree build's r (main.cpp:3026)
-                    if (r64 < 1.0e-6) r64 = 1.0e-6;
-                    const double ir = 1. / r64;
-                    const double *rv64 = p.rvec64 + 3 * (size_t)aidx;
-                    double c = 0.;
-                    c -= rdx * ir * rv64[0];
-                    c -= rdy * ir * rv64[1];
-                    c -= rdz * ir * rv64[2];
-                    pos = hb == AG_HB_D1 ? c > 0. : c >= 0.;
-                    cos_t = (float)c;
-                } else {
-                    pos = hb == AG_HB_D1 ? cos_t > 0.f : cos_t >= 0.f;
-                }
-                if (hb == AG_HB_D1) {
-                    if (!pos) {
-                        racc = 0.f;
+                // constriction grid: atoms within the cutoff (main.cpp:2969-2973)
+                if (want_cmap && !bh) {
+                    if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
+                        double rdx, rdy, rdz;
+                        ccount += ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz) <= p.ccut;
                     } else {
-                        const float c2 = cos_t * cos_t;
-                        racc = rexp == 2 ? c2 : (rexp == 4 ? c2 * c2 : cos_t);
-                        if (aidx != best_idx) {
-                            float ct = hrv[0] * rv0 + hrv[1] * rv1 + hrv[2] * rv2;
-                            ct = fminf(fmaxf(ct, -1.f), 1.f);
-                            Hramp = 0.5f - 0.5f * cosf(acosf(ct) * (120.f / 90.f));
+                        ccount += r <= (float)p.ccut;
+                    }
+                }
+                // electrostatics of pairs the far kernel leaves out
+                if (want_e) {
+                    bool near_e;
+                    if (g < 1022.f) near_e = true;
+                    else if (g > 1027.f) near_e = false;
+                    else {
+                        // the far kernel's point coordinates: run centre, offset in the run, y, z
+                        int qx, qy, qz;
+                        point_index(qx, qy, qz);
+                        const int kxE = qx % kFarK;
+                        near_e = far_bin4(a.xE, a.yE, a.zE, far_cx(qx - kxE, nex, s128), far_dk(kxE, s128),
+                                          (float)(qy - ney) * s128, (float)(qz - nez) * s128, far_bias(s128)) <
+                                 4u * kFarBinMin;
+                    }
+                    if (near_e) {
+                        // 1/max(r, 0.5) with r clamped at APPROX_ZERO
+                        const float inv_rmax = fminf(__builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f)), 2.f);
+                        e_near += unif(a.qe) * inv_rmax * reps_s[min(bin, kNearRepsLen - 1)];
+                    }
+                }
+                // Loop 2 of the tree build takes the atoms its 8 A query returns,
+                // at their FP32 distance (main.cpp:3019-3028)
+                if (bh) {
+                    cut = !in8;
+                    r = rB;
+                    bin = (int)((double)rB * 128.0);
+                }
+                if (cut || !p.has_affinity) continue;
+                const int meta = ameta;
+                const int type = meta & 0xff, hb = (meta >> 8) & 0xff, rexp = (meta >> 16) & 0xff;
+                const bool dis = (meta >> 24) & 1;
+                if (p.disorder_h && dis && type == p.hydrogen_type) continue;
+                float racc = 1.f, rdon = 1.f, Hramp = 1.f;
+                // directional classes only: the unit vector to the point
+                if (hb == AG_HB_D1 || hb == AG_HB_A2 || hb == AG_HB_A1) {
+                    const float inv_r = __builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f));
+                    const float dn0 = dxf * inv_r, dn1 = dyf * inv_r, dn2 = dzf * inv_r;
+                    const float rv0 = unif(a.rv[0]), rv1 = unif(a.rv[1]), rv2 = unif(a.rv[2]);
+                    const int aidx = aindex;
+                    if (hb == AG_HB_D1 || (hb == AG_HB_A2 && !dis)) {
+                        // cos(theta) of the D1 / ordered-A2 branches; FP64 near 0
+                        float cos_t = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
+                        bool pos;
+                        if (fabsf(cos_t) < 1.0e-5f) {
+                            double rdx, rdy, rdz;
+                            double r64 = ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
+                            if (bh) r64 = (double)rB;   // the tree build's r (main.cpp:3026)
+                            if (r64 < 1.0e-6) r64 = 1.0e-6;
+                            const double ir = 1. / r64;
+                            const double *rv64 = p.rvec64 + 3 * (size_t)aidx;
+                            double c = 0.;
+                            c -= rdx * ir * rv64[0];
+                            c -= rdy * ir * rv64[1];
+                            c -= rdz * ir * rv64[2];
+                            pos = hb == AG_HB_D1 ? c > 0. : c >= 0.;
+                            cos_t = (float)c;
+                        } else {
+                            pos = hb == AG_HB_D1 ? cos_t > 0.f : cos_t >= 0.f;
+                        }
+                        if (hb == AG_HB_D1) {
+                            if (!pos) {
+                                racc = 0.f;
+                            } else {
+                                const float c2 = cos_t * cos_t;
+                                racc = rexp == 2 ? c2 : (rexp == 4 ? c2 * c2 : cos_t);
+                                if (aidx != best_idx) {
+                                    float ct = hrv[0] * rv0 + hrv[1] * rv1 + hrv[2] * rv2;
+                                    ct = fminf(fmaxf(ct, -1.f), 1.f);
+                                    Hramp = 0.5f - 0.5f * cosf(acosf(ct) * (120.f / 90.f));
+                                }
+                            }
+                        } else {
+                            // Goodford lone-pair terms.  With x = dn.rv2 and
+                            // y = (dn x rv2).rv / |dn x rv2|:
+                            //   t0 = pi/2 - acos(x)         -> cos(t0) = sqrt(1 - x^2)
+                            //   ti = |acos(y) - pi/2|       -> sin(2 ti) = 2|y| sqrt(1 - y^2)
+                            const float rw0 = unif(a.rv2[0]), rw1 = unif(a.rv2[1]), rw2 = unif(a.rv2[2]);
+                            float t0 = dn0 * rw0 + dn1 * rw1 + dn2 * rw2;
+                            t0 = fminf(fmaxf(t0, -1.f), 1.f);
+                            const float cos_t0 = __builtin_amdgcn_sqrtf(fmaxf(1.f - t0 * t0, 0.f));
+                            rdon = 0.f;
+                            if (pos) {
+                                const float cr0 = dn1 * rw2 - dn2 * rw1;
+                                const float cr1 = dn2 * rw0 - dn0 * rw2;
+                                const float cr2 = dn0 * rw1 - dn1 * rw0;
+                                float rd2 = cr0 * cr0 + cr1 * cr1 + cr2 * cr2;
+                                if (rd2 < 1.0e-6f) rd2 = 1.0e-6f;
+                                const float inv_rd = __builtin_amdgcn_rsqf(rd2);
+                                float ti = (cr0 * inv_rd) * rv0 + (cr1 * inv_rd) * rv1 + (cr2 * inv_rd) * rv2;
+                                ti = fminf(fmaxf(ti, -1.f), 1.f);
+                                const float sin2ti =
+                                    2.f * fabsf(ti) * __builtin_amdgcn_sqrtf(fmaxf(1.f - ti * ti, 0.f));
+                                rdon = (0.9f + 0.1f * sin2ti) * cos_t0;
+                            } else if (cos_t >= -0.34202f) {
+                                const float u = 0.116978f - cos_t * cos_t;
+                                rdon = 562.25f * (u * u * u) * cos_t0;
+                            }
+                        }
+                    } else if (hb == AG_HB_A1) {
+                        const float cos_t = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
+                        rdon = cos_t <= 0.f ? 0.f : cos_t * cos_t;
+                    } else if (hb == AG_HB_A2 && dis) {
+                        // cos(theta - 1.24791) = c cos(1.24791) + sqrt(1 - c^2) sin(1.24791);
+                        // theta <= 1.24791 + pi/2  <=>  c >= -sin(1.24791)
+                        float c = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
+                        c = fminf(fmaxf(c, -1.f), 1.f);
+                        racc = 0.f;
+                        rdon = 0.f;
+                        if (c >= -sdis) {
+                            const float cc = c * cdis + __builtin_amdgcn_sqrtf(fmaxf(1.f - c * c, 0.f)) * sdis;
+                            rdon = (cc * cc) * (cc * cc);
+                            racc = rdon;
                         }
                     }
-                } else {
-                    // Goodford lone-pair terms.  With x = dn.rv2 and
-                    // y = (dn x rv2).rv / |dn x rv2|:
-                    //   t0 = pi/2 - acos(x)         -> cos(t0) = sqrt(1 - x^2)
-                    //   ti = |acos(y) - pi/2|       -> sin(2 ti) = 2|y| sqrt(1 - y^2)
-                    const float rw0 = unif(a.rv2[0]), rw1 = unif(a.rv2[1]), rw2 = unif(a.rv2[2]);
-                    float t0 = dn0 * rw0 + dn1 * rw1 + dn2 * rw2;
-                    t0 = fminf(fmaxf(t0, -1.f), 1.f);
-                    const float cos_t0 = __builtin_amdgcn_sqrtf(fmaxf(1.f - t0 * t0, 0.f));
-                    rdon = 0.f;
-                    if (pos) {
-                        const float cr0 = dn1 * rw2 - dn2 * rw1;
-                        const float cr1 = dn2 * rw0 - dn0 * rw2;
-                        const float cr2 = dn0 * rw1 - dn1 * rw0;
-                        float rd2 = cr0 * cr0 + cr1 * cr1 + cr2 * cr2;
-                        if (rd2 < 1.0e-6f) rd2 = 1.0e-6f;
-                        const float inv_rd = __builtin_amdgcn_rsqf(rd2);
-                        float ti = (cr0 * inv_rd) * rv0 + (cr1 * inv_rd) * rv1 + (cr2 * inv_rd) * rv2;
-                        ti = fminf(fmaxf(ti, -1.f), 1.f);
-                        const float sin2ti = 2.f * fabsf(ti) * __builtin_amdgcn_sqrtf(fmaxf(1.f - ti * ti, 0.f));
-                        rdon = (0.9f + 0.1f * sin2ti) * cos_t0;
-                    } else if (cos_t >= -0.34202f) {
-                        const float u = 0.116978f - cos_t * cos_t;
-                        rdon = 562.25f * (u * u * u) * cos_t0;
+                }  // directional classes
+                const int indx_n = min(bin, AG_NEINT - 1);
+                const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
+                const float S = sol_s[indx_r];
+                // desolvation (main.cpp:3375-3390): slot m gains
+                // (sp[m] vol + volp[m] solq) S per pair, i.e. sp[m] sum(vol S) +
+                // volp[m] sum(solq S) over the point's pairs: two sums per point
+                // instead of a term per slot, combined after the loop
+                dv_acc = __builtin_fmaf(unif(a.vol), S, dv_acc);
+                dq_acc = __builtin_fmaf(unif(a.solq), S, dq_acc);
+                // vdW/H-bond table values of this pair for the pass's slots
+                const float *row = tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots;
+                float E[kMapSlots];
+    #pragma unroll
+                for (int q4 = 0; q4 < kMapSlots / 4; q4++) {
+                    if (q4 < NPLAIN / 4 || q4 == kMapSlots / 4 - 1) {
+                        const float4 v = reinterpret_cast<const float4 *>(row)[q4];
+                        E[4 * q4] = v.x; E[4 * q4 + 1] = v.y; E[4 * q4 + 2] = v.z; E[4 * q4 + 3] = v.w;
+                    } else {
+                        E[4 * q4] = E[4 * q4 + 1] = E[4 * q4 + 2] = E[4 * q4 + 3] = 0.f;
                     }
                 }
-            } else if (hb == AG_HB_A1) {
-                const float cos_t = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
-                rdon = cos_t <= 0.f ? 0.f : cos_t * cos_t;
-            } else if (hb == AG_HB_A2 && dis) {
-                // cos(theta - 1.24791) = c cos(1.24791) + sqrt(1 - c^2) sin(1.24791);
-                // theta <= 1.24791 + pi/2  <=>  c >= -sin(1.24791)
-                float c = -(dn0 * rv0) - dn1 * rv1 - dn2 * rv2;
-                c = fminf(fmaxf(c, -1.f), 1.f);
-                racc = 0.f;
-                rdon = 0.f;
-                if (c >= -sdis) {
-                    const float cc = c * cdis + __builtin_amdgcn_sqrtf(fmaxf(1.f - c * c, 0.f)) * sdis;
-                    rdon = (cc * cc) * (cc * cc);
-                    racc = rdon;
+                // plain slots: the table value
+    #pragma unroll
+                for (int m = 0; m < NPLAIN; m++) acc[m] += E[m];
+                // H-bonding slots (main.cpp:3286-3374): the (probe, atom) case is a
+                // set of 0/1 coefficients chosen per atom class.  The class is the
+                // same for every lane, so the case is a scalar branch: a plain
+                // table add, nothing, or the directional mix.  A disordered
+                // partner uses the hydrogen-type row at r - 1.10 A
+                // (main.cpp:3313-3349), FP64 like the reference.
+                const int dsh = dis ? 4 : 0;
+                bool any_mix = false;
+    #pragma unroll
+                for (int k = 0; k < kHbSlots; k++) {
+                    const int code = (hw >> (8 * k + dsh)) & 15;
+                    any_mix |= (code & ~HC_PLAIN) != 0;
                 }
-            }
-            }  // directional classes
-            const int indx_n = min(bin, AG_NEINT - 1);
-            const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
-            const float S = sol_s[indx_r];
-            // desolvation (main.cpp:3375-3390): slot m gains
-            // (sp[m] vol + volp[m] solq) S per pair, i.e. sp[m] sum(vol S) +
-            // volp[m] sum(solq S) over the point's pairs: two sums per point
-            // instead of a term per slot, combined after the loop
-            dv_acc = __builtin_fmaf(unif(a.vol), S, dv_acc);
-            dq_acc = __builtin_fmaf(unif(a.solq), S, dq_acc);
-            // vdW/H-bond table values of this pair for the pass's slots
-            const float *row = tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots;
-            float E[kMapSlots];
-#pragma unroll
-            for (int q4 = 0; q4 < kMapSlots / 4; q4++) {
-                if (q4 < NPLAIN / 4 || q4 == kMapSlots / 4 - 1) {
-                    const float4 v = reinterpret_cast<const float4 *>(row)[q4];
-                    E[4 * q4] = v.x; E[4 * q4 + 1] = v.y; E[4 * q4 + 2] = v.z; E[4 * q4 + 3] = v.w;
+                if (!any_mix) {
+    #pragma unroll
+                    for (int k = 0; k < kHbSlots; k++) {
+                        const int code = (hw >> (8 * k + dsh)) & 15;
+                        if (code & HC_PLAIN) acc[kPlainSlots + k] += E[kPlainSlots + k];
+                    }
                 } else {
-                    E[4 * q4] = E[4 * q4 + 1] = E[4 * q4 + 2] = E[4 * q4 + 3] = 0.f;
-                }
-            }
-            // plain slots: the table value
-#pragma unroll
-            for (int m = 0; m < NPLAIN; m++) acc[m] += E[m];
-            // H-bonding slots (main.cpp:3286-3374): the (probe, atom) case is a
-            // set of 0/1 coefficients chosen per atom class.  The class is the
-            // same for every lane, so the case is a scalar branch: a plain
-            // table add, nothing, or the directional mix.  A disordered
-            // partner uses the hydrogen-type row at r - 1.10 A
-            // (main.cpp:3313-3349), FP64 like the reference.
-            const int dsh = dis ? 4 : 0;
-            bool any_mix = false;
-#pragma unroll
-            for (int k = 0; k < kHbSlots; k++) {
-                const int code = (hw >> (8 * k + dsh)) & 15;
-                any_mix |= (code & ~HC_PLAIN) != 0;
-            }
-            if (!any_mix) {
-#pragma unroll
-                for (int k = 0; k < kHbSlots; k++) {
-                    const int code = (hw >> (8 * k + dsh)) & 15;
-                    if (code & HC_PLAIN) acc[kPlainSlots + k] += E[kPlainSlots + k];
-                }
-            } else {
-                float Eh[kHbSlots] = {0.f, 0.f, 0.f, 0.f};
-                if (dis && p.hydrogen_type >= 0) {
-                    double rdx, rdy, rdz;
-                    const double r64 = bh ? (double)rB : ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
-                    const int hb_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
-                    const float4 v = reinterpret_cast<const float4 *>(
-                        tab + ((size_t)p.hydrogen_type * AG_NEINT + hb_bin) * kMapSlots)[kPlainSlots / 4];
-                    Eh[0] = v.x; Eh[1] = v.y; Eh[2] = v.z; Eh[3] = v.w;
-                }
-                const float dfac = racc - rdon;
-#pragma unroll
-                for (int k = 0; k < kHbSlots; k++) {
-                    const int m = kPlainSlots + k;
-                    const int code = (hw >> (8 * k + dsh)) & 15;
-                    const float e = E[m];
-                    if (code & HC_PLAIN) acc[m] += e;
-                    if ((code & ~HC_PLAIN) == 0) continue;
-                    // 0/1 coefficients as scalar float bit patterns
-                    const float kF = __int_as_float((code & HC_RACC) ? 0x3f800000 : 0);
-                    const float rsph = __builtin_amdgcn_fmed3f(e * 0.01f, 0.f, 1.f);
-                    const float f = __builtin_fmaf(kF, dfac, rdon);
-                    const float mix = __builtin_fmaf(1.f - f, rsph, f);
-                    if (code & HC_HRAMP) acc[m] = __builtin_fmaf((dis ? Eh[k] : e) * Hramp, mix, acc[m]);
-                    if (code & HC_MINMAX) {
-                        const float v = e * mix;
-                        hmin[k] = fminf(hmin[k], v);
-                        hmax[k] = fmaxf(hmax[k], v);
+                    float Eh[kHbSlots] = {0.f, 0.f, 0.f, 0.f};
+                    if (dis && p.hydrogen_type >= 0) {
+                        double rdx, rdy, rdz;
+                        const double r64 = bh ? (double)rB : ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
+                        const int hb_bin = min(max(0, (int)((r64 - 1.10) * 128.0)), AG_NEINT - 1);
+                        const float4 v = reinterpret_cast<const float4 *>(
+                            tab + ((size_t)p.hydrogen_type * AG_NEINT + hb_bin) * kMapSlots)[kPlainSlots / 4];
+                        Eh[0] = v.x; Eh[1] = v.y; Eh[2] = v.z; Eh[3] = v.w;
+                    }
+                    const float dfac = racc - rdon;
+    #pragma unroll
+                    for (int k = 0; k < kHbSlots; k++) {
+                        const int m = kPlainSlots + k;
+                        const int code = (hw >> (8 * k + dsh)) & 15;
+                        const float e = E[m];
+                        if (code & HC_PLAIN) acc[m] += e;
+                        if ((code & ~HC_PLAIN) == 0) continue;
+                        // 0/1 coefficients as scalar float bit patterns
+                        const float kF = __int_as_float((code & HC_RACC) ? 0x3f800000 : 0);
+                        const float rsph = __builtin_amdgcn_fmed3f(e * 0.01f, 0.f, 1.f);
+                        const float f = __builtin_fmaf(kF, dfac, rdon);
+                        const float mix = __builtin_fmaf(1.f - f, rsph, f);
+                        if (code & HC_HRAMP) acc[m] = __builtin_fmaf((dis ? Eh[k] : e) * Hramp, mix, acc[m]);
+                        if (code & HC_MINMAX) {
+                            const float v = e * mix;
+                            hmin[k] = fminf(hmin[k], v);
+                            hmax[k] = fmaxf(hmax[k], v);
+                        }
                     }
                 }
+                if (p.d_slot >= 0) d_acc += unif(a.dvol) * S;
             }
-            if (p.d_slot >= 0) d_acc += unif(a.dvol) * S;
-        }
         }
     }
 

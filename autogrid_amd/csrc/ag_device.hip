@@ -1588,8 +1588,9 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         std::copy(r.disorder, r.disorder + k, dis + o);
         std::fill(atom_conf + o, atom_conf + o + k, i);
     };
-    // large batches copy on several host threads, one conformation at a time
-    const int n_thr = N > 200000 ? std::min(8, n) : 1;
+    // large batches copy on several host threads (at most 16), one
+    // conformation at a time
+    const int n_thr = N > 200000 ? std::max(1, std::min({16, n, (int)std::thread::hardware_concurrency()})) : 1;
     if (n_thr > 1) {
         std::atomic<int> next{0};
         std::vector<std::thread> th;

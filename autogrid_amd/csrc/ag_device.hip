@@ -276,32 +276,76 @@ __global__ void k_zero_cells(int *count, int *cursor, int n, int *status, int *q
     if (i == 0) *status = 0;
 }
 
-// Exclusive scan of count[0..n) into start[0..n], one workgroup of 1024:
-// each thread sums its own slice, the slice sums are scanned within each
-// wave by shuffles and across the 16 waves through LDS, and each thread
-// then writes its slice's prefixes.
+// Exclusive scan of count[0..n) into start[0..n], one workgroup of 1024
+// walking tiles of 1024 * kScanPer cells.  Each thread takes kScanPer
+// consecutive cells of a tile (16-byte loads and stores, so a wave covers
+// one contiguous 4 KiB span); the thread sums are scanned within each wave
+// by shuffles and across the 16 waves through LDS, and the total of the
+// tiles before is carried in a register.
+constexpr int kScanPer = 16;
+
 __global__ void __launch_bounds__(1024) k_scan(const int *count, int *start, int n) {
     __shared__ int wave_sum[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int per = (n + 1023) / 1024;
-    const int b = min(n, t * per), e = min(n, b + per);
-    int s = 0;
-    for (int i = b; i < e; i++) s += count[i];
-    int x = s;   // inclusive scan over the wave
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off);
-        if (lane >= off) x += y;
+    int carry = 0;
+    for (int base = 0; base < n; base += 1024 * kScanPer) {   // uniform: every thread reaches both barriers
+        const int i0 = base + t * kScanPer;
+        const bool whole = i0 + kScanPer <= n;
+        int v[kScanPer];
+        if (whole) {
+            const int4 *q = reinterpret_cast<const int4 *>(count + i0);
+#pragma unroll
+            for (int j = 0; j < kScanPer / 4; j++) {
+                const int4 a = q[j];
+                v[4 * j] = a.x;
+                v[4 * j + 1] = a.y;
+                v[4 * j + 2] = a.z;
+                v[4 * j + 3] = a.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanPer; j++) v[j] = i0 + j < n ? count[i0 + j] : 0;
+        }
+        int s = 0;
+#pragma unroll
+        for (int j = 0; j < kScanPer; j++) s += v[j];
+        int x = s;   // inclusive scan over the wave
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wave_sum[w] = x;
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int k = 0; k < 16; k++) {
+            const int ws = wave_sum[k];
+            before += k < w ? ws : 0;
+            total += ws;
+        }
+        __syncthreads();   // the next tile rewrites wave_sum
+        int run = carry + before + x - s;
+        if (whole) {
+            int4 *o = reinterpret_cast<int4 *>(start + i0);
+#pragma unroll
+            for (int j = 0; j < kScanPer / 4; j++) {
+                int4 r;
+                r.x = run;
+                r.y = r.x + v[4 * j];
+                r.z = r.y + v[4 * j + 1];
+                r.w = r.z + v[4 * j + 2];
+                run = r.w + v[4 * j + 3];
+                o[j] = r;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanPer; j++) {
+                if (i0 + j < n) start[i0 + j] = run;
+                run += v[j];
+            }
+        }
+        carry += total;
     }
-    if (lane == 63) wave_sum[w] = x;
-    __syncthreads();
-    int before = 0;
-    for (int k = 0; k < w; k++) before += wave_sum[k];
-    int run = before + x - s;
-    for (int i = b; i < e; i++) {
-        start[i] = run;
-        run += count[i];
-    }
-    if (t == 1023) start[n] = before + x;
+    if (t == 0) start[n] = carry;
 }
 
 __global__ void k_cell_scatter(int n, const int *atom_cell, const int *start, int *cursor, int *slot_atom) {

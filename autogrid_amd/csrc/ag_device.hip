@@ -258,14 +258,9 @@ __global__ void k_cell_count(const ConfDev *confs, const int *atom_conf, int n, 
     if (cell >= 0) atomicAdd(&count[cell], 1);
 }
 
-// Fill kernel for counters and staging (no memset nodes in captured graphs).
-__global__ void k_zero_int(int *p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 // Clears the cell counts and cursors [0, n), the status word and the nq
-// work counters of one map set, in one launch.
+// work counters of one map set, in one launch: a kernel rather than memset
+// nodes, since a plan's map set is captured in a graph.
 __global__ void k_zero_cells(int *count, int *cursor, int n, int *status, int *queue, int nq) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {

@@ -223,6 +223,23 @@ __device__ __forceinline__ float far_bias(float s128) {
 __device__ __forceinline__ float far_dk(int k, float s128) { return ((float)k - kFarMid) * s128; }
 __device__ __forceinline__ float far_cx(int ix0, int ne0, float s128) { return ((float)(ix0 - ne0) + kFarMid) * s128; }
 
+// Row q (0 <= q < n1y * n1z) of the far kernel's run order -> (iy, iz).
+// Rows go in blocks of 8 y by 8 z (smaller at the grid's upper edges), y
+// fastest inside a block, blocks in y order within each band of 8 z.  The
+// 64 lanes of a wave then hold runs of one x-run from an 8 x 8 patch of
+// rows rather than eight whole x-rows, so their distances to an atom, and
+// the table entries they read, spread less: fewer LDS bank conflicts.
+__device__ __forceinline__ void far_row(int q, int n1y, int n1z, int &iy, int &iz) {
+    const int bz = q / (8 * n1y);
+    const int o = q - bz * 8 * n1y;
+    const int nzb = min(8, n1z - 8 * bz);   // >= 1 for q in range
+    const int by = o / (8 * nzb);
+    const int o2 = o - by * 8 * nzb;
+    const int nyb = min(8, n1y - 8 * by);   // >= 1 for q in range
+    iy = 8 * by + o2 % nyb;
+    iz = 8 * bz + o2 / nyb;
+}
+
 // The far kernel's per-pair table offset, reproduced bit for bit by the near
 // kernel when it decides which atoms the far kernel left out (same
 // operations in the same order; the build turns off FP contraction).
@@ -1181,10 +1198,11 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
         const int run = (item - as_const(p.tile_prefix)[ci]) * kFarRuns + lane;
         const bool active = run < cf.nruns;
         const int n1x = cf.n1[0], n1y = cf.n1[1];
-        const int row = run / cf.runs_per_row;
-        const int ix0 = (run % cf.runs_per_row) * kFarK;
-        const int iy = row % n1y;
-        const int iz = row / n1y;
+        const int rows = n1y * cf.n1[2];
+        const int xr = run / rows;                 // x-run outermost, rows in patches inside it
+        const int ix0 = xr * kFarK;
+        int iy, iz;
+        far_row(run - xr * rows, n1y, cf.n1[2], iy, iz);
         float *const emap = p.maps + cf.map_off + (long long)p.e_slot * cf.P;
         const long long base = (long long)ix0 + (long long)n1x * ((long long)iy + (long long)n1y * iz);
         float s[kFarK], c[kFarK];

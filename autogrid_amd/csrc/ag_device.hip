@@ -1198,11 +1198,20 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
         const int run = (item - as_const(p.tile_prefix)[ci]) * kFarRuns + lane;
         const bool active = run < cf.nruns;
         const int n1x = cf.n1[0], n1y = cf.n1[1];
-        const int rows = n1y * cf.n1[2];
-        const int xr = run / rows;                 // x-run outermost, rows in patches inside it
-        const int ix0 = xr * kFarK;
-        int iy, iz;
-        far_row(run - xr * rows, n1y, cf.n1[2], iy, iz);
+        int ix0, iy, iz;
+        if (CLAMP) {
+            // x-rows in order: measured faster on the clamped table, where
+            // distant pairs share its last entry
+            const int row = run / cf.runs_per_row;
+            ix0 = (run % cf.runs_per_row) * kFarK;
+            iy = row % n1y;
+            iz = row / n1y;
+        } else {
+            const int rows = n1y * cf.n1[2];
+            const int xr = run / rows;   // x-run outermost, rows in patches inside it
+            ix0 = xr * kFarK;
+            far_row(run - xr * rows, n1y, cf.n1[2], iy, iz);
+        }
         float *const emap = p.maps + cf.map_off + (long long)p.e_slot * cf.P;
         const long long base = (long long)ix0 + (long long)n1x * ((long long)iy + (long long)n1y * iz);
         float s[kFarK], c[kFarK];

@@ -1174,6 +1174,11 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
 // compensated sums in LDS and the first adds them in a fixed order, so the
 // result does not depend on scheduling.  Splitting the atoms keeps four
 // waves per SIMD on grids that give only two per SIMD at one wave per run.
+//
+// Run order: without CLAMP the runs are numbered x-run outermost, rows in
+// 8 x 8 patches inside (far_row), so a wave reads fewer distinct table
+// banks; with CLAMP they go x-row by x-row.  The order decides only which
+// lane computes which run, never a run's arithmetic.
 template <bool CLAMP, int CAP>
 __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_eu(4))) k_far(FarParams p) {
     __shared__ float T[CAP + 1 + kFarRuns * kFarK];

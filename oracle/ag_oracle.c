@@ -12,7 +12,13 @@
  *     copied as fixtures) at the reference's tolerance with the sign fixed;
  *   - closed forms: eps(r) against the Mehler-Solmajer formula, the LJ
  *     coefficient known answer of Tests/hsg1_sm_nbpcoeffs.gpf, a one-atom
- *     receptor's radial e map, min-window smoothing against brute force.
+ *     receptor's radial e map, min-window smoothing against brute force;
+ *   - orc_vina_tables: whole rows against the Vina scoring function's
+ *     closed forms in FP64 (tests/test_vina.py);
+ *   - orc_analytic_maps: one-atom E(r) against the brute-force smoothing
+ *     definition and the reference's table, and whole map sets against the
+ *     table path within per-point Lipschitz bounds
+ *     (tests/test_analytic_host.py).
  */
 #include "ag_oracle.h"
 
@@ -713,9 +719,10 @@ int orc_maps_vina(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *
  * sum_i q_i coeff_estat (332 / eps(r)) / max(r, 0.5) over all atoms
  * (main.cpp:2940-2982).  disorder_h is not part of the analytic model.
  *
- * Parity: pinned against the table path (orc_maps) away from bin, cap and
- * switch effects; the functions themselves are restatements of the
- * reference's formulas. */
+ * Parity: on a one-atom receptor E(r) equals the brute-force minimum over
+ * the continuous window of the textbook potential, and lies within the
+ * Lipschitz bound of the reference's table at floor(128 r); whole map sets
+ * lie within per-point Lipschitz bounds of the table path (orc_maps). */
 
 static double an_phi(const ag_pair *pp, double s) {
     const int a = pp->xA, b = pp->xB;

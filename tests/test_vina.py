@@ -6,6 +6,11 @@ and the tables are "parity unpinned" against the reference itself.  What is
 pinned here:
 
 * the engine's table builder against the oracle, bit for bit;
+* whole rows (bins 1..1023) of both against the Vina scoring function's
+  published closed forms in FP64 -- the two Gaussians, the quadratic
+  repulsion and the hydrophobic ramp with the reference's float weights,
+  plus the reference's own H-bond branch -- to the float rounding of the
+  table's += steps;
 * the xs radii against the ADVINA table values (read_parameter_library.cc:
   322-358) -- float values, exact name match, 1.8 A default;
 * structural properties of the tables that a dropped or mis-signed term
@@ -121,6 +126,60 @@ def test_vina_swap_symmetry(tmp_path):
     # beyond rdd 1.5 no hydrophobic term remains: identical rows
     far = rdd > 1.5
     assert np.array_equal(st.e_vdw_hb[tn, mc][far], st.e_vdw_hb[tc, mn][far])
+
+
+def _vina_row_closed_form(rdd, hbond_pair: bool, hydrophobic: bool):
+    """One Vina table row in FP64 from the published functional forms of the
+    Vina scoring function (Trott & Olson, J. Comput. Chem. 31:455, 2010):
+    gauss1 = exp(-(d/0.5)^2), gauss2 = exp(-((d-3)/2)^2), repulsion = d^2
+    for d < 0, hydrophobic = 1 below d = 0.5 falling linearly to 0 at 1.5,
+    with the weights as float32 (the reference's float locals,
+    main.cpp:1977-1981, which are Vina 1.1.1's).  The H-bond term is the
+    reference's own branch (main.cpp:2106-2115), not Vina's ramp: +w for
+    d <= 0.7, then -(d/0.7) w more for -0.7 < d <= 0.  Returns the row and
+    the sum of the absolute terms (for the float rounding bound)."""
+    g1, g2, rep, whb, whp = (float(np.float32(v)) for v in (-0.035579, -0.005156, 0.840245, -0.587439, -0.035069))
+    terms = [g1 * np.exp(-(rdd / 0.5) ** 2), g2 * np.exp(-((rdd - 3.0) / 2.0) ** 2),
+             np.where(rdd < 0, rep * rdd ** 2, 0.0)]
+    if hbond_pair:
+        terms.append(np.where(rdd <= 0.7, whb, 0.0))
+        terms.append(np.where((rdd > -0.7) & (rdd <= 0), -(rdd / 0.7) * whb, 0.0))
+    if hydrophobic:
+        terms.append(np.where(rdd < 0.5, whp, np.where(rdd < 1.5, (0.5 - rdd) * whp, 0.0)))
+    return sum(terms), sum(np.abs(t) for t in terms)
+
+
+@pytest.mark.parametrize("probe,rtype", [("C", "C"), ("NA", "HD"), ("OA", "HD"), ("N", "OA")])
+def test_vina_rows_closed_form(probe, rtype, tmp_path):
+    """Every bin 1..1023 of whole Vina rows against the closed forms above,
+    to the float rounding of the reference's five float += steps: a wrong
+    weight, Gaussian width or centre, or repulsion exponent in the oracle or
+    the engine's builder fails here.  Pairs: a hydrophobic non-H-bonding
+    pair (C-C), two H-bonding pairs (NA and OA probes on HD) and a polar
+    non-H-bonding pair (N on OA)."""
+    st, lig, recp = _vina_job("small", tmp_path)
+    t, m = recp.index(rtype), lig.index(probe)
+    hb = int(st.pairs[m, t]["hbonder"]) > 0
+    cross = int(st.pairs[t, m]["hbonder"]) if (t < st.n_atom_maps and m < len(recp)) else 0
+    assert hb == ((probe, rtype) in (("NA", "HD"), ("OA", "HD")))
+    xs_sum = ag.xs_radius(rtype)[0] + ag.xs_radius(probe)[0]
+    rdd = _rdd(1024, xs_sum)
+    expect, mag = _vina_row_closed_form(rdd, hb, not hb and cross == 0)
+    expect[0] = 0.0
+    bound = 4 * 2.0 ** -24 * mag + 1e-12
+    for got in (ol.vina_tables(st, lig, recp)[t, m], ag.vina_tables(st, lig, recp)[t, m], st.e_vdw_hb[t, m]):
+        err = np.abs(got.astype(np.float64) - expect)
+        assert got[0] == 0.0
+        assert np.all(err[1:] <= bound[1:]), (probe, rtype, int(np.argmax(err - bound)), float(np.max(err - bound)))
+    # the closed forms are sensitive at this bound: a 1% change in any weight
+    # moves some bin by far more than the rounding allowance
+    for k, scale in enumerate(np.eye(3)):
+        g1, g2, rep = (-0.035579, -0.005156, 0.840245)
+        wrong = (1 + 0.01 * scale)
+        alt = (wrong[0] * g1 * np.exp(-(rdd / 0.5) ** 2) + wrong[1] * g2 * np.exp(-((rdd - 3) / 2) ** 2)
+               + wrong[2] * np.where(rdd < 0, rep * rdd ** 2, 0.0))
+        ref = g1 * np.exp(-(rdd / 0.5) ** 2) + g2 * np.exp(-((rdd - 3) / 2) ** 2) + np.where(rdd < 0, rep * rdd ** 2, 0.0)
+        assert np.max(np.abs(alt - ref)[1:] - bound[1:]) > 0, k
 
 
 def test_vina_repulsive_wall(tmp_path):

@@ -324,15 +324,35 @@ class Context:
         return Plan(self, states, potential, batch=True, tune=tune)
 
     def maps_batch(self, states: list, tune: dict | None = None,
-                   potential: int = _abi.AG_POTENTIAL_AD4, outs: list | None = None) -> list:
-        """Maps of many conformations in one graph launch (ag_maps_batch): a
-        list of [n_maps, n1z, n1y, n1x] float32 arrays."""
+                   potential: int = _abi.AG_POTENTIAL_AD4, outs: list | None = None,
+                   fmaps: list | None = None, cmaps: list | None = None) -> list:
+        """Maps of many conformations (ag_maps_batch): a list of
+        [n_maps, n1z, n1y, n1x] float32 arrays.  Caller-supplied outputs
+        (outs, and fmaps / cmaps when the map set asks for a floating or
+        constriction grid) must be C-contiguous arrays of the conformations'
+        shapes, float32 (int32 for cmaps); when all of them are pinned host
+        memory the library overlaps the copies with the computation."""
         st0 = states[0]
         recs, grids, ms, tab, keep = _batch_structs(states)
-        if outs is None:
-            outs = [np.empty((st.n_maps, st.n1[2], st.n1[1], st.n1[0]), np.float32) for st in states]
-        fm = [np.empty((st.n1[2], st.n1[1], st.n1[0]), np.float32) for st in states] if st0.floating_grid else None
-        cm = [np.empty((st.n1[2], st.n1[1], st.n1[0]), np.int32) for st in states] if st0.constriction_grid else None
+        shapes = [(st.n1[2], st.n1[1], st.n1[0]) for st in states]
+
+        def outputs(given, wanted, dtype, lead):
+            if not wanted:
+                return None
+            if given is None:
+                return [np.empty(lead(st) + sh, dtype) for st, sh in zip(states, shapes)]
+            if len(given) != len(states):
+                raise ValueError(f"{len(given)} output arrays for {len(states)} conformations")
+            for i, (a, st, sh) in enumerate(zip(given, states, shapes)):
+                want = lead(st) + sh
+                if not (isinstance(a, np.ndarray) and a.dtype == dtype and a.flags.c_contiguous
+                        and a.shape == want):
+                    raise ValueError(f"output {i} must be a C-contiguous {np.dtype(dtype).name} array of "
+                                     f"shape {want}")
+            return list(given)
+        outs = outputs(outs, True, np.float32, lambda st: (st.n_maps,))
+        fm = outputs(fmaps, bool(st0.floating_grid), np.float32, lambda st: ())
+        cm = outputs(cmaps, bool(st0.constriction_grid), np.int32, lambda st: ())
         ptrs = (C.c_void_p * len(states))(*[o.ctypes.data for o in outs])
         fptrs = (C.c_void_p * len(states))(*[a.ctypes.data for a in fm]) if fm else None
         cptrs = (C.c_void_p * len(states))(*[a.ctypes.data for a in cm]) if cm else None
@@ -540,6 +560,12 @@ class Analytic:
         wk = np.where(st.pairs["hbonder"] > 0, st.weights[1], st.weights[0])
         self.eps0 = np.ascontiguousarray(st.pairs["eps"] / wk, np.float64)
         self.weights0 = np.array(st.weights, np.float64)
+        # Bumped by set() and forward(): the adjoint linearises at the
+        # parameters and the tape of the last forward, and a caller that
+        # holds an older forward compares this number to know whether that
+        # point is still the engine's.
+        self.generation = 0
+        self.device = "cuda"
 
     def set(self, coord=None, charge=None, rij=None, eps=None, weights=None) -> None:
         arrs = []
@@ -555,9 +581,11 @@ class Analytic:
         w = _abi.Weights(*[float(x) for x in weights]) if weights is not None else None
         check(lib().ag_analytic_set(self._h, ptr(coord, 3 * self.n_atoms), ptr(charge, self.n_atoms),
                                     ptr(rij, M * T), ptr(eps, M * T), C.byref(w) if w is not None else None))
+        self.generation += 1
 
     def forward(self, maps_ptr: int) -> None:
         check(lib().ag_analytic_forward(self._h, C.c_void_p(maps_ptr)))
+        self.generation += 1
 
     def adjoint(self, dmaps_ptr: int) -> dict:
         M, T = self.pair_shape

@@ -22,29 +22,50 @@ from . import Analytic, Context, PreparedState, Scorer
 from .poses import cotangent, make_poses, splat
 
 
+def _sync(device) -> None:
+    """Order the engine's stream after torch's work on a GPU device."""
+    if torch.device(device).type == "cuda":
+        torch.cuda.synchronize()
+
+
 class AnalyticMaps(torch.autograd.Function):
     """maps = AnalyticMaps.apply(coord, charge, rij, eps, weights, engine)
 
     coord [N, 3], charge [N], rij / eps [M, T] (eps unweighted), weights [4]
     (vdW, H-bond, electrostatic, desolvation): float64 CPU tensors.  Returns
     the [n_maps, nz, ny, nx] float32 map set on the engine's GPU.  The
-    backward pass runs the engine's adjoint with dL/dmaps."""
+    backward pass runs the engine's adjoint with dL/dmaps.
+
+    The engine's adjoint linearises at the parameters and tape of its last
+    forward.  Each forward keeps its own parameters and the engine's
+    generation number; when the engine has been set or run again since (two
+    applies before one backward), backward sets this forward's parameters
+    back and re-runs the forward into scratch memory before the adjoint, so
+    every graph gets the gradients of its own inputs."""
 
     @staticmethod
     def forward(ctx, coord, charge, rij, eps, weights, engine: Analytic):
-        engine.set(coord.detach().cpu().numpy(), charge.detach().cpu().numpy(), rij.detach().cpu().numpy(),
-                   eps.detach().cpu().numpy(), weights.detach().cpu().numpy())
-        maps = torch.empty(engine.shape, dtype=torch.float32, device="cuda")
-        torch.cuda.synchronize()
+        params = tuple(np.array(t.detach().cpu().numpy(), np.float64) for t in (coord, charge, rij, eps, weights))
+        engine.set(*params)
+        maps = torch.empty(engine.shape, dtype=torch.float32, device=engine.device)
+        _sync(engine.device)
         engine.forward(maps.data_ptr())
         ctx.engine = engine
+        ctx.params = params
+        ctx.generation = engine.generation
         return maps
 
     @staticmethod
     def backward(ctx, dmaps):
+        eng = ctx.engine
         dmaps = dmaps.contiguous().to(torch.float32)
-        torch.cuda.synchronize()
-        g = ctx.engine.adjoint(dmaps.data_ptr())
+        if eng.generation != ctx.generation:
+            eng.set(*ctx.params)
+            scratch = torch.empty(eng.shape, dtype=torch.float32, device=eng.device)
+            _sync(eng.device)
+            eng.forward(scratch.data_ptr())
+        _sync(eng.device)
+        g = eng.adjoint(dmaps.data_ptr())
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a))  # noqa: E731
         return t(g["coord"]), t(g["charge"]), t(g["rij"]), t(g["eps"]), t(g["weights"]), None
 

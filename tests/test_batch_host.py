@@ -70,3 +70,33 @@ def test_bond_vectors_batch_matches_single(tmp_path):
         rexp, rv, rv2, dis = ag.bond_vectors(st, st.coord, st.type, st.hbond)
         assert np.array_equal(g.rexp, rexp) and np.array_equal(g.rvector, rv)
         assert np.array_equal(g.rvector2, rv2) and np.array_equal(g.disorder, dis)
+
+
+def test_maps_batch_rejects_bad_output_arrays(tmp_path):
+    """Caller-supplied batch outputs are checked for dtype, shape and
+    contiguity before any address reaches the library (no device call:
+    the checks run first)."""
+    import dataclasses
+
+    import pytest
+    ens = synth.make_ensemble(str(tmp_path), n_conf=2, seed=3001, base_atoms=800, n_range=(300, 600),
+                              npts_range=(8, 12))
+    states = [synth.conformation_state(ens, i) for i in range(2)]
+    ctx = object.__new__(ag.Context)      # no device: validation precedes the library call
+    good = [np.zeros((st.n_maps, st.n1[2], st.n1[1], st.n1[0]), np.float32) for st in states]
+    bad_cases = [
+        good[:1],                                                    # one array for two conformations
+        [good[0], good[1].astype(np.float64)],                       # wrong dtype
+        [good[0], good[1][:, :-1]],                                  # wrong shape
+        [good[0], np.asfortranarray(good[1])],                       # not C-contiguous
+        [good[1], good[0]] if states[0].points != states[1].points else [good[0], good[1][..., :1]],
+    ]
+    for outs in bad_cases:
+        with pytest.raises(ValueError):
+            ctx.maps_batch(states, outs=outs)
+    sts = [dataclasses.replace(s, floating_grid=1, constriction_grid=1, _keep=[]) for s in states]
+    fm = [np.empty(st.n1[::-1], np.float32) for st in sts]
+    with pytest.raises(ValueError):          # cmap must be int32
+        ctx.maps_batch(sts, outs=good, fmaps=fm, cmaps=[a.copy() for a in fm])
+    with pytest.raises(ValueError):          # fmap of the wrong shape
+        ctx.maps_batch(sts, outs=good, fmaps=[fm[0], fm[0][:1]], cmaps=[a.view(np.int32) for a in fm])

@@ -163,3 +163,44 @@ def test_batch_pinned_outputs_pipeline(request, which):
                 flat[:] = np.nan
     finally:
         hip.hipHostFree(buf)
+
+
+def test_batch_pinned_outputs_with_distance_grids(ensemble12):
+    """The chunked pipeline with floating and constriction grids: every
+    destination (maps, fmap, cmap) in pinned host memory, so each chunk's
+    fmap / cmap copies run beside its maps.  All three must equal the
+    pageable path's bit for bit."""
+    import ctypes as C
+    states = [dataclasses.replace(s, floating_grid=1, constriction_grid=1, constriction_cutoff=6.5, _keep=[])
+              for s in ensemble12]
+    print(f"[gpu] pinned batch outputs with fmap/cmap, {len(states)} conformations", flush=True)
+    hip = _hip()
+    sizes = [st.n_maps * st.points for st in states]
+    pts = [st.points for st in states]
+    total = 4 * (sum(sizes) + 2 * sum(pts))
+    buf = C.c_void_p()
+    assert hip.hipHostMalloc(C.byref(buf), C.c_size_t(total), 0) == 0
+    try:
+        raw = np.ctypeslib.as_array(C.cast(buf, C.POINTER(C.c_uint8)), shape=(total,))
+        outs, fms, cms, o = [], [], [], 0
+        for st, n in zip(states, sizes):
+            outs.append(raw[o:o + 4 * n].view(np.float32).reshape(st.n_maps, st.n1[2], st.n1[1], st.n1[0]))
+            o += 4 * n
+        for st in states:
+            fms.append(raw[o:o + 4 * st.points].view(np.float32).reshape(st.n1[2], st.n1[1], st.n1[0]))
+            o += 4 * st.points
+        for st in states:
+            cms.append(raw[o:o + 4 * st.points].view(np.int32).reshape(st.n1[2], st.n1[1], st.n1[0]))
+            o += 4 * st.points
+        raw[:] = 0xFF
+        tune = {"cell_size": 4.0, "cube": True}
+        with ag.Context(0) as ctx:
+            ref = ctx.maps_batch(states, tune=tune)
+            rf, rc = ctx.last_fmaps, ctx.last_cmaps
+            ctx.maps_batch(states, tune=tune, outs=outs, fmaps=fms, cmaps=cms)
+            for i in range(len(states)):
+                assert np.array_equal(outs[i].view(np.uint32), ref[i].view(np.uint32)), f"maps {i}"
+                assert np.array_equal(fms[i].view(np.uint32), rf[i].view(np.uint32)), f"fmap {i}"
+                assert np.array_equal(cms[i], rc[i]), f"cmap {i}"
+    finally:
+        hip.hipHostFree(buf)

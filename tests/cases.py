@@ -166,3 +166,58 @@ def write_synthetic_inputs(name: str, dest: str) -> None:
     with open(os.path.join(dest, gpf_name(name)), "w") as f:
         f.write("\n".join(lines) + "\n")
     np.random.default_rng(0)  # no further randomness
+
+
+def xyz_name(gpf_path: str):
+    """The .xyz file a GPF's gridfld line names (its 'fld' replaced by 'xyz',
+    main.cpp:1082-1083), or None."""
+    for ln in open(gpf_path):
+        f = ln.split()
+        if f and f[0].lower() == "gridfld":
+            return f[1].replace("fld", "xyz", 1)
+    return None
+
+
+GLG_PROGRESS_HEAD = "                    Percent   Estimated Time  Time/this plane"
+GLG_SUMMARY_HEAD = "Grid\tAtom\tMinimum   \tMaximum"
+
+
+def glg_record(text: str) -> dict:
+    """The lines of an autogrid4 log that consumers parse, split from the
+    timing text around them:
+      calculating  the 'Calculating N grids over P elements, around K
+                   receptor atoms.' line (main.cpp:2740-2742);
+      progress     the 4 header lines of the per-plane table (main.cpp:2779-2782);
+      planes       per plane: its number, z coordinate and percent done, the
+                   first three fields of main.cpp:3449-3452 (the times after
+                   them are the run's own);
+      summary      the 4 header lines of the extrema table (main.cpp:3530-3533);
+      extrema      per row: map number, type, minimum, maximum (as printed)
+                   and the trailing label (main.cpp:3535-3545);
+      clamp_note   the EINTCLAMP note (main.cpp:3547);
+      completion   the text after the program name on the completion line
+                   (main.cpp:3565)."""
+    lines = text.split("\n")
+    rec = {"calculating": None, "progress": None, "planes": [], "summary": None, "extrema": [],
+           "clamp_note": None, "completion": None}
+    for i, ln in enumerate(lines):
+        if ln.startswith("Calculating ") and " grid" in ln and "element" in ln:
+            rec["calculating"] = ln
+        elif ln == GLG_PROGRESS_HEAD:
+            rec["progress"] = lines[i:i + 4]
+            j = i + 5
+            while j < len(lines) and lines[j].strip():
+                rec["planes"].append(" ".join(lines[j].split()[:3]))
+                j += 1
+        elif ln == GLG_SUMMARY_HEAD:
+            rec["summary"] = lines[i:i + 4]
+            j = i + 4
+            while j < len(lines) and lines[j].strip():
+                f = [x.strip() for x in lines[j].split("\t")]
+                rec["extrema"].append(f[:4] + [f[4] if len(f) > 4 else ""])
+                j += 1
+        elif "Every pairwise-atomic interaction was clamped" in ln:
+            rec["clamp_note"] = ln
+        elif ln.endswith(": Successful Completion."):
+            rec["completion"] = "Successful Completion."
+    return rec

@@ -8,7 +8,11 @@
      - the exit status and whether the log ends in "Successful Completion";
      - per map file, the SHA-256 of its value lines (everything after the 6
        header lines) and the number of values;
-     - the .xyz text and the .fld text without its "Created by" line.
+     - the .xyz text and the .fld text without its "Created by" line;
+     - the log lines consumers parse (cases.glg_record): the
+       'Calculating ...' line, the per-plane table's header and each
+       plane's number, z and percent, the extrema table with its values as
+       printed, the clamp note and the completion line.
 
 3. When oracle/_ref/autogrid4_bhtree is built (make -C oracle ref-bhtree),
    runs it on the tree-build cases (cases.BH_CASES) and records the same
@@ -70,6 +74,9 @@ def main():
                 fld = [ln for ln in open(os.path.join(d, f"{name.split('_CASE')[0] if name.startswith('x1hpv') else name}.maps.fld")).read().split("\n")
                        if not ln.startswith("# Created by")] if name in cases.SYN_CASES else None
                 rec["fld"] = fld
+                xyz = cases.xyz_name(os.path.join(d, gpf))
+                rec["xyz"] = open(os.path.join(d, xyz)).read() if xyz and os.path.exists(os.path.join(d, xyz)) else None
+                rec["glg"] = cases.glg_record(glg)
             out[name] = rec
             print(name, rec["rc"], rec["success"], len(rec.get("maps", {})))
     # the tree build (make -C oracle ref-bhtree) on the cases.BH_CASES variants

@@ -46,3 +46,34 @@ def load_case(name: str, workdir: str):
     finally:
         os.chdir(cwd)
     return st, gpf
+
+
+def _printed_unit(s: str) -> float:
+    """One unit in the last printed digit of a %6.2lf or %9.2le field."""
+    if "e" in s:
+        return 10.0 ** (int(s.split("e")[1]) - 2)
+    return 10.0 ** -len(s.split(".")[1]) if "." in s else 1.0
+
+
+def glg_mismatches(ours: dict, ref: dict) -> list:
+    """Differences between two cases.glg_record() results.  Fixed text
+    compares exactly; each printed extremum may differ from the reference's
+    by the map tolerance max(1e-3, 1e-3 |v|) plus one unit of its last
+    printed digit (both sides round an FP32 / FP64 value to 3 digits)."""
+    out = []
+    for k in ("calculating", "progress", "planes", "summary", "clamp_note", "completion"):
+        if ours.get(k) != ref.get(k):
+            out.append(f"{k}: {ours.get(k)!r} != {ref.get(k)!r}")
+    if len(ours["extrema"]) != len(ref["extrema"]):
+        out.append(f"extrema rows: {len(ours['extrema'])} != {len(ref['extrema'])}")
+        return out
+    for a, b in zip(ours["extrema"], ref["extrema"]):
+        if a[0] != b[0] or a[1] != b[1] or a[4] != b[4]:
+            out.append(f"extrema row labels: {a} != {b}")
+            continue
+        for j in (2, 3):
+            va, vb = float(a[j]), float(b[j])
+            lim = max(TOL_ABS, TOL_REL * abs(vb)) + _printed_unit(b[j]) * (1 + 1e-9)
+            if abs(va - vb) > lim:
+                out.append(f"extrema {b[1]} field {j}: {a[j]} vs {b[j]}")
+    return out

@@ -56,10 +56,12 @@ def test_cli_failure_contract(name, tmp_path):
     assert b"ERROR" in p.stderr
 
 
-@pytest.mark.parametrize("name", ["small", "covalent", "sepdesolv", "hsg1_sm"])
+@pytest.mark.parametrize("name", SUCCESS)
 def test_cli_writes_reference_headers(name, tmp_path):
-    """.xyz, .fld and the six map-header lines are written before the device
-    is opened and must equal the reference's bytes."""
+    """The .xyz text, the .fld, the six map-header lines and the log's
+    'Calculating N grids over P elements' line are written before the
+    device's outcome is looked at, and must equal the reference's bytes
+    (main.cpp:1160-1164, 1897-1956, 2740-2742, 2752-2776)."""
     gpf = cases.materialize(name, str(tmp_path))
     subprocess.run([ag.CLI_PATH, "-p", gpf, "-l", "out.glg"], cwd=tmp_path, capture_output=True, timeout=60)
     g = GOLDEN[name]
@@ -68,6 +70,10 @@ def test_cli_writes_reference_headers(name, tmp_path):
     if g.get("fld"):
         fld = [ln for ln in open(tmp_path / f"{name}.maps.fld").read().split("\n") if not ln.startswith("# Created by")]
         assert fld == g["fld"]
+    assert g["xyz"] is not None
+    xyz = cases.xyz_name(os.path.join(tmp_path, gpf))
+    assert open(tmp_path / xyz).read() == g["xyz"]
+    assert cases.glg_record(open(tmp_path / "out.glg").read())["calculating"] == g["glg"]["calculating"]
 
 
 def test_gpf_keywords_are_case_insensitive_and_comments_ignored(tmp_path):

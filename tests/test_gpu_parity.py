@@ -123,9 +123,12 @@ def test_device_outputs(ctx, tmp_path):
     assert np.array_equal(back, host)
 
 
-@pytest.mark.parametrize("name", ["hsg1_sm", "x1hpv_CASE0", "covalent", "sepdesolv", "fmap_cmap"])
+@pytest.mark.parametrize("name", ["hsg1_sm", "x1hpv_CASE0", "covalent", "sepdesolv", "fmap_cmap", "small", "all13",
+                                  "nodiel", "constdiel"])
 def test_cli_dropin(name, tmp_path):
-    """autogrid4 drop-in: exit 0, completion line, reference headers, values."""
+    """autogrid4 drop-in: exit 0, the log lines consumers parse (per-plane
+    table, extrema table, clamp note, completion), the .xyz text, reference
+    headers and values."""
     print(f"[gpu] cli {name}", flush=True)
     gpf = cases.materialize(name, str(tmp_path))
     p = subprocess.run([ag.CLI_PATH, "-p", gpf, "-l", "out.glg"], cwd=tmp_path, capture_output=True,
@@ -136,6 +139,10 @@ def test_cli_dropin(name, tmp_path):
     st, _ = parity.load_case(name, str(tmp_path / "oracle"))
     ref = ol.maps(ol.oracle_state(st))
     g = GOLDEN[name]
+    # the log lines consumers parse: fixed text exact, extrema to tolerance
+    bad = parity.glg_mismatches(cases.glg_record(glg), g["glg"])
+    assert not bad, bad
+    assert open(tmp_path / cases.xyz_name(os.path.join(tmp_path, gpf))).read() == g["xyz"]
     for k, m in enumerate(cases.map_names(os.path.join(tmp_path, gpf))):
         lines = open(tmp_path / m).read().split("\n")
         assert lines[:6] == g["maps"][m]["header"]

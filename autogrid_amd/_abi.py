@@ -13,8 +13,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libautogrid_amd.so")
-CLI_PATH = os.path.join(_HERE, "bin", "autogrid4")
+# AUTOGRID_AMD_LIB / AUTOGRID_AMD_CLI select another build of the same
+# sources (the host-sanitizer build of `make -C autogrid_amd sanitize`).
+LIB_PATH = os.environ.get("AUTOGRID_AMD_LIB") or os.path.join(_HERE, "libautogrid_amd.so")
+CLI_PATH = os.environ.get("AUTOGRID_AMD_CLI") or os.path.join(_HERE, "bin", "autogrid4")
 
 AG_OK = 0
 STATUS_NAMES = {0: "AG_OK", 1: "AG_ERR_BAD_INPUT", 2: "AG_ERR_NO_DONOR_H", 3: "AG_ERR_OOM",

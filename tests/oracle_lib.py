@@ -17,7 +17,7 @@ from autogrid_amd import _abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle_ag.so")
+ORACLE_SO = os.environ.get("AG_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle_ag.so")
 REF_BIN = os.path.join(ORACLE_DIR, "_ref", "autogrid4")
 
 _lib = None

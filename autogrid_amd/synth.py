@@ -204,8 +204,9 @@ def write_gpf(path: str, stem: str, receptor: str, rec_types, lig_types, npts, c
 def make_workload(name: str, outdir: str, npts=None, z_npts=None, center_offset=(11.5, -7.25, 3.125)) -> dict:
     """Generate receptor + GPF for a named workload into outdir.
 
-    npts overrides the cube size; z_npts (if set) shrinks only the z axis so a
-    slab of the workload can be timed on its own."""
+    npts overrides the cube size (an int, or an (x, y, z) tuple for a box);
+    z_npts (if set) shrinks only the z axis so a slab of the workload can be
+    timed on its own."""
     w = WORKLOADS[name]
     os.makedirs(outdir, exist_ok=True)
     coords, types, q = make_receptor(w.n_atoms, w.radius, w.seed, center=center_offset)
@@ -218,7 +219,9 @@ def make_workload(name: str, outdir: str, npts=None, z_npts=None, center_offset=
         if t not in present:
             present.append(t)
     n = w.npts if npts is None else npts
-    nxyz = (n, n, n if z_npts is None else z_npts)
+    nxyz = tuple(n) if isinstance(n, (tuple, list)) else (n, n, n)
+    if z_npts is not None:
+        nxyz = (nxyz[0], nxyz[1], z_npts)
     gpf = f"{name}.gpf"
     maps = write_gpf(os.path.join(outdir, gpf), name, rec, present, w.ligand_types, nxyz, centroid)
     return {"dir": outdir, "gpf": gpf, "receptor": rec, "maps": maps, "n_atoms": w.n_atoms,

@@ -5,25 +5,28 @@ npts 126 (127^3 = 2,048,383 points), maps A C N NA OA S SA HD F Cl Br I P
 plus e and d (15 maps), AD4.1_bound parameters, distance-dependent
 dielectric.  One step computes the whole map set on the device from
 device-resident inputs: cell list, near-field brick kernel, far-field
-electrostatics kernel.  The metric is the map-set time in seconds.
+electrostatics kernel.  `value` is that map-set time in seconds.
 
-Also reported on the same JSON line:
+Also reported on the same JSON line, each with an `*_is` label:
   roofline      the far-field electrostatics kernel (the dominant one):
                 algorithmic FLOPs 16 per (point, atom) pair, achieved
                 TFLOP/s from its HIP-event time on the engine's stream,
                 against the MI355X FP32 vector peak.
+  wall_excl_ascii_s / wall_with_ascii_output_s
+                the map set from host inputs (plan build, H2D, kernels,
+                D2H into pinned memory), and the autogrid4 drop-in end to
+                end with its ASCII output.
+  accuracy      the GPU maps against the reference autogrid4's .map text
+                (every value) and the FP64 oracle (3 z-planes); W_A.
   cpu_baseline  the reference autogrid4 (built from its sources with
-                OpenMP) on a z-slab of the same workload on this host,
-                scaled to the whole map set by point count.
-  batched       64 receptor conformations (1,500-12,000 atoms, cubic boxes
-                of 41-161 points, maps A C HD N NA OA + e + d): one batch
-                (host tables once, bond vectors on host threads, the
-                conformations in chunks on two compute streams, each
-                chunk's copy into pinned host memory overlapping later
-                chunks' computation) against 64
+                OpenMP) on the whole workload on this host's cores, and a
+                labelled z-slab of the large workload scaled by point count.
+  batched       64 receptor conformations (1,545-11,710 atoms, cubic boxes
+                of 41-155 points, maps A C HD N NA OA + e + d) against 64
                 sequential single-receptor runs (tables, bond vectors,
                 ag_maps_plain with its uploads and copies), both without
-                ASCII output.
+                ASCII output: one captured-graph launch (one-shot and
+                resident replays), and the chunked pinned-output pipeline.
   large         the 20,000-atom, 257^3 map set (seconds per map set,
                 kernel times, pass-E pairs/s).
   analytic      the differentiable mode on the full receptor with a 65^3
@@ -54,32 +57,134 @@ def hip_runtime():
     return C.CDLL("libamdhip64.so.7")
 
 
-def cpu_baseline(workload: str, seconds_target: float = 20.0) -> dict | None:
-    """Reference autogrid4 on a z-slab of the workload; None if not built."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_threads() -> int:
+    """OpenMP threads for the reference: every core this process may run
+    on, up to the reference's own cap of 32 (main.cpp:207), and within
+    OMP_NUM_THREADS where the host sets it (its share of a shared machine)."""
+    n = min(len(os.sched_getaffinity(0)), 32)
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(env))) if env and env.isdigit() else n
+
+
+def run_reference(workload: str, threads: int, npts=None):
+    """The reference autogrid4 (built from its sources by oracle/Makefile)
+    on a workload in a scratch directory: (seconds incl. ASCII output,
+    directory holding its maps, the workload info).  None if not built."""
     from autogrid_amd import synth
     ref = os.path.join(ROOT, "oracle", "_ref", "autogrid4")
     if not os.path.exists(ref):
         return None
-    w = synth.WORKLOADS[workload]
-    cores = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16")))
-    z_npts = 32 if workload == "full" else (8 if workload == "large" else w.npts)
-    d = tempfile.mkdtemp(prefix="agbase_")
+    d = tempfile.mkdtemp(prefix="agref_")
+    info = synth.make_workload(workload, d, npts=npts)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    t0 = time.perf_counter()
+    p = subprocess.run([ref, "-p", info["gpf"], "-l", "ref.glg"], cwd=d, env=env, capture_output=True)
+    t = time.perf_counter() - t0
+    if p.returncode != 0:
+        shutil.rmtree(d, ignore_errors=True)
+        return None
+    return t, d, info
+
+
+def read_map_values(path: str):
+    import numpy as np
+    with open(path, "rb") as f:
+        body = f.read().split(b"\n", 6)[6]
+    return np.array(body.split(), dtype=np.float64)
+
+
+def cpu_leg(st, gpu_maps, map_files, large: bool = True) -> dict:
+    """The reference on the host's cores (the CPU baseline) and the checks
+    of the GPU map set against it and the FP64 oracle.
+
+    cpu_baseline  the full workload run in full by the reference autogrid4
+                  with OpenMP (wall incl. its ASCII output), and a labelled
+                  z-slab of the large workload scaled by point count.
+    accuracy      the GPU map set (device-resident run) against the
+                  reference's .map text at every point of every map (the
+                  text carries 3 decimals, so the bound adds 5e-4), and
+                  against the FP64 oracle at every point of 3 z-planes;
+                  W_A, the exact count of (point, atom) pairs within 8 A,
+                  from the oracle (main.cpp:3036-3049)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    threads = cpu_threads()
+    out = {"cpu_baseline": None, "accuracy": None}
+    r = run_reference("full", threads)
+    if r is None:
+        return out
+    t, d, info = r
+    P, N = st.points, st.n_atoms
+    base = {"value": t, "unit": "s", "cores": threads, "kind": "reference", "cpu_model": cpu_model(),
+            "host_cores_visible": len(os.sched_getaffinity(0)),
+            "sample": f"the whole full workload ({N} atoms, {P} points, {st.n_maps} maps), "
+                      f"OpenMP {threads} threads, wall incl. ASCII map output",
+            "pair_evals_per_s": P * N / t}
+    acc = {}
     try:
-        info = synth.make_workload(workload, d, z_npts=z_npts)
-        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-        t0 = time.perf_counter()
-        p = subprocess.run([ref, "-p", info["gpf"], "-l", "ref.glg"], cwd=d, env=env, capture_output=True)
-        t = time.perf_counter() - t0
-        if p.returncode != 0:
-            return None
-        n1 = w.npts + 1
-        sample_points = n1 * n1 * (z_npts + 1)
-        full_points = n1 ** 3
-        return {"value": t * full_points / sample_points, "unit": "s", "cores": cores, "kind": "reference",
-                "sample": f"npts {w.npts} {w.npts} {z_npts} ({sample_points} points, {t:.2f} s incl. ASCII output), "
-                          f"scaled x{full_points / sample_points:.3f} by point count"}
+        worst, sq, cnt, over = 0.0, 0.0, 0, 0
+        for k, fn in enumerate(map_files):
+            ref = read_map_values(os.path.join(d, fn))
+            g = gpu_maps[k].reshape(-1).astype(np.float64)
+            assert ref.size == g.size, fn
+            err = np.abs(g - ref)
+            lim = np.maximum(1e-3, 1e-3 * np.abs(ref)) + 5e-4
+            worst = max(worst, float(err.max()))
+            sq += float(np.sum(err ** 2))
+            cnt += err.size
+            over += int(np.count_nonzero(err > lim))
+        acc["vs_reference_maps"] = {"max_abs": worst, "rms": (sq / cnt) ** 0.5, "values": cnt,
+                                    "over_tolerance": over,
+                                    "tolerance": "max(1e-3, 1e-3 |ref|) + 5e-4 (3-decimal text)"}
     finally:
         shutil.rmtree(d, ignore_errors=True)
+    nz = st.n1[2]
+    worst, sq, cnt, over = 0.0, 0.0, 0, 0
+    for z in (0, nz // 2, nz - 1):
+        ref = ol.maps(st, z, z + 1)[:, z].reshape(st.n_maps, -1)
+        g = gpu_maps[:, z].reshape(st.n_maps, -1).astype(np.float64)
+        err = np.abs(g - ref)
+        worst = max(worst, float(err.max()))
+        sq += float(np.sum(err ** 2))
+        cnt += err.size
+        over += int(np.count_nonzero(err > np.maximum(1e-3, 1e-3 * np.abs(ref))))
+    acc["vs_fp64_oracle"] = {"max_abs": worst, "rms": (sq / cnt) ** 0.5, "values": cnt, "over_tolerance": over,
+                             "planes": [0, nz // 2, nz - 1], "tolerance": "max(1e-3, 1e-3 |ref|)"}
+    acc["W_A"] = ol.count_pairs(st)
+    acc["W_E"] = P * N
+    out["accuracy"] = acc
+    if large:
+        from autogrid_amd import synth
+        w = synth.WORKLOADS["large"]
+        # the reference's per-point work is the same at every point (both
+        # loops run over every atom), so a centred sub-box of 3 planes
+        # (npts 256 x 96 x 2, about 20 s on 16 cores) scales by point count
+        box = (w.npts, 96, 2)
+        r = run_reference("large", threads, npts=box)
+        if r is not None:
+            t3, d3, _ = r
+            shutil.rmtree(d3, ignore_errors=True)
+            n1 = w.npts + 1
+            sample, full = (box[0] + 1) * (box[1] + 1) * (box[2] + 1), n1 ** 3
+            base["large_extrapolated"] = {
+                "value": t3 * full / sample, "unit": "s", "extrapolated": True,
+                "sample": f"npts {box[0]} {box[1]} {box[2]} centred ({sample} points, {t3:.2f} s incl. ASCII output), "
+                          f"scaled x{full / sample:.1f} by point count"}
+    out["cpu_baseline"] = base
+    return out
 
 
 TRAFFIC_PROFILE = "profiles/full_mapset_5000atoms_127cubed.md"
@@ -143,36 +248,89 @@ def batched_leg(reps: int = 2) -> dict:
                 states = [dataclasses.replace(st, e_vdw_hb=ev, sol_fn=sol, r_eps=reps_, _keep=[]) for st in states]
                 ctx.maps_batch(states, tune=tune, outs=outs)
 
+            def batch_states():
+                ev, sol, reps_ = ag.build_tables(inputs[0])
+                states = ag.bond_vectors_batch(inputs)
+                return [dataclasses.replace(st, e_vdw_hb=ev, sol_fn=sol, r_eps=reps_, _keep=[]) for st in states]
+
+            def batched(tune):
+                ctx.maps_batch(batch_states(), tune=tune, outs=outs)
+
+            def graph_once(tune):
+                # host precompute, ag_batch_create, one launch of the captured
+                # graph (cell list, near pass, persistent far kernel), one copy
+                with ctx.batch(batch_states(), tune=tune) as b:
+                    assert b.map_floats == total
+                    b.execute(dmaps.value)
+                    if hip.hipMemcpy(hbuf, dmaps, C.c_size_t(4 * total), 2) != 0:
+                        raise RuntimeError("hipMemcpy failed")
+
             # autotune once (the choice is what a tuning cache would hold)
             t0 = time.perf_counter()
             tables0 = ag.build_tables(inputs[0])
             with ctx.batch([with_bonds(st, tables0) for st in inputs]) as b:
                 tune = b.tune()
             tune_wall = time.perf_counter() - t0
-            sequential()
-            seq_copy = flat.copy()
-            batched(tune)
-            diff = float(np.max(np.abs(flat - seq_copy)))
-            rel = float(np.max(np.abs(flat - seq_copy) / np.maximum(1.0, np.abs(seq_copy))))
-            t_seq, t_bat = [], []
-            for _ in range(reps):
-                t1 = time.perf_counter()
+            dmaps = C.c_void_p()
+            if hip.hipMalloc(C.byref(dmaps), C.c_size_t(4 * total)) != 0:
+                raise RuntimeError("hipMalloc failed")
+            try:
                 sequential()
-                t_seq.append(time.perf_counter() - t1)
-                t1 = time.perf_counter()
+                seq_copy = flat.copy()
+
+                def diffs():
+                    d = np.abs(flat - seq_copy)
+                    return float(np.max(d)), float(np.max(d / np.maximum(1.0, np.abs(seq_copy))))
                 batched(tune)
-                t_bat.append(time.perf_counter() - t1)
+                diff, rel = diffs()
+                graph_once(tune)
+                gdiff, grel = diffs()
+                t_seq, t_bat, t_graph, t_replay = [], [], [], []
+                for _ in range(reps):
+                    t1 = time.perf_counter()
+                    sequential()
+                    t_seq.append(time.perf_counter() - t1)
+                    t1 = time.perf_counter()
+                    batched(tune)
+                    t_bat.append(time.perf_counter() - t1)
+                    t1 = time.perf_counter()
+                    graph_once(tune)
+                    t_graph.append(time.perf_counter() - t1)
+                # replays of one resident plan: graph launch + one copy
+                with ctx.batch(batch_states(), tune=tune) as b:
+                    b.execute(dmaps.value)
+                    for _ in range(reps + 1):
+                        t1 = time.perf_counter()
+                        b.execute(dmaps.value)
+                        if hip.hipMemcpy(hbuf, dmaps, C.c_size_t(4 * total), 2) != 0:
+                            raise RuntimeError("hipMemcpy failed")
+                        t_replay.append(time.perf_counter() - t1)
+                    replay_gpu_ms = ctx.timing()["total_ms"]
+            finally:
+                hip.hipFree(dmaps)
     finally:
         hip.hipHostFree(hbuf)
     pairs = sum(st.points * st.n_atoms for st in inputs)
-    tb, ts = min(t_bat), min(t_seq)
+    tb, ts, tg, tr = min(t_bat), min(t_seq), min(t_graph), min(t_replay)
     return {"conformations": len(inputs), "points": sum(st.points for st in inputs), "pair_evals": pairs,
             "atoms_range": [min(st.n_atoms for st in inputs), max(st.n_atoms for st in inputs)],
             "npts_range": [min(st.n1[0] - 1 for st in inputs), max(st.n1[0] - 1 for st in inputs)],
-            "batched_s": tb, "sequential_s": ts, "speedup": ts / tb,
+            "sequential_s": ts,
+            "sequential_is": "64 x (host tables + bond vectors + ag_maps_plain: plan build, H2D, kernels, "
+                             "D2H into pinned memory), no ASCII",
+            "graph_s": tg, "graph_speedup": ts / tg,
+            "graph_is": "host tables + bond vectors for all conformations, ag_batch_create, one captured-graph "
+                        "launch (cell list, near pass, persistent far kernel), one D2H into pinned memory",
+            "graph_replay_s": tr, "graph_replay_speedup": ts / tr, "graph_replay_gpu_ms": replay_gpu_ms,
+            "graph_replay_is": "ag_plan_execute replay of the resident batch plan's graph + one D2H",
+            "chunked_s": tb, "chunked_speedup": ts / tb,
+            "chunked_is": "ag_maps_batch with pinned outputs: host precompute, 10 chunks on two compute streams, "
+                          "each chunk's D2H overlapping later chunks",
+            "speedup": ts / tb, "batched_s": tb,
             "batched_pair_evals_per_s": pairs / tb,
             "tune": {"cell_size": tune["cell_size"], "cube": tune["cube"]}, "tune_ms": tune["tune_ms"],
             "tune_wall_s": tune_wall, "max_abs_diff_vs_sequential": diff, "max_rel_diff_vs_sequential": rel,
+            "graph_max_rel_diff_vs_sequential": grel, "graph_max_abs_diff_vs_sequential": gdiff,
             "reps": reps}
 
 
@@ -331,6 +489,8 @@ def main():
               "z-slab across GPUs without communication).", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
+
     import autogrid_amd as ag
     from autogrid_amd import synth
 
@@ -340,7 +500,9 @@ def main():
         cwd = os.getcwd()
         os.chdir(work)
         try:
+            t0 = time.perf_counter()
             st = ag.Job(info["gpf"]).prepare().state()
+            frontend_s = time.perf_counter() - t0     # GPF + PDBQT read, tables, bond vectors
         finally:
             os.chdir(cwd)
         hip = hip_runtime()
@@ -368,10 +530,34 @@ def main():
         P, N = st.points, st.n_atoms
         flops = FLOPS_PER_PAIR_E * P * N
         achieved = flops / (far_avg * 1e-3) / 1e12
-        audit = plan.audit()          # table-bin flips against FP64 distances (outside the timed region)
+        # outside the timed region: the maps for the accuracy checks, and the
+        # table-bin audit against FP64 distances
+        gpu_maps = np.empty((st.n_maps, st.n1[2], st.n1[1], st.n1[0]), np.float32)
+        if hip.hipMemcpy(gpu_maps.ctypes.data_as(C.c_void_p), dptr, C.c_size_t(nbytes), 2) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        audit = plan.audit()
         hip.hipFree(dptr)
         plan.close()
+
+        # the map set from host inputs without ASCII output: ag_maps_plain
+        # (plan build, H2D, kernels, D2H into pinned host memory)
+        hbuf = C.c_void_p()
+        if hip.hipHostMalloc(C.byref(hbuf), C.c_size_t(nbytes), 0) != 0:
+            raise RuntimeError("hipHostMalloc failed")
+        try:
+            host = np.ctypeslib.as_array(C.cast(hbuf, C.POINTER(C.c_float)),
+                                         shape=(st.n_maps, st.n1[2], st.n1[1], st.n1[0]))
+            ctx.maps_plain(st, out=host)
+            walls = []
+            for _ in range(max(3, args.steps // 4)):
+                t1 = time.perf_counter()
+                ctx.maps_plain(st, out=host)
+                walls.append(time.perf_counter() - t1)
+            host_equal = bool(np.array_equal(host.view(np.uint32), gpu_maps.view(np.uint32)))
+        finally:
+            hip.hipHostFree(hbuf)
         ctx.close()
+        wall_excl = min(walls)
 
         # whole drop-in with ASCII output, for the wall time the reference reports
         cli_ok, wall_cli = False, None
@@ -390,12 +576,16 @@ def main():
     large = None if args.no_large else large_leg()
     batch = None if args.no_batch else batched_leg()
     analytic = None if args.no_analytic else analytic_leg()
-    base = None if args.no_cpu_baseline else cpu_baseline(args.workload)
+    cpu = {"cpu_baseline": None, "accuracy": None}
+    if not args.no_cpu_baseline and args.workload == "full":
+        cpu = cpu_leg(st, gpu_maps, info["maps"], large=not args.no_large)
     value = ms_per_step / 1000.0
     out = {
         "metric": "map-set wall s (5k atoms,126^3,14 maps); batched & fwd+adj vs plain GPU",
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "value_is": "device-resident map-set time: ag_plan_execute on uploaded inputs (cell list, near "
+                    "kernel, far kernel, status check), wall clock per step",
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {N} atoms, {st.n1[0]}x{st.n1[1]}x{st.n1[2]} points, "
                                f"{st.n_maps} maps (13 affinity + e + d), spacing 0.375",
@@ -404,18 +594,28 @@ def main():
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "frac_nonpacked_peak": achieved / FP32_PEAK_NONPACKED_TFLOPS,
-                     "flops_per_launch": flops, "avg_launch_ms": far_avg,
+                     "flops_per_launch": flops, "flops_rule": "16 per (point, atom) pair, all P*N pairs",
+                     "avg_launch_ms": far_avg,
                      "traffic": profiled_traffic() if args.workload == "full" else None,
                      "traffic_source": TRAFFIC_PROFILE if args.workload == "full" else None},
         "kernels_ms": {"cell_list": sum(cell_ms) / len(cell_ms), "near": sum(near_ms) / len(near_ms),
                        "far": far_avg},
         "pair_evals_per_s": P * N / (far_avg * 1e-3),
+        "pair_evals_per_s_is": "P*N pass-E pairs over the far kernel's time",
         "bin_audit": audit,
+        "frontend_s": frontend_s,
+        "wall_excl_ascii_s": wall_excl,
+        "wall_excl_ascii_is": "ag_maps_plain from host inputs: plan build, H2D, kernels, D2H into pinned memory "
+                              "(front end not included, see frontend_s)",
+        "wall_excl_ascii_equals_device_run": host_equal,
         "wall_with_ascii_output_s": wall_cli if cli_ok else None,
+        "wall_with_ascii_output_is": "the autogrid4 drop-in end to end: GPF/PDBQT read, tables, device, "
+                                     "15 ASCII maps, .fld, .xyz, log",
+        "accuracy": cpu["accuracy"],
         "large": large,
         "batched": batch,
         "analytic": analytic,
-        "cpu_baseline": base,
+        "cpu_baseline": cpu["cpu_baseline"],
     }
     print(json.dumps(out))
 

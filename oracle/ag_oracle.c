@@ -935,3 +935,80 @@ int orc_format_value(double e, char *out) {
 int orc_format_rmin(double r, char *out) {
     return sprintf(out, "%.3f\n", (float)(floor(r * 1000.0 + 0.5) / 1000.0));
 }
+
+/* ------------------------------------------------------------------------
+ * W_A: the number of (point, atom) pairs Loop 2 keeps, i.e. atoms with
+ * r <= SOFTNBC by the reference's FP64 distance (main.cpp:3036-3049), over
+ * z planes [z0, z1).  Atoms are binned into cells of SOFTNBC + 0.1 A so
+ * each point tests the 27 cells around its own; the test itself is the
+ * oracle's Loop-2 distance.  Returns the count, or -1 on a bad grid.
+ * ---------------------------------------------------------------------- */
+long long orc_count_pairs(const ag_receptor *rec, const ag_grid *grid, int z0, int z1) {
+    const int N = rec->n_atoms;
+    const int n1x = grid->n1[0], n1y = grid->n1[1];
+    const double h = grid->spacing, cell = SOFTNBC + 0.1;
+    if (N <= 0) return 0;
+    if (z0 < 0 || z1 > grid->n1[2] || z0 >= z1) return -1;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = -grid->ne[k] * h - cell;
+        hi[k] = (grid->n1[k] - 1 - grid->ne[k]) * h + cell;
+    }
+    int nc[3];
+    for (int k = 0; k < 3; k++) nc[k] = (int)((hi[k] - lo[k]) / cell) + 1;
+    const long long ncell = (long long)nc[0] * nc[1] * nc[2];
+    int *start = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    int *cellof = (int *)malloc(sizeof(int) * (size_t)N);
+    int *order = (int *)malloc(sizeof(int) * (size_t)N);
+    if (!start || !cellof || !order) {
+        free(start); free(cellof); free(order);
+        return -1;
+    }
+    for (int i = 0; i < N; i++) {
+        int ci[3], out = 0;
+        for (int k = 0; k < 3; k++) {
+            const double x = rec->coord[3 * (size_t)i + k];
+            if (x < lo[k] || x >= hi[k]) out = 1;
+            ci[k] = out ? 0 : (int)((x - lo[k]) / cell);
+        }
+        cellof[i] = out ? -1 : ci[0] + nc[0] * (ci[1] + nc[1] * ci[2]);
+        if (!out) start[cellof[i] + 1]++;
+    }
+    for (long long c = 0; c < ncell; c++) start[c + 1] += start[c];
+    {
+        int *fill = (int *)calloc((size_t)ncell, sizeof(int));
+        for (int i = 0; i < N; i++)
+            if (cellof[i] >= 0) order[start[cellof[i]] + fill[cellof[i]]++] = i;
+        free(fill);
+    }
+    long long total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+    for (int iz = z0; iz < z1; iz++) {
+        double c[3];
+        c[2] = (iz - grid->ne[2]) * h;
+        for (int iy = 0; iy < n1y; iy++) {
+            c[1] = (iy - grid->ne[1]) * h;
+            for (int ix = 0; ix < n1x; ix++) {
+                c[0] = (ix - grid->ne[0]) * h;
+                int ci[3];
+                for (int k = 0; k < 3; k++) ci[k] = (int)((c[k] - lo[k]) / cell);
+                for (int dz = -1; dz <= 1; dz++)
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            const int x = ci[0] + dx, y = ci[1] + dy, z = ci[2] + dz;
+                            if (x < 0 || y < 0 || z < 0 || x >= nc[0] || y >= nc[1] || z >= nc[2]) continue;
+                            const long long cc = x + (long long)nc[0] * (y + (long long)nc[1] * z);
+                            for (int s = start[cc]; s < start[cc + 1]; s++) {
+                                const double *a = rec->coord + 3 * (size_t)order[s];
+                                const double r = hypot((a[0] - c[0]), hypot((a[1] - c[1]), (a[2] - c[2])));
+                                if (!(r > SOFTNBC)) total++;
+                            }
+                        }
+            }
+        }
+    }
+    free(start);
+    free(cellof);
+    free(order);
+    return total;
+}

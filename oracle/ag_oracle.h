@@ -40,6 +40,9 @@ int orc_bond_vectors(const ag_setup *s, int n, const double *coord, const int32_
 int orc_maps(const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
              const ag_tables *tab, double *maps, double *fmap, int32_t *cmap,
              int z0, int z1);
+/* W_A: (point, atom) pairs with r <= 8 A by Loop 2's FP64 distance over z
+ * planes [z0, z1) (main.cpp:3036-3049); -1 on a bad plane range. */
+long long orc_count_pairs(const ag_receptor *rec, const ag_grid *grid, int z0, int z1);
 /* xs_vdw radius of a type (read_parameter_library.cc:322-358); 1 if listed. */
 int orc_xs_radius(const char *type, double *radius);
 /* Vina tables [n_rec_types][n_atom_maps][1024] (main.cpp:2075-2127). */

@@ -45,6 +45,8 @@ def olib() -> C.CDLL:
         L.orc_maps_vina.argtypes = L.orc_maps.argtypes
         L.orc_analytic_maps.argtypes = [C.POINTER(_abi.Receptor), C.POINTER(_abi.Grid), C.POINTER(_abi.MapSet),
                                         vp, C.c_int, C.c_double, C.c_double, C.c_int, vp, vp]
+        L.orc_count_pairs.argtypes = [C.POINTER(_abi.Receptor), C.POINTER(_abi.Grid), C.c_int, C.c_int]
+        L.orc_count_pairs.restype = C.c_longlong
         L.orc_format_value.argtypes = [C.c_double, C.c_char_p]
         L.orc_format_rmin.argtypes = [C.c_double, C.c_char_p]
         _lib = L
@@ -91,6 +93,15 @@ def oracle_state(st: ag.PreparedState) -> ag.PreparedState:
     rexp, rv, rv2, dis = bond_vectors(st)
     return dataclasses.replace(st, e_vdw_hb=ev, sol_fn=sol, r_eps=reps, rexp=rexp, rvector=rv, rvector2=rv2,
                                disorder=dis, _keep=[])
+
+
+def count_pairs(st: ag.PreparedState, z0: int = 0, z1: int | None = None) -> int:
+    """W_A over planes z0..z1: (point, atom) pairs with r <= 8 A (Loop 2)."""
+    rec, grid, _, _, _ = st.structs()
+    n = olib().orc_count_pairs(C.byref(rec), C.byref(grid), z0, st.n1[2] if z1 is None else z1)
+    if n < 0:
+        raise RuntimeError("oracle: bad plane range")
+    return int(n)
 
 
 def maps(st: ag.PreparedState, z0: int = 0, z1: int | None = None, fmap=False, cmap=False):

@@ -203,3 +203,38 @@ def test_electrostatics_is_linear_in_charge(tmp_path):
     e1 = ol.maps(ost)[st.elec_index]
     e2 = ol.maps(dataclasses.replace(ost, charge=2.0 * ost.charge, _keep=[]))[st.elec_index]
     assert np.array_equal(2.0 * e1, e2)
+
+
+def test_count_pairs_against_brute_force(tmp_path):
+    """orc_count_pairs (W_A) equals an all-pairs count with numpy's hypot on
+    the small case, plane ranges add up, and an atom exactly 8 A from a
+    point counts (the reference skips only r > 8)."""
+    import dataclasses
+    st, _ = parity.load_case("small", str(tmp_path))
+    h = st.spacing
+    want = 0
+    for iz in (0, 7, 20, 40):
+        z = (iz - st.ne[2]) * h
+        y = (np.arange(st.n1[1]) - st.ne[1]) * h
+        x = (np.arange(st.n1[0]) - st.ne[0]) * h
+        X, Y = np.meshgrid(x, y, indexing="xy")
+        a = st.coord
+        r = np.hypot(a[:, 0, None, None] - X, np.hypot(a[:, 1, None, None] - Y, a[:, 2, None, None] - z))
+        n = int(np.count_nonzero(~(r > 8.0)))
+        assert ol.count_pairs(st, iz, iz + 1) == n, iz
+        want += n
+    assert want > 0
+    total = ol.count_pairs(st)
+    assert total == ol.count_pairs(st, 0, 20) + ol.count_pairs(st, 20, st.n1[2])
+    # an atom at exactly 8.0 A along x from the grid origin point counts
+    c = np.array([[8.0, 0.0, 0.0]])
+    one = dataclasses.replace(st, coord=c, charge=np.zeros(1), vol=np.zeros(1), solpar=np.zeros(1),
+                              type=st.type[:1].copy(), hbond=np.zeros(1, np.int32), rexp=np.zeros(1, np.int32),
+                              rvector=np.zeros((1, 3)), rvector2=np.zeros((1, 3)), disorder=np.zeros(1, np.uint8),
+                              _keep=[])
+    iz = st.ne[2]
+    X = (np.arange(st.n1[0]) - st.ne[0]) * h
+    Y = (np.arange(st.n1[1]) - st.ne[1]) * h
+    r = np.hypot(8.0 - X[None, :], np.hypot(0.0 - Y[:, None], 0.0))
+    assert ol.count_pairs(one, iz, iz + 1) == int(np.count_nonzero(r <= 8.0))
+    assert np.any(r == 8.0)

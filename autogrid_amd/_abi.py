@@ -28,6 +28,7 @@ AG_NEINT = 1024
 AG_POTENTIAL_AD4 = 0
 AG_POTENTIAL_VINA = 1
 AG_NDIEL = 8192
+AG_ERR_UNSUPPORTED = 6
 
 
 class AgError(RuntimeError):

@@ -20,17 +20,24 @@
 //               cell list): affinity, covalent and d maps, plus the tape the
 //               adjoint needs (closest H, the atoms that gave each point's
 //               H-bond min and max);
-//   k_an_e      one lane per grid point over all atoms: the e map;
+//   k_an_e      runs of 13 or 16 points along x per lane over one chunk of
+//               atoms, and k_an_esum the chunks' sums per point: the e map;
 //   k_an_vh     one lane per grid point: the cotangent of the closest H's
 //               bond vector through every Hramp at that point;
 //   k_an_hsum   one workgroup per donor H: sums k_an_vh's vectors over the
-//               points whose closest H it is;
+//               points whose closest H it is (of the points where they are
+//               nonzero, packed);
 //   k_an_atom   one workgroup per atom: gathers over the grid points within
 //               8 A of it the gradients of the 8 A terms with respect to its
 //               position, bond vectors, charge, pair parameters and the
 //               desolvation weight;
-//   k_an_eadj   atoms x point chunks: the e map's gradients, written as
-//               per-chunk partial sums that k_an_ered adds in a fixed order.
+//   k_an_count / k_an_gscan / k_an_pack
+//               points where a cotangent is nonzero, packed in point order
+//               (the e map's for k_an_eadj, the bond-vector one for
+//               k_an_hsum); k_an_gmask flags the points k_an_atom reads;
+//   k_an_eadj   atoms x chunks of packed points: the e map's gradients,
+//               written as per-chunk partial sums that k_an_ered adds in a
+//               fixed order.
 // Nothing accumulates through atomics: every gradient is a fixed-order sum,
 // so results are bitwise repeatable.
 
@@ -55,7 +62,8 @@ constexpr float kCut2 = 64.0f;
 constexpr float kInv2Sig2 = 1.0f / (2.0f * 3.6f * 3.6f);
 constexpr float kClamp = 100000.f;       // EINTCLAMP
 constexpr float kCellA = 4.0f;           // cell edge (A)
-constexpr int kEChunk = 2048;            // points per e-adjoint chunk
+constexpr int kEAtoms = 128;             // atoms per chunk of the forward e kernel (at least)
+constexpr int kEMaxChunks = 64;          // chunks of the forward e kernel (at most)
 // Mehler-Solmajer constants (distdepdiel.cpp:30-57)
 constexpr float kMsA = -8.5525f, kMsB = 78.4f + 8.5525f, kMsK = 7.7839f, kMsLB = 0.003627f * (78.4f + 8.5525f);
 
@@ -146,16 +154,18 @@ __device__ __forceinline__ LJ lj_eval(const APair &pp, float r, float w) {
     const float a = (float)pp.a, b = (float)pp.b;
     LJ o;
     float dPds, dPdR = 0.f, dPdeps = 0.f;
+    // hardware reciprocals (about 1 ulp; rho^12 carries 12 of them, ~1e-6
+    // relative, far inside the tolerances); a - b is a small integer
+    const float is = __builtin_amdgcn_rcpf(s);
     if (!pp.expl) {
-        const float rho = pp.R / s;
+        const float rho = pp.R * is;
         const float pa = powi(rho, pp.a), pb = powi(rho, pp.b);
-        const float k = 1.f / (a - b);
+        const float k = __builtin_amdgcn_rcpf(a - b);
         o.E = pp.eps * k * (b * pa - a * pb);
-        dPds = pp.eps * k * a * b * (pb - pa) / s;
-        if (dsdr != 0.f) dPdR = pp.eps * k * a * b * (pa - pb) / pp.R;
+        dPds = pp.eps * k * a * b * (pb - pa) * is;
+        if (dsdr != 0.f) dPdR = pp.eps * k * a * b * (pa - pb) * __builtin_amdgcn_rcpf(pp.R);
         dPdeps = k * (b * pa - a * pb);
     } else {
-        const float is = 1.f / s;
         const float pa = powi(is, pp.a), pb = powi(is, pp.b);
         o.E = pp.cA * pa - pp.cB * pb;
         dPds = (-a * pp.cA * pa + b * pp.cB * pb) * is;
@@ -368,13 +378,15 @@ __global__ void __launch_bounds__(kAT) k_an_near(AParams p) {
                 const float d[3] = {a.x - c[0], a.y - c[1], a.z - c[2]};
                 const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 if (!(r2 < kCut2)) continue;
-                const float r = fmaxf(sqrtf(r2), 1.0e-6f);
-                const float ir = 1.f / r;
+                // hardware rsq and exp2, about 1 ulp
+                const float r2c = fmaxf(r2, 1.0e-12f);
+                const float ir = __builtin_amdgcn_rsqf(r2c);
+                const float r = r2c * ir;
                 const float dn[3] = {d[0] * ir, d[1] * ir, d[2] * ir};
                 const int type = a.meta & 0xff, hb = (a.meta >> 8) & 0xff;
                 float dS;
                 const float S = switch_s(r, dS);
-                const float G = p.w_desolv * expf(-r2 * kInv2Sig2) * S;
+                const float G = p.w_desolv * __builtin_amdgcn_exp2f(r2 * (-kInv2Sig2 * 1.44269504088896341f)) * S;
                 const Dir dr = dir_of(a, hb, dn, ch, rvH);
                 const float solq = a.solpar + p.qasp * fabsf(a.q);
 #pragma unroll
@@ -436,63 +448,103 @@ struct EParams {
     int dddiel;
     float inv_diel;           // 332 / diel for a constant dielectric
     float *emap;
+    // forward in runs (k_an_e, k_an_esum)
+    const float4 *fatoms;     // (cs x, cs y, cs z, w): coordinates scaled by cs, see k_an_e
+    float sc;                 // cs * spacing
+    float rcap;               // 2 / cs
+    int echunk, nechunks;     // atoms per chunk, chunks
+    float *epart;             // [nechunks][P] partial sums
     // adjoint
     const float *ge;          // cotangent of the e map
     float4 *partial;          // [chunks][n_atoms]: dx, dy, dz, dq
     int nchunks;
 };
 
-// 332 / eps(r) / max(r, 0.5) and its r-derivative.
-__device__ __forceinline__ float efun(const EParams &p, float r, float &dfdr) {
-    const float rm = fmaxf(r, 0.5f);
-    const float drm = r > 0.5f ? 1.f : 0.f;
-    if (!p.dddiel) {
-        const float f = p.inv_diel / rm;
-        dfdr = -f / rm * drm;
-        return f;
+struct __align__(16) EAtom {
+    float x, y, z, w;
+};
+
+template <int K>
+struct ERun {
+    static constexpr float mid = 0.5f * (K - 1);
+    // (k - mid)^2 and the factor K - 1 - 2k of u * spacing
+    static __device__ __forceinline__ constexpr float c2(int k) { return ((float)k - mid) * ((float)k - mid); }
+    static __device__ __forceinline__ constexpr float m(int k) { return (float)(K - 1 - 2 * k); }
+};
+
+template <int K, bool DD>
+__global__ void __launch_bounds__(64) k_an_e(EParams p) {
+    const int run = blockIdx.x * 64 + threadIdx.x;
+    const int rpr = (p.n1[0] + K - 1) / K;
+    const int nruns = rpr * p.n1[1] * p.n1[2];
+    const bool active = run < nruns;
+    const int rr = active ? run : nruns - 1;   // an idle lane computes a real run and stores nothing
+    const int row = rr / rpr;
+    const int ix0 = (rr - row * rpr) * K;
+    const int iy = row % p.n1[1], iz = row / p.n1[1];
+    const float sc = p.sc;
+    const float py = (float)(iy - p.ne[1]) * sc, pz = (float)(iz - p.ne[2]) * sc;
+    const float cx = ((float)(ix0 - p.ne[0]) + ERun<K>::mid) * sc;
+    const float s2 = sc * sc;
+    const float bias = fmaxf(1.0e-12f, ERun<K>::mid * ERun<K>::mid * s2 * 0x1p-19f);
+    const int a0 = blockIdx.y * p.echunk;
+    const int a1 = min(a0 + p.echunk, p.n_atoms);
+    const float rcap = p.rcap;   // 2 / cs
+    float part[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) part[k] = 0.f;
+    // atoms are read-only here: scalar loads through the constant address space
+    const __attribute__((address_space(4))) EAtom *atoms = (const __attribute__((address_space(4))) EAtom *)p.fatoms;
+    auto ld = [&](int i) {
+        const __attribute__((address_space(4))) EAtom &q = atoms[i];
+        return EAtom{q.x, q.y, q.z, q.w};
+    };
+    if (a0 < a1) {
+        EAtom nxt = ld(a0);
+        for (int a = a0; a < a1; a++) {
+            const EAtom at = nxt;
+            nxt = ld(min(a + 1, a1 - 1));
+            const float dy = at.y - py, dz = at.z - pz;
+            const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, bias));
+            const float u = at.x - cx;
+            const float U = __builtin_fmaf(u, u, dyz2);
+            const float us = u * sc;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const float rb2 = __builtin_fmaf(us, ERun<K>::m(k), __builtin_fmaf(s2, ERun<K>::c2(k), U));
+                const float rs = __builtin_amdgcn_rsqf(rb2);
+                const float h = fminf(rs, rcap);
+                if (DD) {
+                    const float e2 = __builtin_amdgcn_exp2f(-(rb2 * rs));
+                    const float den = __builtin_fmaf(kMsK, e2, 1.f);
+                    const float g = den * __builtin_amdgcn_rcpf(__builtin_fmaf(kMsA, den, kMsB));
+                    part[k] = __builtin_fmaf(at.w, h * g, part[k]);
+                } else {
+                    part[k] = __builtin_fmaf(at.w, h, part[k]);
+                }
+            }
+        }
     }
-    const float ex = expf(-kMsLB * r);
-    const float den = 1.f + kMsK * ex;
-    const float eps = kMsA + kMsB / den;
-    const float deps = kMsB * kMsK * kMsLB * ex / (den * den);
-    const float f = 332.f / (eps * rm);
-    dfdr = -f * (deps / eps + drm / rm);
-    return f;
+    if (active) {
+        float *out = p.epart + (long long)blockIdx.y * p.P + ix0 + (long long)p.n1[0] * (iy + (long long)p.n1[1] * iz);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (ix0 + k < p.n1[0]) out[k] = part[k];
+    }
 }
 
-__global__ void __launch_bounds__(kAT) k_an_e(EParams p) {
-    __shared__ float4 sa[kAT];
-    const long long pt = (long long)blockIdx.x * kAT + threadIdx.x;
-    const bool act = pt < p.P;
-    float c[3] = {0.f, 0.f, 0.f};
-    if (act) {
-        const int ix = (int)(pt % p.n1[0]), iy = (int)((pt / p.n1[0]) % p.n1[1]),
-                  iz = (int)(pt / ((long long)p.n1[0] * p.n1[1]));
-        c[0] = (float)(ix - p.ne[0]) * p.h;
-        c[1] = (float)(iy - p.ne[1]) * p.h;
-        c[2] = (float)(iz - p.ne[2]) * p.h;
-    }
-    float s = 0.f, comp = 0.f;
-    for (int a0 = 0; a0 < p.n_atoms; a0 += kAT) {
-        __syncthreads();
-        const int ai = a0 + threadIdx.x;
-        sa[threadIdx.x] = ai < p.n_atoms ? p.atoms[ai] : make_float4(0.f, 0.f, 0.f, 0.f);
-        __syncthreads();
-        const int na = min(kAT, p.n_atoms - a0);
-        float part = 0.f;
-        for (int j = 0; j < na; j++) {
-            const float4 at = sa[j];
-            const float dx = at.x - c[0], dy = at.y - c[1], dz = at.z - c[2];
-            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            float df;
-            part += at.w * efun(p, r, df);
-        }
-        const float y = part - comp;
+// The e map: per point the chunks' partial sums in chunk order, Kahan-compensated.
+__global__ void k_an_esum(const float *epart, int nchunk, long long P, float *emap) {
+    const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pt >= P) return;
+    float s = 0.f, c = 0.f;
+    for (int j = 0; j < nchunk; j++) {
+        const float y = epart[(long long)j * P + pt] - c;
         const float t = s + y;
-        comp = (t - s) - y;
+        c = (t - s) - y;
         s = t;
     }
-    if (act) p.emap[pt] = s;
+    emap[pt] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -585,7 +637,8 @@ __device__ __forceinline__ void block_sum(float (&v)[NV], float *red) {
 struct HsumParams {
     const int *closest_h;
     const float *vh;
-    long long P;
+    const int *list;         // points with a nonzero vh, in point order
+    const int *count;        // their number (device)
     const int *h_atoms;      // donor H atom indices, one per workgroup
     float *out;              // [n][3]
 };
@@ -593,10 +646,13 @@ struct HsumParams {
 __global__ void __launch_bounds__(kAT) k_an_hsum(HsumParams p) {
     __shared__ float red[4 * 3];
     const int h = p.h_atoms[blockIdx.x];
+    const int n = *p.count;
     float v[3] = {0.f, 0.f, 0.f};
-    for (long long pt = threadIdx.x; pt < p.P; pt += kAT)
+    for (int i = threadIdx.x; i < n; i += kAT) {
+        const long long pt = p.list[i];
         if (p.closest_h[pt] == h)
             for (int k = 0; k < 3; k++) v[k] += p.vh[3 * pt + k];
+    }
     block_sum<3>(v, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) p.out[3 * blockIdx.x + k] = v[k];
@@ -606,11 +662,25 @@ __global__ void __launch_bounds__(kAT) k_an_hsum(HsumParams p) {
 // per affinity map.
 constexpr int kAtomOut = 11 + 2 * kAMaxAff;
 
+// Per point: 1 if any cotangent the atom gather reads there (affinity,
+// desolvation and d slots) is nonzero.  Where all are zero a point adds
+// exactly nothing to any atom's gradient, so k_an_atom skips it.
+__global__ void __launch_bounds__(kAT) k_an_gmask(AParams p, unsigned char *mask) {
+    const long long pt = (long long)blockIdx.x * kAT + threadIdx.x;
+    if (pt >= p.P) return;
+    bool any = p.d_slot >= 0 && p.dmaps[(long long)p.d_slot * p.P + pt] != 0.f;
+    for (int k = 0; k < p.naff; k++)
+        any = any || p.dmaps[(long long)p.aff_slot[k] * p.P + pt] != 0.f ||
+              p.dmaps[(long long)p.des_slot[k] * p.P + pt] != 0.f;
+    mask[pt] = any;
+}
+
 struct AtomAdjParams {
     AParams a;
     const AAtom *by_index;   // atoms by index
     const int *atom_list;    // atoms whose 8 A sphere meets the box, one per workgroup
-    float *out;              // [n_list][kAtomOut]
+    float *out;
+    const unsigned char *mask;   // k_an_gmask              // [n_list][kAtomOut]
 };
 
 __global__ void __launch_bounds__(kAT) k_an_atom(AtomAdjParams q) {
@@ -633,19 +703,55 @@ __global__ void __launch_bounds__(kAT) k_an_atom(AtomAdjParams q) {
     for (int k = 0; k < kAtomOut; k++) acc[k] = 0.f;
     const float solq = a.solpar + p.qasp * fabsf(a.q);
     const float sgnq = a.q > 0.f ? 1.f : (a.q < 0.f ? -1.f : 0.f);
-    for (long long j = threadIdx.x; j < npt; j += kAT) {
+    // The points of the cube within 8 A whose cotangents are not all zero
+    // (k_an_gmask) are appended to an LDS list in cube order, a tile of kAT
+    // cube points at a time; each full kAT of the list is then processed
+    // with every lane busy.  The rest add exactly nothing.
+    __shared__ int list[2 * kAT];
+    __shared__ int wcount[kAT / 64];
+    int nlist = 0;
+    for (long long t0 = 0; t0 < npt || nlist > 0; t0 += kAT) {
+        if (t0 < npt) {
+            const long long j = t0 + threadIdx.x;
+            bool keep = false;
+            int cell = 0;
+            if (j < npt) {
+                const int ix = lo[0] + (int)(j % nx), iy = lo[1] + (int)((j / nx) % ny),
+                          iz = lo[2] + (int)(j / ((long long)nx * ny));
+                const long long pt = (long long)ix + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
+                const float dx = a.x - (float)(ix - p.ne[0]) * p.h, dy = a.y - (float)(iy - p.ne[1]) * p.h,
+                            dz = a.z - (float)(iz - p.ne[2]) * p.h;
+                keep = dx * dx + dy * dy + dz * dz < kCut2 && q.mask[pt];
+                cell = (int)j;
+            }
+            const unsigned long long m = __ballot(keep);
+            if ((threadIdx.x & 63) == 0) wcount[threadIdx.x >> 6] = __popcll(m);
+            __syncthreads();
+            int before = nlist;
+            for (int w = 0; w < (int)(threadIdx.x >> 6); w++) before += wcount[w];
+            if (keep) list[before + __popcll(m & ((1ull << (threadIdx.x & 63)) - 1))] = cell;
+            int added = 0;
+            for (int w = 0; w < kAT / 64; w++) added += wcount[w];
+            nlist += added;
+            __syncthreads();
+            if (nlist < kAT && t0 + kAT < npt) continue;   // gather a full tile first
+        }
+        // process the first min(nlist, kAT) entries, then move the rest down
+        const int take = min(nlist, kAT);
+        if ((int)threadIdx.x < take) {
+        const long long j = list[threadIdx.x];
         const int ix = lo[0] + (int)(j % nx), iy = lo[1] + (int)((j / nx) % ny), iz = lo[2] + (int)(j / ((long long)nx * ny));
+        const long long pt = (long long)ix + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
         const float c[3] = {(float)(ix - p.ne[0]) * p.h, (float)(iy - p.ne[1]) * p.h, (float)(iz - p.ne[2]) * p.h};
         const float d[3] = {a.x - c[0], a.y - c[1], a.z - c[2]};
         const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        if (!(r2 < kCut2)) continue;
-        const long long pt = (long long)ix + (long long)p.n1[0] * ((long long)iy + (long long)p.n1[1] * iz);
-        const float r = fmaxf(sqrtf(r2), 1.0e-6f);
-        const float ir = 1.f / r;
+        const float r2c = fmaxf(r2, 1.0e-12f);
+        const float ir = __builtin_amdgcn_rsqf(r2c);   // hardware rsq and exp2, about 1 ulp
+        const float r = r2c * ir;
         const float dn[3] = {d[0] * ir, d[1] * ir, d[2] * ir};
         float dS;
         const float S = switch_s(r, dS);
-        const float ex = expf(-r2 * kInv2Sig2);
+        const float ex = __builtin_amdgcn_exp2f(r2 * (-kInv2Sig2 * 1.44269504088896341f));
         const float G0 = p.w_desolv * ex;               // sol(r)
         const float dG0 = -2.f * r * kInv2Sig2 * G0;    // d sol / dr
         const int ch = p.closest_h[pt];
@@ -740,6 +846,15 @@ __global__ void __launch_bounds__(kAT) k_an_atom(AtomAdjParams q) {
         // dn = d / r, r = |d|, d = x - c
         const float dot = gdn[0] * dn[0] + gdn[1] * dn[1] + gdn[2] * dn[2];
         for (int k = 0; k < 3; k++) acc[k] += (gdn[k] - dot * dn[k]) * ir + g_r * dn[k];
+        }
+        __syncthreads();
+        const int rest = nlist - take;
+        const int moved = (int)threadIdx.x < rest ? list[take + threadIdx.x] : 0;
+        __syncthreads();
+        if ((int)threadIdx.x < rest) list[threadIdx.x] = moved;
+        nlist = rest;
+        __syncthreads();
+        if (t0 >= npt && nlist == 0) break;
     }
     block_sum<kAtomOut>(acc, red);
     if (threadIdx.x == 0)
@@ -747,41 +862,165 @@ __global__ void __launch_bounds__(kAT) k_an_atom(AtomAdjParams q) {
 }
 
 // e-map adjoint: lane = atom, workgroup = (atom block, point chunk).
-__global__ void __launch_bounds__(kAT) k_an_eadj(EParams p) {
-    __shared__ float sg[kAT];
-    const int ai = blockIdx.x * kAT + threadIdx.x;
-    const int chunk = blockIdx.y;
+// The e map's adjoint gathers, per atom, over the points where the
+// cotangent g is nonzero (the poses' trilinear footprint, a fraction of the
+// box).  Those points are packed first, in point order, as (x, y, z, g):
+// k_an_gcount counts them per block of kGPack points, k_an_gscan turns the
+// counts into offsets (one workgroup) and k_an_gpack writes them.
+constexpr int kGPack = 1024;             // points per packing block (4 per lane)
+constexpr int kEAdjChunk = 1024;         // packed points per e-adjoint chunk
+
+// Points where a predicate holds, packed in point order: k_an_count counts
+// them per block of kGPack points, k_an_gscan turns the counts into offsets
+// (one workgroup) and k_an_pack writes their indices.
+struct NonzeroE {             // the e map's cotangent
+    const float *ge;
+    __device__ bool operator()(long long pt) const { return ge[pt] != 0.f; }
+};
+struct NonzeroVh {            // the closest H's bond-vector cotangent (k_an_vh)
+    const float *vh;
+    __device__ bool operator()(long long pt) const {
+        return vh[3 * pt] != 0.f || vh[3 * pt + 1] != 0.f || vh[3 * pt + 2] != 0.f;
+    }
+};
+
+template <typename Pred>
+__global__ void __launch_bounds__(256) k_an_count(Pred pred, long long P, int *counts) {
+    __shared__ int wc[4];
+    const long long b = (long long)blockIdx.x * kGPack;
+    int n = 0;
+    for (int k = 0; k < 4; k++) {
+        const long long pt = b + k * 256 + threadIdx.x;
+        n += pt < P && pred(pt);
+    }
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// Exclusive offsets of the block counts, and their total in offs[n].
+__global__ void __launch_bounds__(1024) k_an_gscan(const int *counts, int n, int *offs) {
+    __shared__ int ws[16];
+    const int per = (n + 1023) / 1024;
+    const int lo = min(n, threadIdx.x * per), hi = min(n, lo + per);
+    int t = 0;
+    for (int i = lo; i < hi; i++) t += counts[i];
+    // inclusive scan of the lane sums: within each wave, then across waves
+    int v = t;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) ws[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 1; k < 16; k++) ws[k] += ws[k - 1];
+    __syncthreads();
+    int run = v - t + (w ? ws[w - 1] : 0);
+    for (int i = lo; i < hi; i++) {
+        offs[i] = run;
+        run += counts[i];
+    }
+    if (threadIdx.x == 1023) offs[n] = ws[15];
+}
+
+template <typename Pred>
+__global__ void __launch_bounds__(256) k_an_pack(Pred pred, long long P, const int *offs, int *packed) {
+    __shared__ int wc[4][4];
+    const long long b = (long long)blockIdx.x * kGPack;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long m[4];
+    for (int k = 0; k < 4; k++) {
+        const long long pt = b + k * 256 + threadIdx.x;
+        m[k] = __ballot(pt < P && pred(pt));
+        if (lane == 0) wc[k][w] = __popcll(m[k]);
+    }
+    __syncthreads();
+    int base = offs[blockIdx.x];
+    for (int k = 0; k < 4; k++) {
+        int before = base;
+        for (int j = 0; j < w; j++) before += wc[k][j];
+        if ((m[k] >> lane) & 1) packed[before + __popcll(m[k] & ((1ull << lane) - 1))] = (int)(b + k * 256 + threadIdx.x);
+        for (int j = 0; j < 4; j++) base += wc[k][j];
+    }
+}
+
+// Count, offsets and indices of the points where pred holds; the count
+// lands in offs[nblk] (device).
+template <typename Pred>
+static void pack_points(Pred pred, long long P, int *counts, int *offs, int *packed, hipStream_t st) {
+    const int nblk = (int)((P + kGPack - 1) / kGPack);
+    k_an_count<<<nblk, 256, 0, st>>>(pred, P, counts);
+    k_an_gscan<<<1, 1024, 0, st>>>(counts, nblk, offs);
+    k_an_pack<<<nblk, 256, 0, st>>>(pred, P, offs, packed);
+}
+
+// One wave per (64 atoms, chunk of kEAdjChunk packed points): per atom
+// sum_p g f(r) and sum_p g d(q w f(r))/d(atom position), f = 332 / (eps(r)
+// max(r, 0.5)) (inv_diel / max(r, 0.5) for a constant dielectric), with
+// hardware rsq / exp2 / rcp.  Chunks past the packed count write zeros, so
+// k_an_ered's fixed-order sum needs no count.
+template <bool DD>
+__global__ void __launch_bounds__(64) k_an_eadj(EParams p, const int *packed, const int *count) {
+    __shared__ float4 sp[256];
+    const int ai = blockIdx.x * 64 + threadIdx.x;
     const bool act = ai < p.n_atoms;
     const float4 at = act ? p.atoms[ai] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const long long p0 = (long long)chunk * kEChunk;
-    const long long p1 = min(p.P, p0 + kEChunk);
+    const int npk = *count;
+    const int p0 = blockIdx.y * kEAdjChunk, p1 = min(npk, p0 + kEAdjChunk);
     float gx = 0.f, gy = 0.f, gz = 0.f, gq = 0.f;
-    for (long long b = p0; b < p1; b += kAT) {
+    constexpr float cs = kMsLB * 1.44269504088896341f;      // exp(-lambda B r) = 2^(-cs r)
+    constexpr float dk = -kMsB * kMsK * kMsLB;             // d(1/eps)/dr = dk e y^2 (see below)
+    for (int b = p0; b < p1; b += 256) {
         __syncthreads();
-        const long long pp = b + threadIdx.x;
-        sg[threadIdx.x] = pp < p1 ? p.ge[pp] : 0.f;
+        if (b + (int)threadIdx.x * 4 < p1)
+            for (int k = 0; k < 4; k++) {
+                const int q = b + threadIdx.x * 4 + k;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (q < p1) {
+                    const long long pt = packed[q];
+                    const int ix = (int)(pt % p.n1[0]), iy = (int)((pt / p.n1[0]) % p.n1[1]),
+                              iz = (int)(pt / ((long long)p.n1[0] * p.n1[1]));
+                    v = make_float4((float)(ix - p.ne[0]) * p.h, (float)(iy - p.ne[1]) * p.h,
+                                    (float)(iz - p.ne[2]) * p.h, p.ge[pt]);
+                }
+                sp[threadIdx.x * 4 + k] = v;
+            }
         __syncthreads();
-        const int nb = (int)min((long long)kAT, p1 - b);
+        const int nb = min(256, p1 - b);
         for (int j = 0; j < nb; j++) {
-            const float g = sg[j];
-            if (g == 0.f) continue;
-            const long long pt = b + j;
-            const int ix = (int)(pt % p.n1[0]), iy = (int)((pt / p.n1[0]) % p.n1[1]),
-                      iz = (int)(pt / ((long long)p.n1[0] * p.n1[1]));
-            const float dx = at.x - (float)(ix - p.ne[0]) * p.h;
-            const float dy = at.y - (float)(iy - p.ne[1]) * p.h;
-            const float dz = at.z - (float)(iz - p.ne[2]) * p.h;
-            const float r = fmaxf(sqrtf(dx * dx + dy * dy + dz * dz), 1.0e-6f);
-            float df;
-            const float f = efun(p, r, df);
-            gq += g * f;
-            const float s = g * at.w * df / r;
-            gx += s * dx;
-            gy += s * dy;
-            gz += s * dz;
+            const float4 P = sp[j];
+            const float dx = at.x - P.x, dy = at.y - P.y, dz = at.z - P.z;
+            const float r2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1.0e-12f);
+            const float rs = __builtin_amdgcn_rsqf(r2);
+            const bool beyond = rs < 2.f;                   // r > 0.5
+            const float rm = beyond ? rs : 2.f;             // 1 / max(r, 0.5)
+            const float drm = beyond ? -rs * rs : 0.f;      // d(1 / max(r, 0.5))/dr
+            float f, dfdr;
+            if (DD) {
+                // 1/eps = den y, den = 1 + k e, y = 1 / (A den + B), e = exp(-lambda B r):
+                // d(1/eps)/dr = B y^2 dden/dr = -B k lambda B e y^2
+                const float e = __builtin_amdgcn_exp2f(-cs * (r2 * rs));
+                const float den = __builtin_fmaf(kMsK, e, 1.f);
+                const float y = __builtin_amdgcn_rcpf(__builtin_fmaf(kMsA, den, kMsB));
+                const float G = den * y;
+                const float dG = dk * e * (y * y);
+                f = 332.f * G * rm;
+                dfdr = 332.f * __builtin_fmaf(dG, rm, G * drm);
+            } else {
+                f = p.inv_diel * rm;
+                dfdr = p.inv_diel * drm;
+            }
+            gq = __builtin_fmaf(P.w, f, gq);
+            const float sc = P.w * at.w * dfdr * rs;       // g q w f'(r) / r
+            gx = __builtin_fmaf(sc, dx, gx);
+            gy = __builtin_fmaf(sc, dy, gy);
+            gz = __builtin_fmaf(sc, dz, gz);
         }
     }
-    if (act) p.partial[(long long)chunk * p.n_atoms + ai] = make_float4(gx, gy, gz, gq);
+    if (act) p.partial[(long long)blockIdx.y * p.n_atoms + ai] = make_float4(gx, gy, gz, gq);
 }
 
 __global__ void k_an_ered(const float4 *partial, int n_atoms, int nchunks, double *out) {
@@ -833,7 +1072,9 @@ struct ag_analytic {
     std::vector<int> h_atoms, atom_list;
     // device
     DevBuf d_atoms, d_by_index, d_rv, d_cells, d_pairs, d_eatoms, d_closest, d_amin, d_amax, d_vh, d_hout,
-        d_hatoms, d_alist, d_aout, d_partial, d_eout;
+        d_hatoms, d_alist, d_aout, d_partial, d_eout, d_fatoms, d_epart, d_gcount, d_gpack, d_vcount, d_vpack,
+        d_gmask;
+    int eK = 16;                       // run length of the forward e kernel
     AParams ap{};
     EParams ep{};
     int nchunks = 0;
@@ -1032,6 +1273,14 @@ static void analytic_upload(ag_analytic *a) {
     for (int i = 0; i < N; i++)
         eat[i] = make_float4((float)a->coord[3 * i], (float)a->coord[3 * i + 1], (float)a->coord[3 * i + 2],
                              (float)(a->charge[i] * a->w[2]));
+    // the forward's atoms: coordinates scaled by cs, weights with the
+    // electrostatic constants (k_an_e)
+    const double cs = a->dddiel ? (double)kMsLB * 1.4426950408889634 : 1.0;
+    const double wk = a->dddiel ? 332.0 * cs : a->inv_diel;
+    std::vector<float4> fat(std::max(N, 1));
+    for (int i = 0; i < N; i++)
+        fat[i] = make_float4((float)(cs * a->coord[3 * i]), (float)(cs * a->coord[3 * i + 1]),
+                             (float)(cs * a->coord[3 * i + 2]), (float)(a->charge[i] * a->w[2] * wk));
     // donor H atoms and atoms whose 8 A sphere meets the box (adjoint work lists)
     a->h_atoms.clear();
     a->atom_list.clear();
@@ -1101,7 +1350,15 @@ static void analytic_upload(ag_analytic *a) {
     e.n_atoms = N;
     e.dddiel = a->dddiel;
     e.inv_diel = (float)a->inv_diel;
-    a->nchunks = (int)((a->P + kEChunk - 1) / kEChunk);
+    e.fatoms = (const float4 *)up(a->d_fatoms, fat.data(), sizeof(float4) * fat.size());
+    e.sc = (float)(cs * g.spacing);
+    e.rcap = (float)(2.0 / cs);
+    e.echunk = std::max(kEAtoms, (N + kEMaxChunks - 1) / kEMaxChunks);
+    e.nechunks = std::max(1, (N + e.echunk - 1) / e.echunk);
+    e.epart = (float *)a->d_epart.get(sizeof(float) * (size_t)e.nechunks * (size_t)a->P);
+    // run length: 13 or 16 points, whichever leaves fewer idle run slots
+    a->eK = 13 * ((g.n1[0] + 12) / 13) < 16 * ((g.n1[0] + 15) / 16) ? 13 : 16;
+    a->nchunks = (int)((a->P + kEAdjChunk - 1) / kEAdjChunk);
     e.nchunks = a->nchunks;
     e.partial = (float4 *)a->d_partial.get(sizeof(float4) * (size_t)a->nchunks * std::max(N, 1));
     up(a->d_hatoms, a->h_atoms.data(), sizeof(int) * a->h_atoms.size());
@@ -1125,7 +1382,16 @@ void analytic_forward(ag_analytic *a, float *d_maps) {
     if (a->elec >= 0) {
         EParams e = a->ep;
         e.emap = d_maps + (size_t)a->elec * a->P;
-        k_an_e<<<grid, kAT, 0, st>>>(e);
+        const long long nruns = (long long)a->grid.n1[1] * a->grid.n1[2] * ((a->grid.n1[0] + a->eK - 1) / a->eK);
+        const dim3 eg((unsigned)((nruns + 63) / 64), (unsigned)e.nechunks);
+        if (a->eK == 13) {
+            if (a->dddiel) k_an_e<13, true><<<eg, 64, 0, st>>>(e);
+            else k_an_e<13, false><<<eg, 64, 0, st>>>(e);
+        } else {
+            if (a->dddiel) k_an_e<16, true><<<eg, 64, 0, st>>>(e);
+            else k_an_e<16, false><<<eg, 64, 0, st>>>(e);
+        }
+        k_an_esum<<<(unsigned)((a->P + 255) / 256), 256, 0, st>>>(e.epart, e.nechunks, a->P, e.emap);
         HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipEventRecord(c->ev[2], st));
@@ -1157,7 +1423,11 @@ void analytic_adjoint(ag_analytic *a, const float *d_dmaps, double *g_coord, dou
     double *d_eout = (double *)a->d_eout.get(sizeof(double) * 4 * (size_t)std::max(N, 1));
     if (nh > 0) {
         k_an_vh<<<grid, kAT, 0, st>>>(p);
-        HsumParams hp{p.closest_h, p.vh, a->P, (const int *)a->d_hatoms.ptr, d_hout};
+        const int nblk = (int)((a->P + kGPack - 1) / kGPack);
+        int *counts = (int *)a->d_vcount.get(sizeof(int) * (2 * (size_t)nblk + 1));
+        int *vlist = (int *)a->d_vpack.get(sizeof(int) * (size_t)a->P);
+        pack_points(NonzeroVh{p.vh}, a->P, counts, counts + nblk, vlist, st);
+        HsumParams hp{p.closest_h, p.vh, vlist, counts + 2 * nblk, (const int *)a->d_hatoms.ptr, d_hout};
         k_an_hsum<<<nh, kAT, 0, st>>>(hp);
     }
     if (nl > 0) {
@@ -1166,6 +1436,9 @@ void analytic_adjoint(ag_analytic *a, const float *d_dmaps, double *g_coord, dou
         q.by_index = (const AAtom *)a->d_by_index.ptr;
         q.atom_list = (const int *)a->d_alist.ptr;
         q.out = d_aout;
+        unsigned char *mask = (unsigned char *)a->d_gmask.get((size_t)a->P);
+        k_an_gmask<<<grid, kAT, 0, st>>>(p, mask);
+        q.mask = mask;
         k_an_atom<<<nl, kAT, 0, st>>>(q);
     }
     HIP_CHECK(hipGetLastError());
@@ -1173,8 +1446,14 @@ void analytic_adjoint(ag_analytic *a, const float *d_dmaps, double *g_coord, dou
     if (want_e) {
         EParams e = a->ep;
         e.ge = d_dmaps + (size_t)a->elec * a->P;
-        dim3 eg((unsigned)((N + kAT - 1) / kAT), (unsigned)a->nchunks);
-        k_an_eadj<<<eg, kAT, 0, st>>>(e);
+        const int nblk = (int)((a->P + kGPack - 1) / kGPack);
+        int *counts = (int *)a->d_gcount.get(sizeof(int) * (2 * (size_t)nblk + 1));
+        int *offs = counts + nblk;
+        int *packed = (int *)a->d_gpack.get(sizeof(int) * (size_t)a->P);
+        pack_points(NonzeroE{e.ge}, a->P, counts, offs, packed, st);
+        const dim3 eg((unsigned)((N + 63) / 64), (unsigned)a->nchunks);
+        if (a->dddiel) k_an_eadj<true><<<eg, 64, 0, st>>>(e, packed, offs + nblk);
+        else k_an_eadj<false><<<eg, 64, 0, st>>>(e, packed, offs + nblk);
         k_an_ered<<<(N + 255) / 256, 256, 0, st>>>(e.partial, N, a->nchunks, d_eout);
         HIP_CHECK(hipGetLastError());
     }

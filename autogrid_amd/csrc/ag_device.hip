@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 #include <memory>
 #include <string>
 #include <thread>
@@ -100,6 +101,8 @@ struct ConfDev {
     long long map_off;            // offset of its [n_maps][P] block in the map output
     long long pt_off;             // offset of its [P] block in the fmap / cmap outputs
     int runs_per_row, nruns;      // far-kernel runs of kFarK points along x
+    int far_const;                // far kernel: the table-free path for atoms past bin 8192 of a wave's box
+    int far_clamp;                // far kernel: some pair may pass the table's end
 };
 
 // Read-only device tables (conformation records, work prefixes) are read
@@ -177,6 +180,7 @@ struct FarParams {
     const float *tab;             // [tab_len]
     int tab_len;
     unsigned tmax4;               // 4 * (tab_len - 1): byte offset of the last entry
+    float c_last;                 // the table's last value, r_eps[8191] (every bin >= 8191)
     float *maps;                  // e map slot: in = near part, out = total
     int e_slot;
 };
@@ -204,24 +208,31 @@ __device__ __forceinline__ unsigned bin4_of(float rb2, float rs) {
 
 // Far-kernel geometry of a run of kFarK points along x.  The run's points
 // sit at cx + dk, dk = (k - kFarMid) s128 (k = 0..kFarK-1, cx the run's
-// centre), and
-//   rb2 = (u - dk)^2 + dy^2 + dz^2 + bias = fma(u, -2 dk, (u^2 + dyz2) + dk^2),
-// u = ax - cx: the points k and kFarK-1-k share dk^2, so kFarK points cost
-// kFarK/2 adds and kFarK fmas (1.5 operations a point instead of a subtract
-// and an fma).  The rounding of u^2 + dyz2 + dk^2 is below
-// 2^-23 (u^2 + dk^2 + dyz2); with u = dk the exact value is only dyz2, so the
-// bias (a floor for dyz2) is 2^-20 kFarMid^2 s128^2, four times that error
-// at the largest dk, which keeps rb2 > 0 and rsq finite.  In bins (128 r)
-// it moves r by bias / (2 r), below 1e-4 of a bin for the far field's
-// r >= 1025 bins at 0.375 A spacing.
+// centre, s128 = 128 spacing), and with u = ax - cx
+//   rb2 = (u - dk)^2 + dy^2 + dz^2 + bias
+//       = fma(u s128, 15 - 2k, fma(s128^2, (j + 1/2)^2, fma(u, u, dyz2))),
+// j = |k - kFarMid| - 1/2: the points k and kFarK-1-k share the middle
+// term, and both factors 15 - 2k and (j + 1/2)^2 are exact small constants
+// the instructions carry as literals, so no register holds them.  Per point
+// that is 1.5 operations instead of a subtract and an fma.  The roundings
+// of the three products and sums are below 2^-21 (7.5 s128)^2 in total when
+// u = dk, where the exact value is only dyz2; the bias (a floor for dyz2)
+// is 2^-19 (7.5 s128)^2, twice that, which keeps rb2 > 0 and rsq finite.
+// In bins (128 r) it moves r by bias / (2 r), about 1e-4 of a bin at the far
+// field's r >= 1025 bins and 0.375 A spacing.
 constexpr float kFarMid = 0.5f * (kFarK - 1);
 
 __device__ __forceinline__ float far_bias(float s128) {
     const float d = kFarMid * s128;
-    return fmaxf(1.0e-3f, d * d * 0x1p-20f);
+    return fmaxf(1.0e-3f, d * d * 0x1p-19f);
 }
-__device__ __forceinline__ float far_dk(int k, float s128) { return ((float)k - kFarMid) * s128; }
 __device__ __forceinline__ float far_cx(int ix0, int ne0, float s128) { return ((float)(ix0 - ne0) + kFarMid) * s128; }
+// (j + 1/2)^2 for point k, and the factor 15 - 2k of u s128
+__device__ __forceinline__ constexpr float far_c2(int k) {
+    return (k >= kFarK / 2 ? (float)(k - kFarK / 2) + 0.5f : (float)(kFarK / 2 - 1 - k) + 0.5f) *
+           (k >= kFarK / 2 ? (float)(k - kFarK / 2) + 0.5f : (float)(kFarK / 2 - 1 - k) + 0.5f);
+}
+__device__ __forceinline__ constexpr float far_m(int k) { return (float)(kFarK - 1 - 2 * k); }
 
 // Row q (0 <= q < n1y * n1z) of the far kernel's run order -> (iy, iz).
 // Rows go in blocks of 8 y by 8 z (smaller at the grid's upper edges), y
@@ -240,16 +251,18 @@ __device__ __forceinline__ void far_row(int q, int n1y, int n1z, int &iy, int &i
     iz = 8 * bz + o2 / nyb;
 }
 
-// The far kernel's per-pair table offset, reproduced bit for bit by the near
-// kernel when it decides which atoms the far kernel left out (same
-// operations in the same order; the build turns off FP contraction).
-__device__ __forceinline__ unsigned far_bin4(float ax, float ay, float az, float cx, float dk, float py, float pz,
-                                             float bias) {
+// The far kernel's per-pair table offset for point k of the run centred at
+// cx, reproduced bit for bit by the near kernel when it decides which atoms
+// the far kernel left out, and by the audit (same operations in the same
+// order; the build turns off FP contraction).
+__device__ __forceinline__ unsigned far_bin4(float ax, float ay, float az, float cx, int k, float s128, float py,
+                                             float pz, float bias) {
     const float dy = ay - py, dz = az - pz;
     const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, bias));
     const float u = ax - cx;
-    const float W = __builtin_fmaf(u, u, dyz2) + dk * dk;
-    const float rb2 = __builtin_fmaf(u, -2.f * dk, W);
+    const float U = __builtin_fmaf(u, u, dyz2);
+    const float W = __builtin_fmaf(s128 * s128, far_c2(k), U);
+    const float rb2 = __builtin_fmaf(u * s128, far_m(k), W);
     return bin4_of(rb2, __builtin_amdgcn_rsqf(rb2));
 }
 
@@ -783,7 +796,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                         int qx, qy, qz;
                         point_index(qx, qy, qz);
                         const int kxE = qx % kFarK;
-                        near_e = far_bin4(a.xE, a.yE, a.zE, far_cx(qx - kxE, nex, s128), far_dk(kxE, s128),
+                        near_e = far_bin4(a.xE, a.yE, a.zE, far_cx(qx - kxE, nex, s128), kxE, s128,
                                           (float)(qy - ney) * s128, (float)(qz - nez) * s128, far_bias(s128)) <
                                  4u * kFarBinMin;
                     }
@@ -1115,12 +1128,12 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
         const double cx = (double)(ix - p.ne[0]) * p.spacing, cy = (double)(iy - p.ne[1]) * p.spacing,
                      cz = (double)(iz - p.ne[2]) * p.spacing;
         const int kx = ix % kFarK;
-        const float cxE = far_cx(ix - kx, p.ne[0], p.s128), dkE = far_dk(kx, p.s128);
+        const float cxE = far_cx(ix - kx, p.ne[0], p.s128);
         const float pyE = (float)(iy - p.ne[1]) * p.s128, pzE = (float)(iz - p.ne[2]) * p.s128;
         const float biasE = far_bias(p.s128);
         for (int a = 0; a < p.n_atoms; a++) {
             const float4 at = p.far_atoms[a];
-            const int fb = (int)(far_bin4(at.x, at.y, at.z, cxE, dkE, pyE, pzE, biasE) >> 2);
+            const int fb = (int)(far_bin4(at.x, at.y, at.z, cxE, kx, p.s128, pyE, pzE, biasE) >> 2);
             double dx, dy, dz;
             const double r64 = ref_dist(p.coord[3 * a], p.coord[3 * a + 1], p.coord[3 * a + 2], cx, cy, cz, dx, dy, dz);
             const int b64 = min((int)(r64 * 128.0), AG_NDIEL - 1);
@@ -1159,14 +1172,75 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
 // far kernel
 // ---------------------------------------------------------------------------
 
-// CLAMP: the table stops at bin 8191 and indices are clamped as the
-// reference clamps them (main.cpp:2956); without it the table is extended
-// to the largest bin any pair can reach, every entry past 8191 holding the
-// reference's last value, and the clamp instruction disappears.
+// Bins at and beyond kFarLastBin all read r_eps[8191]: the reference clamps
+// the index there (main.cpp:2956).  An atom at least kFarConstBins from
+// every point of a wave's box reaches only such bins (FP32 rounding moves a
+// bin by ~1e-3, the margin is one bin), so its pairs need no table.
+constexpr int kFarLastBin = AG_NDIEL - 1;
+constexpr float kFarConstBins = (float)(AG_NDIEL + 1);
+
+// One pair of an atom with the 16 points of a lane's run, accumulated into
+// part[]: the squared distance in bins^2 by the run-centre expansion, rsq,
+// and either the table (bin offset in one fma and one and, the LDS read;
+// CLAMP caps the offset at the table's last entry) or, for an atom beyond
+// the table's last bin from the whole wave (TABLE false), the constant
+// r_eps[8191] folded into the atom's weight.
+template <bool TABLE, bool CLAMP>
+__device__ __forceinline__ void far_pair(const PlainF4 &at, float wc, float py, float pz, float cx, float bias,
+                                         float s128, float s2, const float *T, unsigned tmax4, float *part) {
+    const float dy = at.y - py, dz = at.z - pz;
+    const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, bias));
+    const float u = at.x - cx;
+    const float U = __builtin_fmaf(u, u, dyz2);
+    const float us = u * s128;
+    float W[kFarK / 2];
+#pragma unroll
+    for (int j = 0; j < kFarK / 2; j++) W[j] = __builtin_fmaf(s2, far_c2(kFarK / 2 + j), U);
+    float rs[kFarK];
+    unsigned off[kFarK];
+#pragma unroll
+    for (int k = 0; k < kFarK; k++) {
+        const int j = k >= kFarK / 2 ? k - kFarK / 2 : kFarK / 2 - 1 - k;
+        const float rb2 = __builtin_fmaf(us, far_m(k), W[j]);
+        rs[k] = __builtin_amdgcn_rsqf(rb2);
+        if (TABLE) {
+            off[k] = bin4_of(rb2, rs[k]);
+            if (CLAMP) off[k] = min(off[k], tmax4);
+        }
+    }
+    if (TABLE) {
+        float tv[kFarK];
+#pragma unroll
+        for (int k = 0; k < kFarK; k++)
+            tv[k] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(T) + off[k]);
+#pragma unroll
+        for (int k = 0; k < kFarK; k++) part[k] = __builtin_fmaf(at.w, rs[k] * tv[k], part[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kFarK; k++) part[k] = __builtin_fmaf(wc, rs[k], part[k]);
+    }
+}
+
+// A wave-uniform float moved to a scalar register.
+__device__ __forceinline__ float sreg(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // CAP: LDS table capacity in entries (>= tab_len).  The table is the first
 // thing in the kernel's only LDS array, so it starts at address 0 and a
 // bin's byte offset is its LDS address; after it one int broadcasts the work
-// counter and kFarRuns * kFarK floats carry the second half's sums.
+// counter and kFarRuns * kFarK floats carry the second half's sums.  The
+// table runs to the largest bin a pair can reach (at most kFarTabMax
+// entries), every entry past 8191 holding the reference's last value.
 //
 // An item is kFarRuns runs; the workgroup's two halves (four waves each)
 // take the same runs, the first half the atoms [0, a_mid), the second
@@ -1174,12 +1248,25 @@ __global__ void __launch_bounds__(256) k_audit(AuditParams p) {
 // compensated sums in LDS and the first adds them in a fixed order, so the
 // result does not depend on scheduling.  Splitting the atoms keeps four
 // waves per SIMD on grids that give only two per SIMD at one wave per run.
+// Run order: x-run outermost, rows in 8 x 8 patches inside (far_row), so
+// the 64 runs of a wave lie in a small box.
 //
-// Run order: without CLAMP the runs are numbered x-run outermost, rows in
-// 8 x 8 patches inside (far_row), so a wave reads fewer distinct table
-// banks; with CLAMP they go x-row by x-row.  The order decides only which
-// lane computes which run, never a run's arithmetic.
-template <bool CLAMP, int CAP>
+// Per conformation (ConfDev::far_const, so a conformation's maps are the
+// same in every plan; a launch serves the conformations of its TFREE and
+// passes over the others' items): with few pairs past bin 8192, every atom
+// goes through the table, in index order, one atom ahead, and the table
+// covers every bin.  For large boxes, for each block of 64 atoms the wave tests every
+// atom's distance to its box at once (one atom per lane, one ballot).
+// Atoms at least kFarConstBins from the whole box reach only bins past
+// 8191, whose value is r_eps[8191], and take the table-free path; within a
+// block they go first, then the table's atoms, each in index order (the
+// order depends only on the geometry, so results repeat).  When the
+// conformation's pairs may pass the table's end (ConfDev::far_clamp), a
+// wave whose box could carry a table-path pair there caps its table
+// indices; past bin 8191 every entry is r_eps[8191], so the cap changes no
+// value.  Lanes past the conformation's runs
+// compute at a point of their wave's box and store nothing.
+template <int CAP, bool TFREE>
 __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_eu(4))) k_far(FarParams p) {
     __shared__ float T[CAP + 1 + kFarRuns * kFarK];
     int &item_s = reinterpret_cast<int *>(T)[CAP];
@@ -1187,6 +1274,7 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
     for (int i = threadIdx.x; i < p.tab_len; i += kFarThreads) T[i] = p.tab[i];
     const int half = __builtin_amdgcn_readfirstlane(threadIdx.x / kFarRuns);   // uniform per wave
     const int lane = threadIdx.x % kFarRuns;
+    const int lane64 = threadIdx.x % 64;
     int item;
     if (p.queue) {
         if (threadIdx.x == 0) item_s = p.item_base + atomicAdd(p.queue, 1);
@@ -1200,95 +1288,152 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
     while (item < p.item_end) {
         const int ci = find_conf(as_const(p.tile_prefix), p.nconf, item);
         const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
+        if (cf.far_const != (int)TFREE) goto next_item;   // uniform per workgroup
+        {
         const int run = (item - as_const(p.tile_prefix)[ci]) * kFarRuns + lane;
         const bool active = run < cf.nruns;
         const int n1x = cf.n1[0], n1y = cf.n1[1];
-        int ix0, iy, iz;
-        if (CLAMP) {
-            // x-rows in order: measured faster on the clamped table, where
-            // distant pairs share its last entry
-            const int row = run / cf.runs_per_row;
-            ix0 = (run % cf.runs_per_row) * kFarK;
-            iy = row % n1y;
-            iz = row / n1y;
-        } else {
+        int ix0 = 0, iy = 0, iz = 0;
+        if (active) {
             const int rows = n1y * cf.n1[2];
             const int xr = run / rows;   // x-run outermost, rows in patches inside it
             ix0 = xr * kFarK;
             far_row(run - xr * rows, n1y, cf.n1[2], iy, iz);
         }
+        // the wave's box of points, in grid indices
+        const int bx0 = __builtin_amdgcn_readfirstlane(wave_min(active ? ix0 : INT_MAX));
+        const int bx1 = __builtin_amdgcn_readfirstlane(wave_max(active ? ix0 : INT_MIN));
+        const int by0 = __builtin_amdgcn_readfirstlane(wave_min(active ? iy : INT_MAX));
+        const int by1 = __builtin_amdgcn_readfirstlane(wave_max(active ? iy : INT_MIN));
+        const int bz0 = __builtin_amdgcn_readfirstlane(wave_min(active ? iz : INT_MAX));
+        const int bz1 = __builtin_amdgcn_readfirstlane(wave_max(active ? iz : INT_MIN));
         float *const emap = p.maps + cf.map_off + (long long)p.e_slot * cf.P;
-        const long long base = (long long)ix0 + (long long)n1x * ((long long)iy + (long long)n1y * iz);
         float s[kFarK], c[kFarK];
-        if (active) {
-            const float py = (float)(iy - cf.ne[1]) * cf.s128;
-            const float pz = (float)(iz - cf.ne[2]) * cf.s128;
-            const float cx = far_cx(ix0, cf.ne[0], cf.s128);
-            const float bias = far_bias(cf.s128);
-            // -2 dk per point; dk^2 per mirror pair (k, kFarK-1-k), j = k - kFarK/2
-            float m[kFarK], d2[kFarK / 2];
+        if (bx0 <= bx1) {   // the wave has runs (uniform)
+            if (!active) {  // compute at a point of the box
+                ix0 = bx0;
+                iy = by0;
+                iz = bz0;
+            }
+            const float s128 = cf.s128;
+            const float py = (float)(iy - cf.ne[1]) * s128;
+            const float pz = (float)(iz - cf.ne[2]) * s128;
+            const float cx = far_cx(ix0, cf.ne[0], s128);
+            const float bias = far_bias(s128);
+            const float s2 = s128 * s128;
+            const long long base = (long long)ix0 + (long long)n1x * ((long long)iy + (long long)n1y * iz);
 #pragma unroll
             for (int k = 0; k < kFarK; k++) {
-                const float dk = far_dk(k, cf.s128);
-                m[k] = -2.f * dk;
-                if (k >= kFarK / 2) d2[k - kFarK / 2] = dk * dk;
-                s[k] = (half == 0 && ix0 + k < n1x) ? emap[base + k] : 0.f;
+                s[k] = (half == 0 && active && ix0 + k < n1x) ? emap[base + k] : 0.f;
                 c[k] = 0.f;
             }
             // atoms are read-only here: scalar loads, one atom ahead
-            const AG_CONST PlainF4 *atoms = as_const(reinterpret_cast<const PlainF4 *>(p.atoms + cf.atom_base));
+            const float4 *const vatoms = p.atoms + cf.atom_base;
+            const AG_CONST PlainF4 *atoms = as_const(reinterpret_cast<const PlainF4 *>(vatoms));
             const int n_atoms = cf.n_atoms;
             const int a_mid = (n_atoms + 1) / 2;   // Kahan blocks of each half start at its first atom
             const int a_beg = half ? a_mid : 0;
             const int a_end = half ? n_atoms : a_mid;
-            PlainF4 nxt = ld4(atoms, min(a_beg, max(a_end - 1, 0)));
-            for (int a0 = a_beg; a0 < a_end; a0 += kFarAtomChunk) {
-                const int a1 = min(a0 + kFarAtomChunk, a_end);
-                float part[kFarK];
+            if constexpr (!TFREE) {
+                PlainF4 nxt = ld4(atoms, min(a_beg, max(a_end - 1, 0)));
+                for (int a0 = a_beg; a0 < a_end; a0 += kFarAtomChunk) {
+                    const int a1 = min(a0 + kFarAtomChunk, a_end);
+                    float part[kFarK];
 #pragma unroll
-                for (int k = 0; k < kFarK; k++) part[k] = 0.f;
-                for (int a = a0; a < a1; a++) {
-                    const PlainF4 at = nxt;
-                    nxt = ld4(atoms, min(a + 1, a_end - 1));
-                    const float dy = at.y - py, dz = at.z - pz;
-                    const float dyz2 = __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, bias));
-                    const float u = at.x - cx;
-                    const float U = __builtin_fmaf(u, u, dyz2);
-                    float W[kFarK / 2];
-#pragma unroll
-                    for (int j = 0; j < kFarK / 2; j++) W[j] = U + d2[j];
-                    float rs[kFarK];
-                    unsigned off[kFarK];
+                    for (int k = 0; k < kFarK; k++) part[k] = 0.f;
+                    for (int a = a0; a < a1; a++) {
+                        const PlainF4 at = nxt;
+                        nxt = ld4(atoms, min(a + 1, a_end - 1));
+                        far_pair<true, false>(at, 0.f, py, pz, cx, bias, s128, s2, T, 0u, part);
+                    }
 #pragma unroll
                     for (int k = 0; k < kFarK; k++) {
-                        const int j = k >= kFarK / 2 ? k - kFarK / 2 : kFarK / 2 - 1 - k;
-                        const float rb2 = __builtin_fmaf(u, m[k], W[j]);
-                        rs[k] = __builtin_amdgcn_rsqf(rb2);
-                        off[k] = bin4_of(rb2, rs[k]);
-                        if (CLAMP) off[k] = min(off[k], p.tmax4);
+                        const float y = part[k] - c[k];
+                        const float t = s[k] + y;
+                        c[k] = (t - s[k]) - y;
+                        s[k] = t;
                     }
-                    float tv[kFarK];
-#pragma unroll
-                    for (int k = 0; k < kFarK; k++)
-                        tv[k] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(T) + off[k]);
-#pragma unroll
-                    for (int k = 0; k < kFarK; k++) part[k] = __builtin_fmaf(at.w, rs[k] * tv[k], part[k]);
                 }
+            } else {
+                // box corners in the far kernel's frame (bins); a run spans
+                // kFarK points in x.  Uniform: held in scalar registers,
+                // which the atom test reads directly.
+                const float xlo = sreg((float)(bx0 - cf.ne[0]) * s128);
+                const float xhi = sreg((float)(bx1 + kFarK - 1 - cf.ne[0]) * s128);
+                const float ylo = sreg((float)(by0 - cf.ne[1]) * s128), yhi = sreg((float)(by1 - cf.ne[1]) * s128);
+                const float zlo = sreg((float)(bz0 - cf.ne[2]) * s128), zhi = sreg((float)(bz1 - cf.ne[2]) * s128);
+                const float thr2 = kFarConstBins * kFarConstBins;
+                // a table-path atom is within kFarConstBins of the box, so its
+                // pairs stay below that plus the box diagonal
+                const float diag = __builtin_sqrtf((xhi - xlo) * (xhi - xlo) + (yhi - ylo) * (yhi - ylo) +
+                                                   (zhi - zlo) * (zhi - zlo));
+                const bool clamp = cf.far_clamp && kFarConstBins + diag + 4.f >= (float)(p.tab_len - 1);
+                const float c_last = p.c_last;
+                for (int a0 = a_beg; a0 < a_end; a0 += kFarAtomChunk) {
+                    const int a1 = min(a0 + kFarAtomChunk, a_end);
+                    float part[kFarK];
 #pragma unroll
-                for (int k = 0; k < kFarK; k++) {
-                    const float y = part[k] - c[k];
-                    const float t = s[k] + y;
-                    c[k] = (t - s[k]) - y;
-                    s[k] = t;
+                    for (int k = 0; k < kFarK; k++) part[k] = 0.f;
+                    for (int b0 = a0; b0 < a1; b0 += 64) {
+                        const int nb = min(64, a1 - b0);
+                        bool beyond = false;
+                        if (lane64 < nb) {
+                            const float4 A = vatoms[b0 + lane64];
+                            const float ex = fmaxf(fmaxf(xlo - A.x, A.x - xhi), 0.f);
+                            const float ey = fmaxf(fmaxf(ylo - A.y, A.y - yhi), 0.f);
+                            const float ez = fmaxf(fmaxf(zlo - A.z, A.z - zhi), 0.f);
+                            beyond = __builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez)) >= thr2;
+                        }
+                        const unsigned long long all = nb == 64 ? ~0ull : ((1ull << nb) - 1);
+                        unsigned long long far_set = __ballot(beyond) & all;
+                        unsigned long long tab_set = ~far_set & all;
+                        if (far_set) {
+                            PlainF4 nxt = ld4(atoms, b0 + __builtin_ctzll(far_set));
+                            far_set &= far_set - 1;
+                            while (true) {
+                                const PlainF4 at = nxt;
+                                const bool more = far_set != 0;
+                                if (more) {
+                                    nxt = ld4(atoms, b0 + __builtin_ctzll(far_set));
+                                    far_set &= far_set - 1;
+                                }
+                                far_pair<false, false>(at, at.w * c_last, py, pz, cx, bias, s128, s2, T, 0u, part);
+                                if (!more) break;
+                            }
+                        }
+                        if (tab_set) {
+                            PlainF4 nxt = ld4(atoms, b0 + __builtin_ctzll(tab_set));
+                            tab_set &= tab_set - 1;
+                            while (true) {
+                                const PlainF4 at = nxt;
+                                const bool more = tab_set != 0;
+                                if (more) {
+                                    nxt = ld4(atoms, b0 + __builtin_ctzll(tab_set));
+                                    tab_set &= tab_set - 1;
+                                }
+                                if (clamp) far_pair<true, true>(at, 0.f, py, pz, cx, bias, s128, s2, T, p.tmax4, part);
+                                else far_pair<true, false>(at, 0.f, py, pz, cx, bias, s128, s2, T, 0u, part);
+                                if (!more) break;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < kFarK; k++) {
+                        const float y = part[k] - c[k];
+                        const float t = s[k] + y;
+                        c[k] = (t - s[k]) - y;
+                        s[k] = t;
+                    }
                 }
             }
-            if (half) {
+            if (half && active) {
 #pragma unroll
                 for (int k = 0; k < kFarK; k++) R[k * kFarRuns + lane] = s[k] - c[k];
             }
         }
         __syncthreads();
         if (half == 0 && active) {
+            const long long base = (long long)ix0 + (long long)n1x * ((long long)iy + (long long)n1y * iz);
 #pragma unroll
             for (int k = 0; k < kFarK; k++) {
                 const float y = R[k * kFarRuns + lane] - c[k];
@@ -1296,6 +1441,8 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
                 if (ix0 + k < n1x) emap[base + k] = t;
             }
         }
+        }
+    next_item:
         if (!p.queue) break;
         __syncthreads();
         if (threadIdx.x == 0) item_s = p.item_base + atomicAdd(p.queue, 1);
@@ -1350,7 +1497,7 @@ struct ag_plan {
     bool want_fmap = false, want_cmap = false;
     std::vector<int> zero_slots;       // output maps that are all zero (Vina e/d)
     FarParams far{};
-    bool run_far = false, far_clamp = true;
+    bool run_far = false;
     bool far_atoms_ready = false;
     double coeff_estat = 0.0;
     int far_grid = 0;                  // persistent far-kernel grid
@@ -1497,25 +1644,35 @@ static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStr
     HIP_CHECK(hipGetLastError());
 }
 
-// Far-kernel variants by LDS table capacity: 12288 entries (48 KB, three
-// workgroups per CU) when the table fits, else 16384 (64 KB); the clamped
-// table of huge boxes holds the reference's 8192 entries.
-static const void *far_kernel(bool clamp, int tab_len) {
-    if (clamp) return reinterpret_cast<const void *>(&k_far<true, AG_NDIEL>);
-    if (tab_len <= 12288) return reinterpret_cast<const void *>(&k_far<false, 12288>);
-    return reinterpret_cast<const void *>(&k_far<false, kFarTabMax>);
+// Far-kernel variants by LDS table capacity: 12288 entries (48 KB) when the
+// table fits, else kFarTabMax (64 KB).
+// Variants by LDS table capacity (12288 entries, 48 KB, when the table fits,
+// else kFarTabMax, 64 KB) and by the conformations served (tfree: those of
+// ConfDev::far_const).
+static const void *far_kernel(const FarParams &fp, bool tfree) {
+    if (fp.tab_len <= 12288)
+        return tfree ? reinterpret_cast<const void *>(&k_far<12288, true>)
+                     : reinterpret_cast<const void *>(&k_far<12288, false>);
+    return tfree ? reinterpret_cast<const void *>(&k_far<kFarTabMax, true>)
+                 : reinterpret_cast<const void *>(&k_far<kFarTabMax, false>);
 }
 
-static void launch_far(bool clamp, int tab_len, int blocks, const FarParams &fp, hipStream_t st) {
-    if (clamp) k_far<true, AG_NDIEL><<<blocks, kFarThreads, 0, st>>>(fp);
-    else if (tab_len <= 12288) k_far<false, 12288><<<blocks, kFarThreads, 0, st>>>(fp);
-    else k_far<false, kFarTabMax><<<blocks, kFarThreads, 0, st>>>(fp);
+static void launch_far(int blocks, const FarParams &fp, bool tfree, hipStream_t st) {
+    if (fp.tab_len <= 12288) {
+        if (tfree) k_far<12288, true><<<blocks, kFarThreads, 0, st>>>(fp);
+        else k_far<12288, false><<<blocks, kFarThreads, 0, st>>>(fp);
+    } else {
+        if (tfree) k_far<kFarTabMax, true><<<blocks, kFarThreads, 0, st>>>(fp);
+        else k_far<kFarTabMax, false><<<blocks, kFarThreads, 0, st>>>(fp);
+    }
     HIP_CHECK(hipGetLastError());
 }
 
 // Per-conformation geometry, work prefixes and cell arrays for the plan's
 // cell size.
-static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms) {
+// far_flags[c] (or all zero when null): bit 0 the far kernel's table-free
+// path for conformation c, bit 1 its table-index cap.
+static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms, const int *far_flags) {
     const int n = p.nconf;
     const double cs = p.tune.cell_size;
     p.confs.assign(n, ConfDev{});
@@ -1548,6 +1705,8 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms) {
         cf.pt_off = poff;
         cf.runs_per_row = (g.n1[0] + kFarK - 1) / kFarK;
         cf.nruns = cf.runs_per_row * g.n1[1] * g.n1[2];
+        cf.far_const = far_flags ? far_flags[c] & 1 : 0;
+        cf.far_clamp = far_flags ? far_flags[c] >> 1 & 1 : 0;
         p.brick_prefix[c] = (int)brick;
         p.tile_prefix[c] = (int)tile;
         brick += (long long)cf.nb[0] * cf.nb[1] * cf.nb[2];
@@ -1590,7 +1749,7 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms) {
     p.far.nconf = n;
     p.far.total_tiles = p.total_tiles;
     int occ = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, far_kernel(p.far_clamp, p.far.tab_len),
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, far_kernel(p.far, false),
                                                            kFarThreads, 0));
     p.far_grid = std::max(1, std::min(p.total_tiles, std::max(occ, 1) * p.ctx->n_cu));
 }
@@ -1743,7 +1902,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     }
     p.nplain = nplain_max <= 4 ? 4 : (nplain_max <= 8 ? 8 : 12);
     float *d_tab = upload(st, p.tab, gtab.data(), gtab.size());
-    p.queue_d.get(sizeof(int) * kMaxChunks);
+    p.queue_d.get(sizeof(int) * kQueueSlots);
 
     NearParams np;
     memset(&np, 0, sizeof np);
@@ -1822,15 +1981,28 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     // zeroed, long enough for the largest bin any (point, atom) pair of any
     // conformation reaches
     p.run_far = !vina && ms->elec_index >= 0 && N > 0;
+    std::vector<int> far_flags;
     if (p.run_far) {
         float4 *d_far = (float4 *)p.far_atoms.get(sizeof(float4) * (size_t)N);
         k_far_atoms<<<(N + 255) / 256, 256, 0, st>>>(N, d_coord, d_charge, ms->coeff_estat, d_far);
         HIP_CHECK(hipGetLastError());
         p.far_atoms_ready = true;
-        double rmax = 0.0;
+        // Per conformation: the largest bin any of its pairs reaches
+        // (need) and the share of its pairs past bin 8192 (r >= 64 A),
+        // from a 4 x 4 x 4 lattice of its box against at most 512 of its
+        // atoms.  The table-free path pays for its per-block atom test only
+        // when that share is large, and it is the one path that caps table
+        // indices, so a conformation that may pass the table's end takes it.
+        // Both depend on the conformation alone: its maps are the same in a
+        // batch as in a plan of its own.
+        long long need_max = 0;
+        far_flags.assign(n, 0);
         for (int i = 0, o = 0; i < n; o += natoms[i], i++) {
             const ag_grid &grid = grids[i];
-            for (int a = o; a < o + natoms[i]; a++) {
+            const int na = natoms[i];
+            if (na == 0) continue;
+            double rmax = 0.0;
+            for (int a = o; a < o + na; a++) {
                 double q = 0.0;
                 for (int d = 0; d < 3; d++) {
                     const double lo = -grid.ne[d] * grid.spacing, hi = (grid.n1[d] - 1 - grid.ne[d]) * grid.spacing;
@@ -1840,10 +2012,34 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
                 }
                 rmax = std::max(rmax, q);
             }
+            const long long need = (long long)(std::sqrt(rmax) * 128.0) + 8;
+            need_max = std::max(need_max, need);
+            const int stride = std::max(1, na / 512);
+            long long far_n = 0, tot_n = 0;
+            for (int qz = 0; qz < 4; qz++)
+                for (int qy = 0; qy < 4; qy++)
+                    for (int qx = 0; qx < 4; qx++) {
+                        const int q[3] = {qx, qy, qz};
+                        double pnt[3];
+                        for (int d = 0; d < 3; d++)
+                            pnt[d] = ((grid.n1[d] - 1) * (q[d] + 0.5) / 4.0 - grid.ne[d]) * grid.spacing;
+                        for (int a = o; a < o + na; a += stride) {
+                            double r2 = 0.0;
+                            for (int d = 0; d < 3; d++) {
+                                const double t = coord[3 * (size_t)a + d] - pnt[d];
+                                r2 += t * t;
+                            }
+                            far_n += r2 >= 64.0 * 64.0;
+                            tot_n++;
+                        }
+                    }
+            const bool clamp = need > kFarTabMax;
+            const bool tfree = clamp || (double)far_n > 0.10 * (double)tot_n;
+            far_flags[i] = (tfree ? 1 : 0) | (clamp ? 2 : 0);
         }
-        const long long need = (long long)(std::sqrt(rmax) * 128.0) + 8;
-        p.far_clamp = need > kFarTabMax;
-        const int tab_len = p.far_clamp ? AG_NDIEL : (int)std::max<long long>(need, kFarBinMin + 1);
+        // The table runs to the largest bin any pair reaches, up to
+        // kFarTabMax entries; entries past bin 8191 all hold r_eps[8191].
+        const int tab_len = (int)std::min<long long>(std::max<long long>(need_max, kFarBinMin + 1), kFarTabMax);
         std::vector<float> ft(tab_len);
         for (int k = 0; k < tab_len; k++) ft[k] = k < kFarBinMin ? 0.f : reps[std::min(k, AG_NDIEL - 1)];
         float *d_ft = upload(st, p.fartab, ft.data(), ft.size());
@@ -1853,9 +2049,10 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         fp.tab = d_ft;
         fp.tab_len = tab_len;
         fp.tmax4 = 4u * (unsigned)(tab_len - 1);
+        fp.c_last = reps[AG_NDIEL - 1];
         fp.e_slot = ms->elec_index;
     }
-    plan_geometry(p, grids, natoms.data());
+    plan_geometry(p, grids, natoms.data(), p.run_far ? far_flags.data() : nullptr);
     HIP_CHECK(hipStreamSynchronize(st));   // host staging vectors go out of scope
     return pl.release();
 }
@@ -1924,8 +2121,8 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
     if (nchunks < 1 || nchunks > kMaxChunks) ag::fail(AG_ERR_BAD_INPUT, "bad chunking of the batch");
     const unsigned cgrid = (unsigned)((p->total_cells + tpb - 1) / tpb);
     if (events) HIP_CHECK(hipEventRecord(c->ev[0], st));
-    k_zero_cells<<<std::max(cgrid, (unsigned)((kMaxChunks + tpb - 1) / tpb)), tpb, 0, st>>>(
-        d_count, d_cursor, p->total_cells, d_status, d_queue, kMaxChunks);
+    k_zero_cells<<<std::max(cgrid, (unsigned)((kQueueSlots + tpb - 1) / tpb)), tpb, 0, st>>>(
+        d_count, d_cursor, p->total_cells, d_status, d_queue, kQueueSlots);
     if (N > 0)
         k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>((const ConfDev *)p->confs_d.ptr,
                                                           p->nconf > 1 ? (const int *)p->atom_conf_d.ptr : nullptr, N,
@@ -2006,9 +2203,14 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
             fp.maps = d_maps;
             fp.item_base = t0;
             fp.item_end = t1;
-            fp.queue = p->persistent ? d_queue + k : nullptr;
             const int blocks = p->persistent ? std::min(p->far_grid, t1 - t0) : t1 - t0;
-            launch_far(p->far_clamp, fp.tab_len, blocks, fp, st);
+            bool has[2] = {false, false};
+            for (int ci = c0; ci < c1; ci++) has[p->confs[ci].far_const] = true;
+            for (int v = 0; v < 2; v++) {
+                if (!has[v]) continue;
+                fp.queue = p->persistent ? d_queue + 2 * k + v : nullptr;
+                launch_far(blocks, fp, v == 1, st);
+            }
         }
         if (p->want_collide)
             for (int ci = c0; ci < c1; ci++) {

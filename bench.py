@@ -468,6 +468,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-conformation batched comparison")
     ap.add_argument("--no-analytic", action="store_true", help="skip the analytic forward/adjoint leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 20,000-atom, 257^3 map set")
+    ap.add_argument("--fit-steps", type=int, default=200, help="Adam steps of the analytic leg's fit (0: none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -575,7 +576,7 @@ def main():
 
     large = None if args.no_large else large_leg()
     batch = None if args.no_batch else batched_leg()
-    analytic = None if args.no_analytic else analytic_leg()
+    analytic = None if args.no_analytic else analytic_leg(fit_steps=args.fit_steps)
     cpu = {"cpu_baseline": None, "accuracy": None}
     if not args.no_cpu_baseline and args.workload == "full":
         cpu = cpu_leg(st, gpu_maps, info["maps"], large=not args.no_large)

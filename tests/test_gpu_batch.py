@@ -49,6 +49,43 @@ def test_batch_equals_plain_bitwise(ensemble, cube):
             assert np.array_equal(g.view(np.uint32), ref.view(np.uint32))
 
 
+def _wide_box(st, spacing=2.0, npts=80):
+    """The conformation on a box of +-80 A: most of its pairs lie past
+    64 A, and its corners reach past the far kernel's largest table."""
+    return dataclasses.replace(st, spacing=spacing, n1=(npts + 1,) * 3, ne=(npts // 2,) * 3, _keep=[])
+
+
+def test_batch_mixed_far_paths_equal_plain(ensemble):
+    """A batch whose conformations take different far-kernel paths: small
+    boxes go through the dielectric table atom by atom, wide boxes take the
+    table-free path for atoms past bin 8192 of a wave's box and cap their
+    table indices.  Each conformation's path depends on it alone, so every
+    conformation still equals its own ag_maps_plain bit for bit; the wide
+    boxes are also held to the oracle."""
+    print("[gpu] batch with small and wide boxes vs plain", flush=True)
+    sts = [ensemble[0], _wide_box(ensemble[1]), ensemble[2], _wide_box(ensemble[3])]
+    wide = sts[1]
+    # precondition: a large share of the wide box's pairs lies past 64 A,
+    # and its corners pass bin 16380
+    ax = (np.arange(8) + 0.5) / 8 * (wide.n1[0] - 1)
+    pts = (np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3) - wide.ne[0]) * wide.spacing
+    r = np.linalg.norm(pts[:, None, :] - wide.coord[None, ::7, :], axis=-1)
+    assert (r >= 64.0).mean() > 0.3
+    corner = np.array([wide.ne[0] * wide.spacing] * 3)
+    assert np.linalg.norm(np.abs(wide.coord) + corner, axis=1).max() * 128 > 16380
+    with ag.Context(0) as ctx:
+        got = ctx.maps_batch(sts, tune={"cell_size": 4.0, "cube": True})
+        for i, (st, g) in enumerate(zip(sts, got)):
+            ref = ctx.maps_plain(st)
+            assert np.array_equal(g.view(np.uint32), ref.view(np.uint32)), f"conformation {i}"
+    for st, g in ((sts[1], got[1]), (sts[3], got[3])):
+        for z in (0, st.n1[2] // 2):
+            ref = ol.maps(st, z, z + 1)[:, z]
+            for k in range(st.n_maps):
+                ok, mx, rms, bad = parity.within(g[k, z], ref[k])
+                assert ok, f"map {k} plane {z}: max {mx}, {bad} points over tolerance"
+
+
 def test_batch_other_cells_against_oracle(ensemble):
     print("[gpu] batch cell 3.0 vs oracle", flush=True)
     with ag.Context(0) as ctx:

@@ -318,6 +318,15 @@ class Context:
         check(lib().ag_maps_plain(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
                                   C.c_void_p(out_ptr), None, None, 1))
 
+    def maps_plain_noalloc(self, st: PreparedState, out_ptr: int, fmap_ptr: int | None = None,
+                           cmap_ptr: int | None = None) -> None:
+        """The map set into device memory at out_ptr ([n_maps][P] float32)
+        with no device allocation (ag_maps_plain_noalloc): every buffer
+        comes from the device's fixed scratch array."""
+        rec, grid, ms, tab, _ = st.structs()
+        check(lib().ag_maps_plain_noalloc(self._h, C.byref(rec), C.byref(grid), C.byref(ms), C.byref(tab),
+                                          C.c_void_p(out_ptr), C.c_void_p(fmap_ptr), C.c_void_p(cmap_ptr)))
+
     def batch(self, states: list, tune: dict | None = None, potential: int = _abi.AG_POTENTIAL_AD4) -> "Plan":
         """Upload many conformations sharing states[0]'s map set and tables
         (ag_batch_create); tune None runs the autotuner."""

@@ -29,6 +29,7 @@ AG_POTENTIAL_AD4 = 0
 AG_POTENTIAL_VINA = 1
 AG_NDIEL = 8192
 AG_ERR_UNSUPPORTED = 6
+AG_NOALLOC_SCRATCH_BYTES = 32 << 20    # include/autogrid_amd.h
 
 
 class AgError(RuntimeError):
@@ -160,6 +161,9 @@ def lib() -> C.CDLL:
         L.ag_vina_tables.argtypes = [C.POINTER(Setup), C.POINTER(MapSet), vp, vp, vp]
         L.ag_job_prepare_vina.argtypes = [vp]
         L.ag_maps_vina.argtypes = L.ag_maps_plain.argtypes
+        L.ag_maps_plain_noalloc.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
+                                            C.POINTER(Tables), vp, vp, vp]
+        L.ag_maps_plain_noalloc.restype = C.c_int
         L.ag_plan_create_potential.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
                                                C.POINTER(Tables), C.c_int, C.POINTER(vp)]
         L.ag_build_tables.argtypes = [C.POINTER(Setup), C.POINTER(MapSet), vp, vp, vp]

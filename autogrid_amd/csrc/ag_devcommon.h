@@ -30,13 +30,33 @@ constexpr int kBatchTaper = 3;                    // of which the last ones halv
 // the next plan: a run of plans (one per receptor, or one per batch) then
 // allocates from HIP once.  Best fit among blocks at most twice the request
 // (plus 1 MiB); at most kMaxHeld bytes are held, the rest is freed.
+//
+// Arena form (DevPool(base, size)): a fixed device block the pool does not
+// own, handed out front to back in 256-byte steps and never returned or
+// freed; a request past its end fails with AG_ERR_UNSUPPORTED.  Buffers
+// drawing on it therefore never call hipMalloc or hipFree.
 struct DevPool {
     static constexpr size_t kMaxHeld = size_t(16) << 30;
     std::mutex mu;
     std::vector<std::pair<size_t, void *>> blocks;   // (capacity, pointer)
     size_t held = 0;
+    char *arena = nullptr;
+    size_t arena_size = 0, arena_used = 0;
+    DevPool() = default;
+    DevPool(void *base, size_t size) : arena(static_cast<char *>(base)), arena_size(size) {}
     void *take(size_t bytes, size_t &cap) {
         std::lock_guard<std::mutex> lk(mu);
+        if (arena) {
+            const size_t want = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+            if (want > arena_size - arena_used)
+                ag::fail(AG_ERR_UNSUPPORTED, "the map set needs more device memory than the " +
+                                                 std::to_string(arena_size >> 20) +
+                                                 " MiB scratch of the allocation-free entry point holds");
+            void *p = arena + arena_used;
+            arena_used += want;
+            cap = want;
+            return p;
+        }
         int best = -1;
         for (int i = 0; i < (int)blocks.size(); i++) {
             const size_t c = blocks[i].first;
@@ -51,6 +71,7 @@ struct DevPool {
     }
     void give(void *p, size_t cap) {
         std::lock_guard<std::mutex> lk(mu);
+        if (arena) return;
         if (held + cap > kMaxHeld) {
             (void)hipFree(p);
             return;
@@ -145,6 +166,7 @@ struct ag_ctx {
     DevBuf maps;                       // output staging of ag_maps_plain
     HostStage stage;                   // receptor arrays of a plan on their way up
     HostStage status_word;             // pinned target of a plan's status flag after each map set
+    void *scratch = nullptr;           // this device's allocation-free scratch (ag_maps_plain_noalloc)
     hipEvent_t ev[4] = {};
     float last_ms[4] = {0, 0, 0, 0};   // cell list, near, far, total of the last execute
     int n_cu = 0;

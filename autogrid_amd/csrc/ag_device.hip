@@ -43,6 +43,12 @@
 #include "ag_internal.h"
 #include "ag_devcommon.h"
 
+// Device scratch of the allocation-free entry point (ag_maps_plain_noalloc):
+// one array per GPU, present from the library's load until the process
+// ends.  Host code never uses this name as an address: each device's
+// address comes from hipGetSymbolAddress at context creation (ag_ctx::scratch).
+static __device__ __align__(256) unsigned char scratch[AG_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 constexpr int kBX = 8, kBY = 8, kBZ = 4;          // brick of the near kernel
@@ -1511,6 +1517,21 @@ struct ag_plan {
 
 namespace agdev {
 
+// The device address of `scratch` on `device` (current), looked up once
+// per device and kept: every GPU holds its own copy of the array.
+static void *scratch_address(int device) {
+    static std::mutex mu;
+    static std::vector<void *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+    if (!addr[device]) {
+        void *p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[device] = p;
+    }
+    return addr[device];
+}
+
 ag_status context_create(ag_ctx **out, int device) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
@@ -1526,6 +1547,7 @@ ag_status context_create(ag_ctx **out, int device) {
     for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
     for (auto &e : c->chunk_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    c->scratch = scratch_address(device);
     *out = c;
     return AG_OK;
 }
@@ -1755,14 +1777,19 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms, c
 }
 
 // Build a plan over n conformations (a single plan is n = 1).
+// An arena pool (the allocation-free entry point) also keeps the host
+// staging in pageable memory instead of the context's pinned buffer.
 static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids, const ag_mapset *ms,
-                           const ag_tables *tab, int potential, bool persistent, const Tune &tune) {
+                           const ag_tables *tab, int potential, bool persistent, const Tune &tune,
+                           std::shared_ptr<DevPool> pool = nullptr) {
     const bool vina = potential == AG_POTENTIAL_VINA;
     if (potential != AG_POTENTIAL_AD4 && !vina) ag::fail(AG_ERR_BAD_INPUT, "unknown potential");
     if (n < 1) ag::fail(AG_ERR_BAD_INPUT, "a plan needs at least one conformation");
     for (int i = 0; i < n; i++) validate(&recs[i], &grids[i], ms, tab, vina);
     HIP_CHECK(hipSetDevice(c->device));
-    std::unique_ptr<ag_plan> pl(new ag_plan(c->pool));
+    if (!pool) pool = c->pool;
+    const bool no_alloc = pool->arena != nullptr;
+    std::unique_ptr<ag_plan> pl(new ag_plan(pool));
     ag_plan &p = *pl;
     p.ctx = c;
     p.nconf = n;
@@ -1792,7 +1819,8 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     const size_t o_coord = carve(8 * n3), o_charge = carve(8 * nn), o_vol = carve(8 * nn), o_solpar = carve(8 * nn);
     const size_t o_rvec = carve(8 * n3), o_rvec2 = carve(8 * n3), o_type = carve(4 * nn), o_hbond = carve(4 * nn);
     const size_t o_rexp = carve(4 * nn), o_conf = carve(4 * nn), o_dis = carve(nn);
-    char *const h = (char *)c->stage.get(off + 256);
+    std::vector<char> pageable(no_alloc ? off + 256 : 0);
+    char *const h = no_alloc ? pageable.data() : (char *)c->stage.get(off + 256);
     double *const coord = (double *)(h + o_coord), *const charge = (double *)(h + o_charge);
     double *const vol = (double *)(h + o_vol), *const solpar = (double *)(h + o_solpar);
     double *const rvec = (double *)(h + o_rvec), *const rvec2 = (double *)(h + o_rvec2);
@@ -2252,8 +2280,10 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
 }
 
 static void check_status(ag_plan *p) {
-    // into pinned memory: a direct copy, not a staged one
-    int *status = (int *)p->ctx->status_word.get(sizeof(int));
+    // into pinned memory: a direct copy, not a staged one (a plan on an
+    // arena pool allocates no pinned word and copies into the stack)
+    int local = 0;
+    int *status = p->pool->arena ? &local : (int *)p->ctx->status_word.get(sizeof(int));
     HIP_CHECK(hipMemcpyAsync(status, p->status.ptr, sizeof(int), hipMemcpyDeviceToHost, p->ctx->stream));
     HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
     if (*status & 1) ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
@@ -2420,6 +2450,19 @@ void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
 // memory as soon as the kernels are done and on_ready(j) is called for it
 // (j = n_maps for fmap, n_maps + 1 for cmap), so the caller can format and
 // write map j while later maps are still in flight.
+// ag_maps_plain with device outputs, every device buffer drawn from this
+// device's scratch array: a plan on an arena pool over it, built, run and
+// dropped.  Nothing is allocated on the device or pinned on the host.
+void maps_plain_noalloc(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                        const ag_tables *tab, float *maps, float *fmap, int32_t *cmap) {
+    HIP_CHECK(hipSetDevice(c->device));
+    if (!c->scratch) ag::fail(AG_ERR_DEVICE, "no scratch address for this device");
+    auto arena = std::make_shared<DevPool>(c->scratch, (size_t)AG_NOALLOC_SCRATCH_BYTES);
+    std::unique_ptr<ag_plan, void (*)(ag_plan *)> p(
+        plan_build(c, 1, rec, grid, ms, tab, AG_POTENTIAL_AD4, false, Tune(), arena), plan_destroy);
+    plan_execute(p.get(), maps, ms->floating_grid ? fmap : nullptr, ms->constriction_grid ? cmap : nullptr);
+}
+
 void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                          const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap,
                          const std::function<void(int)> &on_ready, int potential) {

@@ -213,6 +213,10 @@ void plan_audit(ag_plan *p, long long out[4]);
 void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                          const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap,
                          const std::function<void(int)> &on_ready, int potential = AG_POTENTIAL_AD4);
+// ag_maps_plain with device outputs, every device buffer drawn from the
+// device's scratch array (no allocation).
+void maps_plain_noalloc(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                        const ag_tables *tab, float *maps, float *fmap, int32_t *cmap);
 void maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                 const ag_mapset *ms, const ag_tables *tab, float *maps,
                 float *fmap, int32_t *cmap, int outputs_on_device, int potential = AG_POTENTIAL_AD4);

@@ -210,6 +210,15 @@ ag_status ag_maps_plain(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid
     return guarded([&] { agdev::maps_plain(ctx, rec, grid, ms, tab, maps, fmap, cmap, outputs_on_device); });
 }
 
+ag_status ag_maps_plain_noalloc(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
+                                const ag_tables *tab, float *maps, float *fmap, int32_t *cmap) {
+    if (!ctx || !rec || !grid || !ms || !tab || !maps) {
+        ag::set_last_error("null argument to ag_maps_plain_noalloc");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] { agdev::maps_plain_noalloc(ctx, rec, grid, ms, tab, maps, fmap, cmap); });
+}
+
 ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                          const ag_tables *tab, ag_plan **out) {
     if (!ctx || !rec || !grid || !ms || !tab || !out) {

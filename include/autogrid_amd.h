@@ -240,6 +240,25 @@ ag_status ag_maps_plain(ag_ctx *ctx, const ag_receptor *rec,
                         const ag_tables *tables, float *maps, float *fmap,
                         int32_t *cmap, int outputs_on_device);
 
+/* Allocation-free map set.  No counterpart in the reference: a variant of
+ * ag_maps_plain with device outputs (outputs_on_device = 1) for callers
+ * that cannot allocate device or pinned memory -- serving loops and
+ * harnesses that forbid hipMalloc.  Every device buffer of the map set (the
+ * receptor arrays, cell list, tables, far-kernel atoms and status flag)
+ * comes from one device array of AG_NOALLOC_SCRATCH_BYTES that each GPU
+ * holds from the library's load until the process ends; the call makes no
+ * hipMalloc, hipFree or pinned-memory allocation.  A map set that needs
+ * more than the array holds returns AG_ERR_UNSUPPORTED with the limit in
+ * ag_last_error() and leaves the outputs untouched.  Inputs are host
+ * pointers as for ag_maps_plain; maps_dev / fmap_dev / cmap_dev are device
+ * pointers.  The call returns after the outputs are written.  All calls on
+ * one GPU share its array, so calls that overlap on one GPU (from several
+ * threads or contexts) race; the array keeps the last call's contents. */
+#define AG_NOALLOC_SCRATCH_BYTES (32u << 20)
+ag_status ag_maps_plain_noalloc(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
+                                const ag_mapset *mapset, const ag_tables *tables, float *maps_dev,
+                                float *fmap_dev, int32_t *cmap_dev);
+
 /* Device-resident form of the same computation: ag_plan_create uploads the
  * receptor, tables and map set to ctx's device once; each ag_plan_execute
  * then builds the cell list and runs the near-field and far-field kernels

@@ -1232,6 +1232,7 @@ __device__ __forceinline__ float sreg(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+
 __device__ __forceinline__ int wave_min(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
     return v;
@@ -1254,8 +1255,11 @@ __device__ __forceinline__ int wave_max(int v) {
 // compensated sums in LDS and the first adds them in a fixed order, so the
 // result does not depend on scheduling.  Splitting the atoms keeps four
 // waves per SIMD on grids that give only two per SIMD at one wave per run.
-// Run order: x-run outermost, rows in 8 x 8 patches inside (far_row), so
-// the 64 runs of a wave lie in a small box.
+// Run order: a wave takes one x-run of a patch of 64 rows (8 x 8 in y and
+// z, far_row), so its 64 runs lie in a small box; the patch's x-runs go to
+// consecutive waves, so the four waves of a half cover consecutive 64-byte
+// pieces of the same 64 rows, and their loads and stores of the e map meet
+// in the same cache lines at about the same time.
 //
 // Per conformation (ConfDev::far_const, so a conformation's maps are the
 // same in every plan; a launch serves the conformations of its TFREE and
@@ -1296,15 +1300,17 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
         const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
         if (cf.far_const != (int)TFREE) goto next_item;   // uniform per workgroup
         {
-        const int run = (item - as_const(p.tile_prefix)[ci]) * kFarRuns + lane;
-        const bool active = run < cf.nruns;
         const int n1x = cf.n1[0], n1y = cf.n1[1];
+        // wave slot (uniform): row patch pr, x-run xr; lane: row pr * 64 + l
+        const int ws = __builtin_amdgcn_readfirstlane((item - as_const(p.tile_prefix)[ci]) * (kFarRuns / 64) +
+                                                      lane / 64);
+        const int pr = ws / cf.runs_per_row;
+        const int q = pr * 64 + lane % 64;
+        const bool active = q < n1y * cf.n1[2];
         int ix0 = 0, iy = 0, iz = 0;
         if (active) {
-            const int rows = n1y * cf.n1[2];
-            const int xr = run / rows;   // x-run outermost, rows in patches inside it
-            ix0 = xr * kFarK;
-            far_row(run - xr * rows, n1y, cf.n1[2], iy, iz);
+            ix0 = (ws - pr * cf.runs_per_row) * kFarK;
+            far_row(q, n1y, cf.n1[2], iy, iz);
         }
         // the wave's box of points, in grid indices
         const int bx0 = __builtin_amdgcn_readfirstlane(wave_min(active ? ix0 : INT_MAX));
@@ -1732,7 +1738,9 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms, c
         p.brick_prefix[c] = (int)brick;
         p.tile_prefix[c] = (int)tile;
         brick += (long long)cf.nb[0] * cf.nb[1] * cf.nb[2];
-        tile += (cf.nruns + kFarRuns - 1) / kFarRuns;
+        // far-kernel items: patches of 64 rows x x-runs, kFarRuns / 64 wave slots each
+        const long long far_waves = (long long)((g.n1[1] * g.n1[2] + 63) / 64) * cf.runs_per_row;
+        tile += (far_waves + kFarRuns / 64 - 1) / (kFarRuns / 64);
         const long long cells = (long long)cf.nc[0] * cf.nc[1] * cf.nc[2];
         if (cell_base + cells > INT32_MAX / 2 || brick > INT32_MAX / 2 || tile > INT32_MAX / 2)
             ag::fail(AG_ERR_UNSUPPORTED, "batch too large for one plan");
@@ -2286,7 +2294,9 @@ static void check_status(ag_plan *p) {
     int *status = p->pool->arena ? &local : (int *)p->ctx->status_word.get(sizeof(int));
     HIP_CHECK(hipMemcpyAsync(status, p->status.ptr, sizeof(int), hipMemcpyDeviceToHost, p->ctx->stream));
     HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
-    if (*status & 1) ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms within 8 A of one 8x8x4 brick of grid points");
+    if (*status & 1)
+        ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms, or more than 256 donor hydrogens, within 8.03 A of "
+                                     "one 8x8x4 brick of grid points");
 }
 
 // Autotuner: times the near-field pass of every candidate layout on a

@@ -242,12 +242,6 @@ def batched_leg(reps: int = 2) -> dict:
                     tables = ag.build_tables(st)
                     ctx.maps_plain(with_bonds(st, tables), out=out)
 
-            def batched(tune):
-                ev, sol, reps_ = ag.build_tables(inputs[0])
-                states = ag.bond_vectors_batch(inputs)
-                states = [dataclasses.replace(st, e_vdw_hb=ev, sol_fn=sol, r_eps=reps_, _keep=[]) for st in states]
-                ctx.maps_batch(states, tune=tune, outs=outs)
-
             def batch_states():
                 ev, sol, reps_ = ag.build_tables(inputs[0])
                 states = ag.bond_vectors_batch(inputs)

@@ -322,3 +322,45 @@ def test_closest_donor_h_ties_take_the_lower_index(ctx, tmp_path):
     for k in range(st.n_maps):
         ok, mx, rms, bad = parity.within(got[k], ref[k])
         assert ok, f"map {k}: max {mx} rms {rms} bad {bad}"
+
+
+def test_overfull_brick_is_refused_and_the_context_recovers(ctx, tmp_path):
+    """The near kernel stages at most 2,048 candidate atoms (and 256 donor
+    H) per 8x8x4 brick, about ten times protein density over the brick's
+    8.03 A staging volume.  Past that the map set fails with
+    AG_ERR_UNSUPPORTED and a message naming the limit (every kernel still
+    runs to its end; the host reads the flag).  The same context then
+    computes an ordinary map set that matches the oracle."""
+    import dataclasses
+    print("[gpu] overfull brick refused", flush=True)
+    gpf = cases.materialize("small", str(tmp_path))
+    path = os.path.join(tmp_path, gpf)
+    lines = open(path).read().split("\n")
+    lines = ["npts 10 10 10" if ln.startswith("npts") else ln for ln in lines]
+    open(path, "w").write("\n".join(lines))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        base = ag.Job(gpf).prepare().state()
+    finally:
+        os.chdir(cwd)
+    # 3,000 atoms: the receptor's 1,000 three times, packed into a 6 A ball
+    pick = np.tile(np.arange(len(base.coord)), 3)
+    rng = np.random.default_rng(11)
+    u = rng.normal(size=(len(pick), 3))
+    u *= (6.0 * rng.random(len(pick)) ** (1 / 3) / np.linalg.norm(u, axis=1))[:, None]
+    st = dataclasses.replace(
+        base, coord=np.ascontiguousarray(u), charge=base.charge[pick].copy(), vol=base.vol[pick].copy(),
+        solpar=base.solpar[pick].copy(), type=base.type[pick].copy(), hbond=base.hbond[pick].copy(),
+        rexp=base.rexp[pick].copy(), rvector=base.rvector[pick].copy(), rvector2=base.rvector2[pick].copy(),
+        disorder=base.disorder[pick].copy(), _keep=[])
+    with pytest.raises(ag.AgError) as e:
+        ctx.maps_plain(st)
+    assert e.value.status == ag._abi.AG_ERR_UNSUPPORTED
+    assert "2048" in str(e.value)
+    got = ctx.maps_plain(base)
+    z = base.n1[2] // 2
+    ref = ol.maps(ol.oracle_state(base), z, z + 1)[:, z]
+    for k in range(base.n_maps):
+        ok, mx, rms, bad = parity.within(got[k, z], ref[k])
+        assert ok, f"map {k}: max {mx} rms {rms} bad {bad}"

@@ -22,7 +22,6 @@
     } while (0)
 
 constexpr int kMaxChunks = 16;                    // pipeline stages of a batch with host outputs (at most)
-constexpr int kQueueSlots = 2 * kMaxChunks;       // far-kernel work counters: two launches per stage
 constexpr int kBatchChunks = 10;                  // chunks a batch with host outputs is cut into
 constexpr int kBatchTaper = 3;                    // of which the last ones halve in size
 

@@ -200,6 +200,19 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
     return x * q + min(x, r) + k;
 }
 
+// Consecutive workgroup ids go round-robin over the 8 XCDs; here XCD x takes
+// chunks x, x + 8, x + 16, ... of C consecutive items, so that every XCD
+// gets items from all over the grid (the work per item varies with the
+// local atom density) while the items of a chunk share its L2.  Ids past the
+// last whole round of 8 chunks map to themselves.
+template <int C>
+__device__ __forceinline__ int xcd_chunked(int bid, int nblocks) {
+    const int full = nblocks / (8 * C) * (8 * C);
+    if (bid >= full) return bid;
+    const int x = bid % 8, k = bid / 8;
+    return ((k / C) * 8 + x) * C + k % C;
+}
+
 // Table offset of the far kernel's bin: with r = rb2 * rsq(rb2) (128 r in
 // bins), y = fma(rb2, rsq(rb2), 2^21 - 1/8) lies in [2^21, 2^22), where the
 // float spacing is 1/4, so the low 23 bits of y hold m = round(4 r - 1/2)
@@ -523,11 +536,14 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     int lx, ly, lz;
     brick_lane<CUBE>(tid, lx, ly, lz);
 
-    // one workgroup per brick of all conformations.  (A persistent loop
+    // one workgroup per brick of all conformations, in chunks of 16 bricks
+    // spread over the XCDs: a brick's time follows the local atom density,
+    // and contiguous eighths of the grid per XCD left the XCDs that hold a
+    // protein's dense core finishing last.  (A persistent loop
     // over a work counter keeps more uniform state live than the scalar
     // register file holds and spills; the far kernel, whose state is
     // small, is the persistent one.)
-    const int item = p.item_base + xcd_remap(blockIdx.x, gridDim.x);
+    const int item = p.item_base + xcd_chunked<16>(blockIdx.x, gridDim.x);
     if (item >= p.total_bricks) return;
     const int ci = find_conf(as_const(p.brick_prefix), p.nconf, item);
     const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
@@ -1262,10 +1278,10 @@ __device__ __forceinline__ int wave_max(int v) {
 // in the same cache lines at about the same time.
 //
 // Per conformation (ConfDev::far_const, so a conformation's maps are the
-// same in every plan; a launch serves the conformations of its TFREE and
-// passes over the others' items): with few pairs past bin 8192, every atom
-// goes through the table, in index order, one atom ahead, and the table
-// covers every bin.  For large boxes, for each block of 64 atoms the wave tests every
+// same in every plan; the branch is uniform per item, and the two paths'
+// registers spill only around the item loop): with few pairs past bin
+// 8192, every atom goes through the table, in index order, one atom ahead,
+// and the table covers every bin.  For large boxes, for each block of 64 atoms the wave tests every
 // atom's distance to its box at once (one atom per lane, one ballot).
 // Atoms at least kFarConstBins from the whole box reach only bins past
 // 8191, whose value is r_eps[8191], and take the table-free path; within a
@@ -1276,7 +1292,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // indices; past bin 8191 every entry is r_eps[8191], so the cap changes no
 // value.  Lanes past the conformation's runs
 // compute at a point of their wave's box and store nothing.
-template <int CAP, bool TFREE>
+template <int CAP>
 __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_eu(4))) k_far(FarParams p) {
     __shared__ float T[CAP + 1 + kFarRuns * kFarK];
     int &item_s = reinterpret_cast<int *>(T)[CAP];
@@ -1298,8 +1314,6 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
     while (item < p.item_end) {
         const int ci = find_conf(as_const(p.tile_prefix), p.nconf, item);
         const AG_CONST ConfDev &cf = as_const(p.confs)[ci];
-        if (cf.far_const != (int)TFREE) goto next_item;   // uniform per workgroup
-        {
         const int n1x = cf.n1[0], n1y = cf.n1[1];
         // wave slot (uniform): row patch pr, x-run xr; lane: row pr * 64 + l
         const int ws = __builtin_amdgcn_readfirstlane((item - as_const(p.tile_prefix)[ci]) * (kFarRuns / 64) +
@@ -1346,7 +1360,7 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
             const int a_mid = (n_atoms + 1) / 2;   // Kahan blocks of each half start at its first atom
             const int a_beg = half ? a_mid : 0;
             const int a_end = half ? n_atoms : a_mid;
-            if constexpr (!TFREE) {
+            if (!cf.far_const) {   // uniform per workgroup
                 PlainF4 nxt = ld4(atoms, min(a_beg, max(a_end - 1, 0)));
                 for (int a0 = a_beg; a0 < a_end; a0 += kFarAtomChunk) {
                     const int a1 = min(a0 + kFarAtomChunk, a_end);
@@ -1453,8 +1467,6 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
                 if (ix0 + k < n1x) emap[base + k] = t;
             }
         }
-        }
-    next_item:
         if (!p.queue) break;
         __syncthreads();
         if (threadIdx.x == 0) item_s = p.item_base + atomicAdd(p.queue, 1);
@@ -1674,25 +1686,16 @@ static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStr
 
 // Far-kernel variants by LDS table capacity: 12288 entries (48 KB) when the
 // table fits, else kFarTabMax (64 KB).
-// Variants by LDS table capacity (12288 entries, 48 KB, when the table fits,
-// else kFarTabMax, 64 KB) and by the conformations served (tfree: those of
-// ConfDev::far_const).
-static const void *far_kernel(const FarParams &fp, bool tfree) {
-    if (fp.tab_len <= 12288)
-        return tfree ? reinterpret_cast<const void *>(&k_far<12288, true>)
-                     : reinterpret_cast<const void *>(&k_far<12288, false>);
-    return tfree ? reinterpret_cast<const void *>(&k_far<kFarTabMax, true>)
-                 : reinterpret_cast<const void *>(&k_far<kFarTabMax, false>);
+// Variants by LDS table capacity: 12288 entries (48 KB) when the table
+// fits, else kFarTabMax (64 KB).
+static const void *far_kernel(const FarParams &fp) {
+    return fp.tab_len <= 12288 ? reinterpret_cast<const void *>(&k_far<12288>)
+                               : reinterpret_cast<const void *>(&k_far<kFarTabMax>);
 }
 
-static void launch_far(int blocks, const FarParams &fp, bool tfree, hipStream_t st) {
-    if (fp.tab_len <= 12288) {
-        if (tfree) k_far<12288, true><<<blocks, kFarThreads, 0, st>>>(fp);
-        else k_far<12288, false><<<blocks, kFarThreads, 0, st>>>(fp);
-    } else {
-        if (tfree) k_far<kFarTabMax, true><<<blocks, kFarThreads, 0, st>>>(fp);
-        else k_far<kFarTabMax, false><<<blocks, kFarThreads, 0, st>>>(fp);
-    }
+static void launch_far(int blocks, const FarParams &fp, hipStream_t st) {
+    if (fp.tab_len <= 12288) k_far<12288><<<blocks, kFarThreads, 0, st>>>(fp);
+    else k_far<kFarTabMax><<<blocks, kFarThreads, 0, st>>>(fp);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -1779,7 +1782,7 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms, c
     p.far.nconf = n;
     p.far.total_tiles = p.total_tiles;
     int occ = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, far_kernel(p.far, false),
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, far_kernel(p.far),
                                                            kFarThreads, 0));
     p.far_grid = std::max(1, std::min(p.total_tiles, std::max(occ, 1) * p.ctx->n_cu));
 }
@@ -1938,7 +1941,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     }
     p.nplain = nplain_max <= 4 ? 4 : (nplain_max <= 8 ? 8 : 12);
     float *d_tab = upload(st, p.tab, gtab.data(), gtab.size());
-    p.queue_d.get(sizeof(int) * kQueueSlots);
+    p.queue_d.get(sizeof(int) * kMaxChunks);
 
     NearParams np;
     memset(&np, 0, sizeof np);
@@ -2157,8 +2160,8 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
     if (nchunks < 1 || nchunks > kMaxChunks) ag::fail(AG_ERR_BAD_INPUT, "bad chunking of the batch");
     const unsigned cgrid = (unsigned)((p->total_cells + tpb - 1) / tpb);
     if (events) HIP_CHECK(hipEventRecord(c->ev[0], st));
-    k_zero_cells<<<std::max(cgrid, (unsigned)((kQueueSlots + tpb - 1) / tpb)), tpb, 0, st>>>(
-        d_count, d_cursor, p->total_cells, d_status, d_queue, kQueueSlots);
+    k_zero_cells<<<std::max(cgrid, (unsigned)((kMaxChunks + tpb - 1) / tpb)), tpb, 0, st>>>(
+        d_count, d_cursor, p->total_cells, d_status, d_queue, kMaxChunks);
     if (N > 0)
         k_cell_count<<<(N + tpb - 1) / tpb, tpb, 0, st>>>((const ConfDev *)p->confs_d.ptr,
                                                           p->nconf > 1 ? (const int *)p->atom_conf_d.ptr : nullptr, N,
@@ -2239,14 +2242,9 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
             fp.maps = d_maps;
             fp.item_base = t0;
             fp.item_end = t1;
+            fp.queue = p->persistent ? d_queue + k : nullptr;
             const int blocks = p->persistent ? std::min(p->far_grid, t1 - t0) : t1 - t0;
-            bool has[2] = {false, false};
-            for (int ci = c0; ci < c1; ci++) has[p->confs[ci].far_const] = true;
-            for (int v = 0; v < 2; v++) {
-                if (!has[v]) continue;
-                fp.queue = p->persistent ? d_queue + 2 * k + v : nullptr;
-                launch_far(blocks, fp, v == 1, st);
-            }
+            launch_far(blocks, fp, st);
         }
         if (p->want_collide)
             for (int ci = c0; ci < c1; ci++) {

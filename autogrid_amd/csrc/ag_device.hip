@@ -66,6 +66,7 @@ constexpr int kFarRuns = 256;                     // runs (lanes of kFarK points
 constexpr int kFarThreads = 2 * kFarRuns;         // two halves of the atom list per run
 constexpr int kFarTabMax = 16380;                 // LDS table entries without the clamp (two workgroups per CU)
 constexpr int kFarAtomChunk = 128;                // Kahan block
+enum : int { kFarTable = 0, kFarTfree = 1, kFarMixed = 2 };   // far-kernel paths of a launch
 constexpr int kFarBinMin = 1025;                  // far kernel takes bins >= this
 constexpr float kGuardBins = 1.0e-3f;             // FP64 recompute band around bin edges
 
@@ -1278,10 +1279,11 @@ __device__ __forceinline__ int wave_max(int v) {
 // in the same cache lines at about the same time.
 //
 // Per conformation (ConfDev::far_const, so a conformation's maps are the
-// same in every plan; the branch is uniform per item, and the two paths'
-// registers spill only around the item loop): with few pairs past bin
-// 8192, every atom goes through the table, in index order, one atom ahead,
-// and the table covers every bin.  For large boxes, for each block of 64 atoms the wave tests every
+// same in every plan; MODE kFarTable and kFarTfree serve launches whose
+// items all take one path, kFarMixed decides per item, uniformly per
+// workgroup, at the price of more spills around the item loop): with few
+// pairs past bin 8192, every atom goes through the table, in index order,
+// one atom ahead, and the table covers every bin.  For large boxes, for each block of 64 atoms the wave tests every
 // atom's distance to its box at once (one atom per lane, one ballot).
 // Atoms at least kFarConstBins from the whole box reach only bins past
 // 8191, whose value is r_eps[8191], and take the table-free path; within a
@@ -1292,7 +1294,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // indices; past bin 8191 every entry is r_eps[8191], so the cap changes no
 // value.  Lanes past the conformation's runs
 // compute at a point of their wave's box and store nothing.
-template <int CAP>
+template <int CAP, int MODE>
 __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_eu(4))) k_far(FarParams p) {
     __shared__ float T[CAP + 1 + kFarRuns * kFarK];
     int &item_s = reinterpret_cast<int *>(T)[CAP];
@@ -1360,7 +1362,8 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
             const int a_mid = (n_atoms + 1) / 2;   // Kahan blocks of each half start at its first atom
             const int a_beg = half ? a_mid : 0;
             const int a_end = half ? n_atoms : a_mid;
-            if (!cf.far_const) {   // uniform per workgroup
+            const bool tfree = MODE == kFarMixed ? cf.far_const != 0 : MODE == kFarTfree;
+            if (!tfree) {   // uniform per workgroup
                 PlainF4 nxt = ld4(atoms, min(a_beg, max(a_end - 1, 0)));
                 for (int a0 = a_beg; a0 < a_end; a0 += kFarAtomChunk) {
                     const int a1 = min(a0 + kFarAtomChunk, a_end);
@@ -1533,6 +1536,13 @@ struct ag_plan {
     }
 };
 
+// The far-kernel mode of conformations [c0, c1): one path for all, or mixed.
+static int far_mode(const ag_plan &p, int c0, int c1) {
+    int n_tfree = 0;
+    for (int ci = c0; ci < c1; ci++) n_tfree += p.confs[ci].far_const != 0;
+    return n_tfree == 0 ? kFarTable : n_tfree == c1 - c0 ? kFarTfree : kFarMixed;
+}
+
 namespace agdev {
 
 // The device address of `scratch` on `device` (current), looked up once
@@ -1686,17 +1696,22 @@ static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStr
 
 // Far-kernel variants by LDS table capacity: 12288 entries (48 KB) when the
 // table fits, else kFarTabMax (64 KB).
-// Variants by LDS table capacity: 12288 entries (48 KB) when the table
-// fits, else kFarTabMax (64 KB).
-static const void *far_kernel(const FarParams &fp) {
-    return fp.tab_len <= 12288 ? reinterpret_cast<const void *>(&k_far<12288>)
-                               : reinterpret_cast<const void *>(&k_far<kFarTabMax>);
+// Variants by LDS table capacity (12288 entries, 48 KB, when the table
+// fits, else kFarTabMax, 64 KB) and by the paths the launch's items take.
+template <int MODE>
+static const void *far_kernel_mode(const FarParams &fp) {
+    return fp.tab_len <= 12288 ? reinterpret_cast<const void *>(&k_far<12288, MODE>)
+                               : reinterpret_cast<const void *>(&k_far<kFarTabMax, MODE>);
+}
+static const void *far_kernel(const FarParams &fp, int mode) {
+    return mode == kFarTable ? far_kernel_mode<kFarTable>(fp)
+         : mode == kFarTfree ? far_kernel_mode<kFarTfree>(fp)
+                             : far_kernel_mode<kFarMixed>(fp);
 }
 
-static void launch_far(int blocks, const FarParams &fp, hipStream_t st) {
-    if (fp.tab_len <= 12288) k_far<12288><<<blocks, kFarThreads, 0, st>>>(fp);
-    else k_far<kFarTabMax><<<blocks, kFarThreads, 0, st>>>(fp);
-    HIP_CHECK(hipGetLastError());
+static void launch_far(int blocks, const FarParams &fp, int mode, hipStream_t st) {
+    void *args[] = {const_cast<FarParams *>(&fp)};
+    HIP_CHECK(hipLaunchKernel(far_kernel(fp, mode), dim3(blocks), dim3(kFarThreads), args, 0, st));
 }
 
 // Per-conformation geometry, work prefixes and cell arrays for the plan's
@@ -1782,7 +1797,7 @@ static void plan_geometry(ag_plan &p, const ag_grid *grids, const int *natoms, c
     p.far.nconf = n;
     p.far.total_tiles = p.total_tiles;
     int occ = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, far_kernel(p.far),
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, far_kernel(p.far, kFarMixed),
                                                            kFarThreads, 0));
     p.far_grid = std::max(1, std::min(p.total_tiles, std::max(occ, 1) * p.ctx->n_cu));
 }
@@ -2175,18 +2190,9 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
                                                       (NearAtom *)p->near_atoms.ptr);
     HIP_CHECK(hipGetLastError());
     if (events) HIP_CHECK(hipEventRecord(c->ev[1], st));
-    // With host outputs the chunks alternate between two compute streams,
-    // so one chunk's far-kernel tail overlaps the next chunk's work (chunks
-    // touch disjoint conformations; both streams follow the cell list).
-    const bool two = host && nchunks > 1;
-    if (two) {
-        HIP_CHECK(hipEventRecord(c->cells_ev, st));
-        HIP_CHECK(hipStreamWaitEvent(c->stream2, c->cells_ev, 0));
-    }
-    const hipStream_t st_main = st;
-    for (int k = 0; k < nchunks; k++) {
-        st = (two && (k & 1)) ? c->stream2 : st_main;
-        const int c0 = chunks[k].first, c1 = chunks[k].second;
+    // The near field (near kernel, floating grid, covalent and zero maps) of
+    // conformations [c0, c1) on stream s.
+    auto near_pass = [&](int c0, int c1, hipStream_t s) {
         const int b0 = p->brick_prefix[c0], b1 = p->brick_prefix[c1];
         for (int g = 0; g < p->groups; g++) {
             NearParams np = p->near[g];
@@ -2194,7 +2200,7 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
             np.cmap = (g == 0 && p->want_cmap) ? d_cmap : nullptr;
             np.item_base = b0;
             if (!(np.has_affinity || np.e_slot >= 0 || np.cmap) || b1 <= b0) continue;
-            launch_near(*p, b1 - b0, np, st);
+            launch_near(*p, b1 - b0, np, s);
         }
         for (int ci = c0; ci < c1; ci++) {
             const ConfDev &cf = p->confs[ci];
@@ -2212,7 +2218,7 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
                 fm.atoms = (const float4 *)p->fmin_atoms.ptr + cf.atom_base;
                 fm.n_atoms = cf.n_atoms;
                 fm.fmap = d_fmap + cf.pt_off;
-                k_fmin<<<(fm.nruns + 255) / 256, 256, 0, st>>>(fm);
+                k_fmin<<<(fm.nruns + 255) / 256, 256, 0, s>>>(fm);
             }
             if (!p->cov_slots.empty()) {
                 CovParams cv;
@@ -2229,13 +2235,16 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
                 for (int q = 0; q < cv.n_cov; q++) cv.slot[q] = p->cov_slots[q];
                 cv.P = cf.P;
                 cv.maps = d_maps + cf.map_off;
-                k_covalent<<<pgrid, 256, 0, st>>>(cv);
+                k_covalent<<<pgrid, 256, 0, s>>>(cv);
             }
             for (int m : p->zero_slots)
-                k_fill<<<pgrid, 256, 0, st>>>(d_maps + cf.map_off + (size_t)m * cf.P, cf.P, 0.f);
+                k_fill<<<pgrid, 256, 0, s>>>(d_maps + cf.map_off + (size_t)m * cf.P, cf.P, 0.f);
         }
         HIP_CHECK(hipGetLastError());
-        if (events && k == nchunks - 1) HIP_CHECK(hipEventRecord(c->ev[2], st));
+    };
+    // The far field of chunk k ([c0, c1), work counter k) and the tree
+    // build's collision rule after it, on stream s.
+    auto far_pass = [&](int k, int c0, int c1, hipStream_t s) {
         const int t0 = p->tile_prefix[c0], t1 = p->tile_prefix[c1];
         if (p->run_far && !skip_far && t1 > t0) {
             FarParams fp = p->far;
@@ -2244,31 +2253,59 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
             fp.item_end = t1;
             fp.queue = p->persistent ? d_queue + k : nullptr;
             const int blocks = p->persistent ? std::min(p->far_grid, t1 - t0) : t1 - t0;
-            launch_far(blocks, fp, st);
+            launch_far(blocks, fp, far_mode(*p, c0, c1), s);
         }
         if (p->want_collide)
             for (int ci = c0; ci < c1; ci++) {
                 const ConfDev &cf = p->confs[ci];
-                k_collide<<<(unsigned)((cf.P + 255) / 256), 256, 0, st>>>(
+                k_collide<<<(unsigned)((cf.P + 255) / 256), 256, 0, s>>>(
                     cf.P, (const unsigned char *)p->collide.ptr + cf.pt_off,
                     p->e_index >= 0 ? d_maps + cf.map_off + (size_t)p->e_index * cf.P : nullptr,
                     d_fmap ? d_fmap + cf.pt_off : nullptr);
             }
         HIP_CHECK(hipGetLastError());
-        if (host) {
-            HIP_CHECK(hipEventRecord(c->chunk_ev[k], st));
-            HIP_CHECK(hipStreamWaitEvent(c->copy_stream, c->chunk_ev[k], 0));
-            for (int ci = c0; ci < c1; ci++) {
-                const ConfDev &cf = p->confs[ci];
-                HIP_CHECK(hipMemcpyAsync(host->maps[ci], d_maps + cf.map_off, sizeof(float) * (size_t)p->n_maps * cf.P,
+    };
+    // Chunk k's outputs to the host on the copy stream, once s has them.
+    auto copy_out = [&](int k, int c0, int c1, hipStream_t s) {
+        HIP_CHECK(hipEventRecord(c->chunk_ev[k], s));
+        HIP_CHECK(hipStreamWaitEvent(c->copy_stream, c->chunk_ev[k], 0));
+        for (int ci = c0; ci < c1; ci++) {
+            const ConfDev &cf = p->confs[ci];
+            HIP_CHECK(hipMemcpyAsync(host->maps[ci], d_maps + cf.map_off, sizeof(float) * (size_t)p->n_maps * cf.P,
+                                     hipMemcpyDeviceToHost, c->copy_stream));
+            if (d_fmap)
+                HIP_CHECK(hipMemcpyAsync(host->fmap[ci], d_fmap + cf.pt_off, sizeof(float) * (size_t)cf.P,
                                          hipMemcpyDeviceToHost, c->copy_stream));
-                if (d_fmap)
-                    HIP_CHECK(hipMemcpyAsync(host->fmap[ci], d_fmap + cf.pt_off, sizeof(float) * (size_t)cf.P,
-                                             hipMemcpyDeviceToHost, c->copy_stream));
-                if (d_cmap)
-                    HIP_CHECK(hipMemcpyAsync(host->cmap[ci], d_cmap + cf.pt_off, sizeof(int32_t) * (size_t)cf.P,
-                                             hipMemcpyDeviceToHost, c->copy_stream));
-            }
+            if (d_cmap)
+                HIP_CHECK(hipMemcpyAsync(host->cmap[ci], d_cmap + cf.pt_off, sizeof(int32_t) * (size_t)cf.P,
+                                         hipMemcpyDeviceToHost, c->copy_stream));
+        }
+    };
+    const hipStream_t st_main = st;
+    if (host && nchunks > 1) {
+        // Pipelined batch: the whole batch's near field first, on the main
+        // stream, then the chunks' far fields alternating between the two
+        // compute streams (chunks touch disjoint conformations), each chunk's
+        // copies behind its far kernel.  Copies start after the first chunk's
+        // far field and overlap the rest; two far launches in flight fill
+        // each other's tails.  (Near and far passes interleaved chunk by
+        // chunk measured 8 ms slower on the 64-conformation ensemble: a near
+        // pass beside the other stream's persistent far kernel waits for CU
+        // slots.)
+        near_pass(0, p->nconf, st_main);
+        HIP_CHECK(hipEventRecord(c->cells_ev, st_main));
+        HIP_CHECK(hipStreamWaitEvent(c->stream2, c->cells_ev, 0));
+        for (int k = 0; k < nchunks; k++) {
+            const hipStream_t s = (k & 1) ? c->stream2 : st_main;
+            far_pass(k, chunks[k].first, chunks[k].second, s);
+            copy_out(k, chunks[k].first, chunks[k].second, s);
+        }
+    } else {
+        for (int k = 0; k < nchunks; k++) {
+            near_pass(chunks[k].first, chunks[k].second, st_main);
+            if (events && k == nchunks - 1) HIP_CHECK(hipEventRecord(c->ev[2], st_main));
+            far_pass(k, chunks[k].first, chunks[k].second, st_main);
+            if (host) copy_out(k, chunks[k].first, chunks[k].second, st_main);
         }
     }
     st = st_main;

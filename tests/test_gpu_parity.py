@@ -364,3 +364,31 @@ def test_overfull_brick_is_refused_and_the_context_recovers(ctx, tmp_path):
     for k in range(base.n_maps):
         ok, mx, rms, bad = parity.within(got[k, z], ref[k])
         assert ok, f"map {k}: max {mx} rms {rms} bad {bad}"
+
+
+def test_protein_density_box_against_oracle(ctx, tmp_path):
+    """The reference's x1hpv receptor (1,849 atoms, a real protein's
+    density) with its CASE0 map set on a 121^3 box, the bench's protein
+    workload: many bricks stage over 100 candidates, and the bricks go to
+    the XCDs in chunks of 16 over a grid whose work per brick varies.  Three
+    planes of every map against the oracle, to the parity bar."""
+    print("[gpu] x1hpv 121^3 planes vs oracle", flush=True)
+    gpf = cases.materialize("x1hpv_CASE0", str(tmp_path))
+    path = os.path.join(tmp_path, gpf)
+    text = open(path).read()
+    assert "npts 6 6 6" in text
+    open(path, "w").write(text.replace("npts 6 6 6", "npts 120 120 120"))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        st = ag.Job(gpf).prepare().state()
+    finally:
+        os.chdir(cwd)
+    assert st.n1 == (121, 121, 121) and st.n_atoms == 1849
+    got = ctx.maps_plain(st)
+    ost = ol.oracle_state(st)
+    for z in (0, 60, 120):
+        ref = ol.maps(ost, z, z + 1)[:, z]
+        for k in range(st.n_maps):
+            ok, mx, rms, bad = parity.within(got[k, z], ref[k])
+            assert ok, f"plane {z} map {k}: max {mx} rms {rms} bad {bad}"

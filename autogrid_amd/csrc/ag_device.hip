@@ -2044,9 +2044,11 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         // Per conformation: the largest bin any of its pairs reaches
         // (need) and the share of its pairs past bin 8192 (r >= 64 A),
         // from a 4 x 4 x 4 lattice of its box against at most 512 of its
-        // atoms.  The table-free path pays for its per-block atom test only
-        // when that share is large, and it is the one path that caps table
-        // indices, so a conformation that may pass the table's end takes it.
+        // atoms.  The table-free path saves 3 of 6.875 instructions on each
+        // such pair but its per-block atom test costs about 0.8 per pair, so
+        // it pays from a share of about 30 %; it is also the one path that
+        // caps table indices, so a conformation that may pass the table's
+        // end takes it.
         // Both depend on the conformation alone: its maps are the same in a
         // batch as in a plan of its own.
         long long need_max = 0;
@@ -2088,7 +2090,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
                         }
                     }
             const bool clamp = need > kFarTabMax;
-            const bool tfree = clamp || (double)far_n > 0.10 * (double)tot_n;
+            const bool tfree = clamp || (double)far_n > 0.30 * (double)tot_n;
             far_flags[i] = (tfree ? 1 : 0) | (clamp ? 2 : 0);
         }
         // The table runs to the largest bin any pair reaches, up to

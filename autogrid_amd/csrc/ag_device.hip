@@ -516,8 +516,13 @@ __device__ __forceinline__ void brick_lane(int tid, int &lx, int &ly, int &lz) {
     }
 }
 
-// CUBE selects the lane layout inside the 8x8x4 brick (brick_lane).
-template <int NPLAIN, bool CUBE>
+// CUBE selects the lane layout inside the 8x8x4 brick (brick_lane).  PAGED
+// false: a brick whose candidates (or donor H) do not fit one LDS list sets
+// the status word and is left to the PAGED form, which the host runs for the
+// whole map set when that word is set (the lists' pages then live in
+// registers beside the accumulators, with spills the common case should not
+// pay for).
+template <int NPLAIN, bool CUBE, bool PAGED>
 __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_eu(5))) k_near(NearParams p) {
     __shared__ int cand[kCandCap];
     __shared__ int hcand[kHCap];
@@ -587,80 +592,89 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     }
     const int ncx = c1[0] - c0[0] + 1, ncy = c1[1] - c0[1] + 1, ncz = c1[2] - c0[2] + 1;
     const int ncell = (ncx > 0 && ncy > 0 && ncz > 0) ? ncx * ncy * ncz : 0;
-    if (tid == 0) { ncand_s = 0; nh_s = 0; }
-    __syncthreads();
+    // Staging pass: the brick's candidates (atoms within 8.03 A of the box),
+    // in cell order and slot order within a cell, are numbered from 0 as
+    // found, and the donor H among them likewise.  Candidates numbered
+    // cpage .. cpage + kCandCap - 1 go to cand[] and donor H numbered hpage ..
+    // hpage + kHCap - 1 to hcand[] (a negative page stores none); ncand_s
+    // and nh_s end as the totals.  A brick with more than fits is processed
+    // page by page, each page staged by another pass, in the same order, so
+    // its sums are those one unbounded list would give.
     const float r2cut = kNearRadius * kNearRadius;
-    for (int cbase = 0; cbase < ncell; cbase += kNearThreads) {
-        const int ci = cbase + tid;
-        int cstart = 0, ccount = 0;
-        if (ci < ncell) {
-            const int gx = c0[0] + ci % ncx, gy = c0[1] + (ci / ncx) % ncy, gz = c0[2] + ci / (ncx * ncy);
-            const int g = cf.cell_base + gx + cf.nc[0] * (gy + cf.nc[1] * gz);
-            cstart = p.cell_start[g];
-            ccount = p.cell_start[g + 1] - cstart;
-        }
-        cell_off[tid] = ccount;
+    auto stage = [&](int cpage, int hpage) {
+        if (tid == 0) { ncand_s = 0; nh_s = 0; }
         __syncthreads();
-        for (int off = 1; off < kNearThreads; off <<= 1) {
-            int v = tid >= off ? cell_off[tid - off] : 0;
-            __syncthreads();
-            cell_off[tid] += v;
-            __syncthreads();
-        }
-        const int total = cell_off[kNearThreads - 1];
-        const int my_end = cell_off[tid];
-        __syncthreads();
-        cell_off[tid + 1] = my_end;
-        if (tid == 0) cell_off[0] = 0;
-        cstart_s[tid] = cstart;
-        __syncthreads();
-        for (int j0 = 0; j0 < total; j0 += kNearThreads) {
-            const int j = j0 + tid;
-            bool ok = false, is_h = false;
-            int slot = -1;
-            if (j < total) {
-                int lo = 0, hi = kNearThreads;  // last cell with cell_off[c] <= j
-                while (hi - lo > 1) {
-                    int mid = (lo + hi) >> 1;
-                    if (cell_off[mid] <= j) lo = mid; else hi = mid;
-                }
-                slot = cstart_s[lo] + (j - cell_off[lo]);
-                const NearAtom &a = p.atoms[slot];
-                float dd = 0.f;
-                const double av[3] = {a.x, a.y, a.z};
-                for (int d = 0; d < 3; d++) {
-                    double t = av[d] < bmin[d] ? bmin[d] - av[d] : (av[d] > bmax[d] ? av[d] - bmax[d] : 0.0);
-                    dd += (float)(t * t);
-                }
-                ok = dd <= r2cut;
-                const int hb = (a.meta >> 8) & 0xff;
-                is_h = ok && (hb == AG_HB_DS || hb == AG_HB_D1);
+        for (int cbase = 0; cbase < ncell; cbase += kNearThreads) {
+            const int ci = cbase + tid;
+            int cstart = 0, ccount = 0;
+            if (ci < ncell) {
+                const int gx = c0[0] + ci % ncx, gy = c0[1] + (ci / ncx) % ncy, gz = c0[2] + ci / (ncx * ncy);
+                const int g = cf.cell_base + gx + cf.nc[0] * (gy + cf.nc[1] * gz);
+                cstart = p.cell_start[g];
+                ccount = p.cell_start[g + 1] - cstart;
             }
-            int tot;
-            const int pos = block_compact(ok, wave_tot, tot);
-            const int base = ncand_s;
-            if (ok && base + pos < kCandCap) cand[base + pos] = slot;
-            int htot;
-            const int hpos_i = block_compact(is_h, wave_tot, htot);
-            const int hbase = nh_s;
-            if (is_h && hbase + hpos_i < kHCap) hcand[hbase + hpos_i] = slot;
+            cell_off[tid] = ccount;
             __syncthreads();
-            if (tid == 0) { ncand_s = base + tot; nh_s = hbase + htot; }
+            for (int off = 1; off < kNearThreads; off <<= 1) {
+                int v = tid >= off ? cell_off[tid - off] : 0;
+                __syncthreads();
+                cell_off[tid] += v;
+                __syncthreads();
+            }
+            const int total = cell_off[kNearThreads - 1];
+            const int my_end = cell_off[tid];
             __syncthreads();
+            cell_off[tid + 1] = my_end;
+            if (tid == 0) cell_off[0] = 0;
+            cstart_s[tid] = cstart;
+            __syncthreads();
+            for (int j0 = 0; j0 < total; j0 += kNearThreads) {
+                const int j = j0 + tid;
+                bool ok = false, is_h = false;
+                int slot = -1;
+                if (j < total) {
+                    int lo = 0, hi = kNearThreads;  // last cell with cell_off[c] <= j
+                    while (hi - lo > 1) {
+                        int mid = (lo + hi) >> 1;
+                        if (cell_off[mid] <= j) lo = mid; else hi = mid;
+                    }
+                    slot = cstart_s[lo] + (j - cell_off[lo]);
+                    const NearAtom &a = p.atoms[slot];
+                    float dd = 0.f;
+                    const double av[3] = {a.x, a.y, a.z};
+                    for (int d = 0; d < 3; d++) {
+                        double t = av[d] < bmin[d] ? bmin[d] - av[d] : (av[d] > bmax[d] ? av[d] - bmax[d] : 0.0);
+                        dd += (float)(t * t);
+                    }
+                    ok = dd <= r2cut;
+                    const int hb = (a.meta >> 8) & 0xff;
+                    is_h = ok && (hb == AG_HB_DS || hb == AG_HB_D1);
+                }
+                int tot;
+                const int pos = block_compact(ok, wave_tot, tot);
+                const int base = ncand_s;
+                const int cj = base + pos - cpage;
+                if (cpage >= 0 && ok && cj >= 0 && cj < kCandCap) cand[cj] = slot;
+                int htot;
+                const int hpos_i = block_compact(is_h, wave_tot, htot);
+                const int hbase = nh_s;
+                const int hj = hbase + hpos_i - hpage;
+                if (hpage >= 0 && is_h && hj >= 0 && hj < kHCap) hcand[hj] = slot;
+                __syncthreads();
+                if (tid == 0) { ncand_s = base + tot; nh_s = hbase + htot; }
+                __syncthreads();
+            }
+        }
+    };
+    stage(0, 0);
+    if constexpr (!PAGED) {
+        if (ncand_s > kCandCap || nh_s > kHCap) {   // uniform per workgroup
+            if (tid == 0) atomicOr(p.status, 1);
+            return;
         }
     }
+
     const int ncand = ncand_s, nh = nh_s;
-    if (ncand > kCandCap || nh > kHCap) {
-        if (tid == 0) atomicOr(p.status, 1);
-    } else {
-    // donor-H candidates into LDS for the closest-H search
-    for (int h = tid; h < nh; h += kNearThreads) {
-        const NearAtom &a = p.atoms[hcand[h]];
-        hpos[h][0] = a.x; hpos[h][1] = a.y; hpos[h][2] = a.z;
-        hidx[h] = a.index;
-        hrvs[h][0] = a.rv[0]; hrvs[h][1] = a.rv[1]; hrvs[h][2] = a.rv[2];
-    }
-    __syncthreads();
 
     float acc[kMapSlots];
 #pragma unroll
@@ -680,30 +694,44 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     // distances may tie, so there they are formed and compared as the
     // reference does.  The start, 999999^2, is exact and its root is 999999.
     double best_d2 = 999999.0 * 999999.0;
-    int best_idx = 0x7fffffff, best_h = -1;
-    for (int h = 0; h < nh; h++) {
+    int best_idx = 0x7fffffff;
+    float hrv[3] = {0.f, 0.f, 0.f};   // bond vector of the closest H so far
+    for (int h0 = 0; h0 < nh; h0 += kHCap) {   // pages of donor H
+        if constexpr (PAGED) {
+            if (h0 > 0) {
+                __syncthreads();
+                stage(-1, h0);
+            }
+        }
+        const int hn = min(nh - h0, kHCap);
+        for (int h = tid; h < hn; h += kNearThreads) {
+            const NearAtom &a = p.atoms[hcand[h]];
+            hpos[h][0] = a.x; hpos[h][1] = a.y; hpos[h][2] = a.z;
+            hidx[h] = a.index;
+            hrvs[h][0] = a.rv[0]; hrvs[h][1] = a.rv[1]; hrvs[h][2] = a.rv[2];
+        }
+        __syncthreads();
+        for (int h = 0; h < hn; h++) {
 #pragma clang fp contract(off)
-        const double dx = hpos[h][0] - cx, dy = hpos[h][1] - cy, dz = hpos[h][2] - cz;
-        const double d2 = dx * dx + dy * dy + dz * dz;   // as ref_dist forms it
-        const int ix_h = hidx[h];
-        bool take;
-        if (d2 < best_d2 * (1.0 - 0x1p-49)) {
-            take = true;
-        } else if (d2 > best_d2 * (1.0 + 0x1p-49)) {
-            take = false;
-        } else {
-            const double r = sqrt(d2), rb = sqrt(best_d2);
-            take = r < rb || (r == rb && ix_h < best_idx);
+            const double dx = hpos[h][0] - cx, dy = hpos[h][1] - cy, dz = hpos[h][2] - cz;
+            const double d2 = dx * dx + dy * dy + dz * dz;   // as ref_dist forms it
+            const int ix_h = hidx[h];
+            bool take;
+            if (d2 < best_d2 * (1.0 - 0x1p-49)) {
+                take = true;
+            } else if (d2 > best_d2 * (1.0 + 0x1p-49)) {
+                take = false;
+            } else {
+                const double r = sqrt(d2), rb = sqrt(best_d2);
+                take = r < rb || (r == rb && ix_h < best_idx);
+            }
+            if (take) {
+                best_d2 = d2;
+                best_idx = ix_h;
+                hrv[0] = hrvs[h][0]; hrv[1] = hrvs[h][1]; hrv[2] = hrvs[h][2];
+            }
         }
-        if (take) {
-            best_d2 = d2;
-            best_idx = ix_h;
-            best_h = h;
-        }
-    }
-    float hrv[3] = {0.f, 0.f, 0.f};
-    if (best_h >= 0) {
-        hrv[0] = hrvs[best_h][0]; hrv[1] = hrvs[best_h][1]; hrv[2] = hrvs[best_h][2];
+        if constexpr (!PAGED) break;
     }
 
     // The point indices are formed again from the thread index where they
@@ -736,16 +764,23 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     const int hcw = wlane < 7 ? (p.hcode[0][wlane] | p.hcode[1][wlane] << 8 | p.hcode[2][wlane] << 16 |
                                  p.hcode[3][wlane] << 24)
                               : 0;
-    {
+    for (int cpage = 0; cpage < ncand; cpage += kCandCap) {   // pages of candidates
+        if constexpr (PAGED) {
+            if (cpage > 0) {
+                __syncthreads();
+                stage(cpage, -1);
+            }
+        }
+        const int cn = min(ncand - cpage, kCandCap);
         // candidate records go straight into scalar registers through the
         // constant address space (the atom array is read-only here).  The
         // wave holds 64 candidate indices in one register and loads the
         // next record's coordinates, class and index one candidate ahead,
         // which also brings both of its cache lines into the scalar cache.
         const AG_CONST NearAtom *catoms = as_const(p.atoms);
-        for (int j0 = 0; j0 < ncand; j0 += 64) {
-            const int vcand = cand[min(j0 + wlane, ncand - 1)];
-            const int jn = min(64, ncand - j0);
+        for (int j0 = 0; j0 < cn; j0 += 64) {
+            const int vcand = cand[min(j0 + wlane, cn - 1)];
+            const int jn = min(64, cn - j0);
             int idx = __builtin_amdgcn_readlane(vcand, 0);
             double nx = catoms[idx].x, ny = catoms[idx].y, nz = catoms[idx].z;
             int nmeta = catoms[idx].meta, nindex = catoms[idx].index;
@@ -1001,6 +1036,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 if (p.d_slot >= 0) d_acc += unif(a.dvol) * S;
             }
         }
+        if constexpr (!PAGED) break;
     }
 
     int qx, qy, qz;
@@ -1025,7 +1061,6 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
         if (p.collide) p.collide[cf.pt_off + pt] = collide ? 1 : 0;
         if (p.d_slot >= 0) maps[(long long)p.d_slot * P + pt] = d_acc;
     }
-    }  // candidate lists within capacity
 }
 
 // The tree build's collision rule (main.cpp:2906-2927): a point within
@@ -1528,6 +1563,7 @@ struct ag_plan {
     bool far_atoms_ready = false;
     double coeff_estat = 0.0;
     int far_grid = 0;                  // persistent far-kernel grid
+    bool paged = false;                // near kernel in its paged form (a brick overflowed one LDS list)
     hipGraphExec_t graph = nullptr;    // captured map set (persistent mode)
     float *g_maps = nullptr, *g_fmap = nullptr;
     int32_t *g_cmap = nullptr;
@@ -1680,7 +1716,11 @@ static int hb_case(int mh, int ah) {
 }
 
 static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStream_t st) {
-#define AG_NEAR(NP, CB) k_near<NP, CB><<<grid, kNearThreads, 0, st>>>(np)
+#define AG_NEAR(NP, CB)                                                                 \
+    do {                                                                                \
+        if (p.paged) k_near<NP, CB, true><<<grid, kNearThreads, 0, st>>>(np);           \
+        else k_near<NP, CB, false><<<grid, kNearThreads, 0, st>>>(np);                  \
+    } while (0)
     if (p.tune.cube) {
         if (p.nplain == 4) AG_NEAR(4, true);
         else if (p.nplain == 8) AG_NEAR(8, true);
@@ -1694,8 +1734,6 @@ static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStr
     HIP_CHECK(hipGetLastError());
 }
 
-// Far-kernel variants by LDS table capacity: 12288 entries (48 KB) when the
-// table fits, else kFarTabMax (64 KB).
 // Variants by LDS table capacity (12288 entries, 48 KB, when the table
 // fits, else kFarTabMax, 64 KB) and by the paths the launch's items take.
 template <int MODE>
@@ -2324,16 +2362,34 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
     enqueue(p, d_maps, d_fmap, d_cmap, events, skip_far, {{0, p->nconf}}, nullptr);
 }
 
-static void check_status(ag_plan *p) {
+// Whether a brick of the last map set overflowed the near kernel's LDS lists
+// (syncs the context's stream).
+static bool overflowed(ag_plan *p) {
     // into pinned memory: a direct copy, not a staged one (a plan on an
     // arena pool allocates no pinned word and copies into the stack)
     int local = 0;
     int *status = p->pool->arena ? &local : (int *)p->ctx->status_word.get(sizeof(int));
     HIP_CHECK(hipMemcpyAsync(status, p->status.ptr, sizeof(int), hipMemcpyDeviceToHost, p->ctx->stream));
     HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
-    if (*status & 1)
-        ag::fail(AG_ERR_UNSUPPORTED, "more than 2048 atoms, or more than 256 donor hydrogens, within 8.03 A of "
-                                     "one 8x8x4 brick of grid points");
+    return (*status & 1) != 0;
+}
+
+// Runs one map set of p through `run` and checks the overflow word: when a
+// brick had more than 2048 candidate atoms or 256 donor H within 8.03 A, the
+// plan switches to the paged near kernel for good (its graph, if any, is
+// captured again) and runs the map set once more.
+template <class F>
+static void run_checked(ag_plan *p, F &&run) {
+    run();
+    if (!overflowed(p)) return;
+    if (p->paged) ag::fail(AG_ERR_DEVICE, "the paged near kernel reported an overflow");
+    p->paged = true;
+    if (p->graph) {
+        HIP_CHECK(hipGraphExecDestroy(p->graph));
+        p->graph = nullptr;
+    }
+    run();
+    if (overflowed(p)) ag::fail(AG_ERR_DEVICE, "the paged near kernel reported an overflow");
 }
 
 // Autotuner: times the near-field pass of every candidate layout on a
@@ -2377,8 +2433,7 @@ static Tune autotune(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *g
             std::unique_ptr<ag_plan> p(plan_build(c, ns, srec.data(), sgrid.data(), ms, tab, potential, true, t));
             float ms_near = 0.f;
             for (int rep = 0; rep < 2; rep++) {
-                enqueue(p.get(), d_maps, d_fmap, d_cmap, true, true);
-                check_status(p.get());
+                run_checked(p.get(), [&] { enqueue(p.get(), d_maps, d_fmap, d_cmap, true, true); });
                 HIP_CHECK(hipEventElapsedTime(&ms_near, c->ev[1], c->ev[2]));
             }
             if (ms_near < best_ms) {
@@ -2451,44 +2506,44 @@ void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
     HIP_CHECK(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     if (!p->persistent) {
-        enqueue(p, d_maps, d_fmap, d_cmap, true, false);
-        check_status(p);
+        run_checked(p, [&] { enqueue(p, d_maps, d_fmap, d_cmap, true, false); });
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[0], c->ev[0], c->ev[1]));
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[1], c->ev[1], c->ev[2]));
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[2], c->ev[2], c->ev[3]));
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
         return;
     }
-    if (!p->graph || p->g_maps != d_maps || p->g_fmap != d_fmap || p->g_cmap != d_cmap) {
-        if (p->graph) {
-            HIP_CHECK(hipGraphExecDestroy(p->graph));
-            p->graph = nullptr;
+    run_checked(p, [&] {
+        if (!p->graph || p->g_maps != d_maps || p->g_fmap != d_fmap || p->g_cmap != d_cmap) {
+            if (p->graph) {
+                HIP_CHECK(hipGraphExecDestroy(p->graph));
+                p->graph = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue(p, d_maps, d_fmap, d_cmap, false, false);
+            } catch (...) {
+                hipGraph_t dead = nullptr;
+                (void)hipStreamEndCapture(st, &dead);
+                if (dead) (void)hipGraphDestroy(dead);
+                throw;
+            }
+            HIP_CHECK(hipStreamEndCapture(st, &g));
+            hipError_t e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                p->graph = nullptr;
+                ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
+            }
+            p->g_maps = d_maps;
+            p->g_fmap = d_fmap;
+            p->g_cmap = d_cmap;
         }
-        hipGraph_t g = nullptr;
-        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        try {
-            enqueue(p, d_maps, d_fmap, d_cmap, false, false);
-        } catch (...) {
-            hipGraph_t dead = nullptr;
-            (void)hipStreamEndCapture(st, &dead);
-            if (dead) (void)hipGraphDestroy(dead);
-            throw;
-        }
-        HIP_CHECK(hipStreamEndCapture(st, &g));
-        hipError_t e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            p->graph = nullptr;
-            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
-        }
-        p->g_maps = d_maps;
-        p->g_fmap = d_fmap;
-        p->g_cmap = d_cmap;
-    }
-    HIP_CHECK(hipEventRecord(c->ev[0], st));
-    HIP_CHECK(hipGraphLaunch(p->graph, st));
-    HIP_CHECK(hipEventRecord(c->ev[3], st));
-    check_status(p);
+        HIP_CHECK(hipEventRecord(c->ev[0], st));
+        HIP_CHECK(hipGraphLaunch(p->graph, st));
+        HIP_CHECK(hipEventRecord(c->ev[3], st));
+    });
     HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
     c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = -1.f;
 }
@@ -2649,10 +2704,11 @@ void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids,
         // 10 chunks, the last 3 halving: on the 64-conformation ensemble this
         // measured 112-114 ms of GPU time against 116-117 ms for 16 equal chunks
         const auto chunks = plan_chunks(*p, kBatchChunks, kBatchTaper);
-        HIP_CHECK(hipEventRecord(c->ev[0], st));
-        enqueue(p.get(), d_maps, d_fmap, d_cmap, false, false, chunks, &host);
-        HIP_CHECK(hipEventRecord(c->ev[3], st));
-        check_status(p.get());
+        run_checked(p.get(), [&] {
+            HIP_CHECK(hipEventRecord(c->ev[0], st));
+            enqueue(p.get(), d_maps, d_fmap, d_cmap, false, false, chunks, &host);
+            HIP_CHECK(hipEventRecord(c->ev[3], st));
+        });
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
         c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = -1.f;
         return;

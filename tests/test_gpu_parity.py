@@ -324,15 +324,18 @@ def test_closest_donor_h_ties_take_the_lower_index(ctx, tmp_path):
         assert ok, f"map {k}: max {mx} rms {rms} bad {bad}"
 
 
-def test_overfull_brick_is_refused_and_the_context_recovers(ctx, tmp_path):
-    """The near kernel stages at most 2,048 candidate atoms (and 256 donor
-    H) per 8x8x4 brick, about ten times protein density over the brick's
-    8.03 A staging volume.  Past that the map set fails with
-    AG_ERR_UNSUPPORTED and a message naming the limit (every kernel still
-    runs to its end; the host reads the flag).  The same context then
-    computes an ordinary map set that matches the oracle."""
+def test_overfull_brick_runs_paged_against_oracle(ctx, tmp_path):
+    """The near kernel stages at most 2,048 candidate atoms and 256 donor H
+    per 8x8x4 brick in LDS, about ten times protein density over the
+    brick's 8.03 A staging volume.  A denser receptor sets the status word;
+    the plan then switches to the paged near kernel, which stages and
+    processes the lists page by page in the same order, and runs the map
+    set again.  6,000 atoms (about 360 donor H) in a 6 A ball: every map
+    against the oracle at every point; the same context then computes an
+    ordinary map set; and a batch holding both receptors (its plan switches
+    as a whole) equals each one's own plain run bit for bit."""
     import dataclasses
-    print("[gpu] overfull brick refused", flush=True)
+    print("[gpu] overfull brick, paged near kernel", flush=True)
     gpf = cases.materialize("small", str(tmp_path))
     path = os.path.join(tmp_path, gpf)
     lines = open(path).read().split("\n")
@@ -344,8 +347,8 @@ def test_overfull_brick_is_refused_and_the_context_recovers(ctx, tmp_path):
         base = ag.Job(gpf).prepare().state()
     finally:
         os.chdir(cwd)
-    # 3,000 atoms: the receptor's 1,000 three times, packed into a 6 A ball
-    pick = np.tile(np.arange(len(base.coord)), 3)
+    # 6,000 atoms: the receptor's 1,000 six times, packed into a 6 A ball
+    pick = np.tile(np.arange(len(base.coord)), 6)
     rng = np.random.default_rng(11)
     u = rng.normal(size=(len(pick), 3))
     u *= (6.0 * rng.random(len(pick)) ** (1 / 3) / np.linalg.norm(u, axis=1))[:, None]
@@ -354,16 +357,23 @@ def test_overfull_brick_is_refused_and_the_context_recovers(ctx, tmp_path):
         solpar=base.solpar[pick].copy(), type=base.type[pick].copy(), hbond=base.hbond[pick].copy(),
         rexp=base.rexp[pick].copy(), rvector=base.rvector[pick].copy(), rvector2=base.rvector2[pick].copy(),
         disorder=base.disorder[pick].copy(), _keep=[])
-    with pytest.raises(ag.AgError) as e:
-        ctx.maps_plain(st)
-    assert e.value.status == ag._abi.AG_ERR_UNSUPPORTED
-    assert "2048" in str(e.value)
-    got = ctx.maps_plain(base)
+    rexp, rv, rv2, dis = ag.bond_vectors(st, st.coord, st.type, st.hbond)
+    st = dataclasses.replace(st, rexp=rexp, rvector=rv, rvector2=rv2, disorder=dis, _keep=[])
+    assert (st.hbond == 2).sum() > 256
+    got = ctx.maps_plain(st)
+    ref = ol.maps(ol.oracle_state(st))
+    for k in range(st.n_maps):
+        ok, mx, rms, bad = parity.within(got[k], ref[k])
+        assert ok, f"dense map {k}: max {mx} rms {rms} bad {bad}"
+    plain = ctx.maps_plain(base)
     z = base.n1[2] // 2
-    ref = ol.maps(ol.oracle_state(base), z, z + 1)[:, z]
+    bref = ol.maps(ol.oracle_state(base), z, z + 1)[:, z]
     for k in range(base.n_maps):
-        ok, mx, rms, bad = parity.within(got[k, z], ref[k])
+        ok, mx, rms, bad = parity.within(plain[k, z], bref[k])
         assert ok, f"map {k}: max {mx} rms {rms} bad {bad}"
+    both = ctx.maps_batch([base, st], tune={"cell_size": 4.0, "cube": True})
+    assert np.array_equal(both[0].view(np.uint32), plain.view(np.uint32))
+    assert np.array_equal(both[1].view(np.uint32), got.view(np.uint32))
 
 
 def test_protein_density_box_against_oracle(ctx, tmp_path):

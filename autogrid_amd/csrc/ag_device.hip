@@ -192,20 +192,12 @@ struct FarParams {
     int e_slot;
 };
 
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
-    // Consecutive workgroup ids go round-robin over the 8 XCDs; give each XCD
-    // a contiguous range of tiles so neighbouring tiles share its L2.  XCD x
-    // receives q + (x < r) workgroups and owns tiles [x*q + min(x, r), ...).
-    const int q = nblocks / 8, r = nblocks % 8;
-    const int x = bid % 8, k = bid / 8;
-    return x * q + min(x, r) + k;
-}
-
 // Consecutive workgroup ids go round-robin over the 8 XCDs; here XCD x takes
 // chunks x, x + 8, x + 16, ... of C consecutive items, so that every XCD
 // gets items from all over the grid (the work per item varies with the
-// local atom density) while the items of a chunk share its L2.  Ids past the
-// last whole round of 8 chunks map to themselves.
+// local atom density, and in the far kernel's table-free path with the
+// distance to the atoms) while the items of a chunk share its L2.  Ids past
+// the last whole round of 8 chunks map to themselves.
 template <int C>
 __device__ __forceinline__ int xcd_chunked(int bid, int nblocks) {
     const int full = nblocks / (8 * C) * (8 * C);
@@ -1345,7 +1337,7 @@ __global__ void __launch_bounds__(kFarThreads) __attribute__((amdgpu_waves_per_e
         item = item_s;
     } else {
         __syncthreads();
-        item = p.item_base + xcd_remap(blockIdx.x, gridDim.x);
+        item = p.item_base + xcd_chunked<16>(blockIdx.x, gridDim.x);
     }
     item = __builtin_amdgcn_readfirstlane(item);
     while (item < p.item_end) {

@@ -2073,7 +2073,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         p.far_atoms_ready = true;
         // Per conformation: the largest bin any of its pairs reaches
         // (need) and the share of its pairs past bin 8192 (r >= 64 A),
-        // from a 4 x 4 x 4 lattice of its box against at most 512 of its
+        // from a 4 x 4 x 4 lattice of its box against about 128 of its
         // atoms.  The table-free path saves 3 of 6.875 instructions on each
         // such pair but its per-block atom test costs about 0.8 per pair, so
         // it pays from a share of about 30 %; it is also the one path that
@@ -2100,7 +2100,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
             }
             const long long need = (long long)(std::sqrt(rmax) * 128.0) + 8;
             need_max = std::max(need_max, need);
-            const int stride = std::max(1, na / 512);
+            const int stride = std::max(1, na / 128);
             long long far_n = 0, tot_n = 0;
             for (int qz = 0; qz < 4; qz++)
                 for (int qy = 0; qy < 4; qy++)

@@ -344,27 +344,8 @@ class Context:
         st0 = states[0]
         recs, grids, ms, tab, keep = _batch_structs(states)
         shapes = [(st.n1[2], st.n1[1], st.n1[0]) for st in states]
-
-        def outputs(given, wanted, dtype, lead):
-            if not wanted:
-                return None
-            if given is None:
-                return [np.empty(lead(st) + sh, dtype) for st, sh in zip(states, shapes)]
-            if len(given) != len(states):
-                raise ValueError(f"{len(given)} output arrays for {len(states)} conformations")
-            for i, (a, st, sh) in enumerate(zip(given, states, shapes)):
-                want = lead(st) + sh
-                if not (isinstance(a, np.ndarray) and a.dtype == dtype and a.flags.c_contiguous
-                        and a.shape == want):
-                    raise ValueError(f"output {i} must be a C-contiguous {np.dtype(dtype).name} array of "
-                                     f"shape {want}")
-            return list(given)
-        outs = outputs(outs, True, np.float32, lambda st: (st.n_maps,))
-        fm = outputs(fmaps, bool(st0.floating_grid), np.float32, lambda st: ())
-        cm = outputs(cmaps, bool(st0.constriction_grid), np.int32, lambda st: ())
-        ptrs = (C.c_void_p * len(states))(*[o.ctypes.data for o in outs])
-        fptrs = (C.c_void_p * len(states))(*[a.ctypes.data for a in fm]) if fm else None
-        cptrs = (C.c_void_p * len(states))(*[a.ctypes.data for a in cm]) if cm else None
+        outs, fm, cm, ptrs, fptrs, cptrs = _host_outputs(shapes, st0.n_maps, bool(st0.floating_grid),
+                                                         bool(st0.constriction_grid), outs, fmaps, cmaps)
         t = _tune_struct(tune)
         check(lib().ag_maps_batch(self._h, len(states), recs, grids, C.byref(ms), C.byref(tab), ptrs, fptrs, cptrs,
                                   int(potential), C.byref(t) if t is not None else None))
@@ -386,6 +367,33 @@ class Context:
         flags = (_abi.AG_JOB_BHTREE if bhtree else 0) | (_abi.AG_JOB_BINARY_MAPS if binary_maps else 0)
         check(lib().ag_run_gpf_flags(self._h, os.fsencode(gpf_path), os.fsencode(glg_path) if glg_path else None,
                                      program.encode(), flags))
+
+
+def _host_outputs(shapes: list, n_maps: int, want_fmap: bool, want_cmap: bool, outs, fmaps, cmaps):
+    """Host output arrays of a batch (allocated where not given, checked
+    where given: C-contiguous, float32 maps [n_maps, n1z, n1y, n1x] and
+    fmaps [n1z, n1y, n1x], int32 cmaps) and ctypes pointer arrays to them."""
+    def outputs(given, wanted, dtype, lead):
+        if not wanted:
+            return None
+        if given is None:
+            return [np.empty(lead + sh, dtype) for sh in shapes]
+        if len(given) != len(shapes):
+            raise ValueError(f"{len(given)} output arrays for {len(shapes)} conformations")
+        for i, (a, sh) in enumerate(zip(given, shapes)):
+            want = lead + tuple(sh)
+            if not (isinstance(a, np.ndarray) and a.dtype == dtype and a.flags.c_contiguous and a.shape == want):
+                raise ValueError(f"output {i} must be a C-contiguous {np.dtype(dtype).name} array of "
+                                 f"shape {want}")
+        return list(given)
+    outs = outputs(outs, True, np.float32, (n_maps,))
+    fm = outputs(fmaps, want_fmap, np.float32, ())
+    cm = outputs(cmaps, want_cmap, np.int32, ())
+    n = len(shapes)
+    ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+    fptrs = (C.c_void_p * n)(*[a.ctypes.data for a in fm]) if fm else None
+    cptrs = (C.c_void_p * n)(*[a.ctypes.data for a in cm]) if cm else None
+    return outs, fm, cm, ptrs, fptrs, cptrs
 
 
 def _tune_struct(tune: dict | None):
@@ -443,6 +451,10 @@ class Plan:
             check(lib().ag_plan_create_potential(ctx.handle, C.byref(rec), C.byref(grid), C.byref(ms),
                                                  C.byref(tab), int(potential), C.byref(h)))
             self.n_maps = st.n_maps
+            states = [st]
+        self.shapes = [(s.n1[2], s.n1[1], s.n1[0]) for s in states]
+        self.want_fmap = bool(states[0].floating_grid)
+        self.want_cmap = bool(states[0].constriction_grid)
         self._h = h
         self.ctx = ctx
         n = lib().ag_plan_conformations(h)
@@ -464,6 +476,21 @@ class Plan:
         """All maps into device memory at maps_dev_ptr ([n_maps][P] float32)."""
         check(lib().ag_plan_execute(self._h, C.c_void_p(maps_dev_ptr), C.c_void_p(fmap_dev_ptr),
                                     C.c_void_p(cmap_dev_ptr)))
+
+    def execute_host(self, maps_dev_ptr: int, outs: list | None = None, fmaps: list | None = None,
+                     cmaps: list | None = None, fmap_dev_ptr: int | None = None,
+                     cmap_dev_ptr: int | None = None) -> list:
+        """Execute into the device buffers and copy every conformation's
+        maps (and fmap / cmap) into host arrays (ag_plan_execute_host):
+        the maps other than e are copied while the far-field kernel runs.
+        Returns the list of [n_maps, n1z, n1y, n1x] float32 arrays; the
+        fmap / cmap arrays are kept in last_fmaps / last_cmaps."""
+        outs, fm, cm, ptrs, fptrs, cptrs = _host_outputs(self.shapes, self.n_maps, self.want_fmap,
+                                                         self.want_cmap, outs, fmaps, cmaps)
+        check(lib().ag_plan_execute_host(self._h, C.c_void_p(maps_dev_ptr), C.c_void_p(fmap_dev_ptr),
+                                         C.c_void_p(cmap_dev_ptr), ptrs, fptrs, cptrs))
+        self.last_fmaps, self.last_cmaps = fm, cm
+        return outs
 
     def audit(self) -> dict:
         """Table-bin audit against the reference's FP64 distances (ag_plan_audit)."""

@@ -154,6 +154,7 @@ def lib() -> C.CDLL:
         L.ag_plan_create.argtypes = [vp, C.POINTER(Receptor), C.POINTER(Grid), C.POINTER(MapSet),
                                      C.POINTER(Tables), C.POINTER(vp)]
         L.ag_plan_execute.argtypes = [vp, vp, vp, vp]
+        L.ag_plan_execute_host.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.ag_plan_destroy.argtypes = [vp]
         L.ag_plan_audit.argtypes = [vp, C.POINTER(C.c_int64 * 4)]
         L.ag_context_timing.argtypes = [vp, C.POINTER(C.c_float)]
@@ -199,7 +200,7 @@ def lib() -> C.CDLL:
                      "ag_scorer_create", "ag_scorer_set_positions", "ag_scorer_score", "ag_scorer_adjoint",
                      "ag_scorer_destroy"):
             getattr(L, name).restype = C.c_int
-        for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_destroy", "ag_plan_audit", "ag_context_timing",
+        for name in ("ag_plan_create", "ag_plan_execute", "ag_plan_execute_host", "ag_plan_destroy", "ag_plan_audit", "ag_context_timing",
                      "ag_context_create", "ag_context_destroy", "ag_job_open", "ag_job_prepare",
                      "ag_job_views", "ag_run_gpf", "ag_maps_plain", "ag_job_write_maps", "ag_xs_radius",
                      "ag_vina_tables", "ag_job_prepare_vina", "ag_maps_vina", "ag_plan_create_potential"):

@@ -1600,6 +1600,7 @@ ag_status context_create(ag_ctx **out, int device) {
     HIP_CHECK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreateWithFlags(&c->cells_ev, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&c->near_ev, hipEventDisableTiming));
     for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
     for (auto &e : c->chunk_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
@@ -1629,6 +1630,7 @@ void context_destroy(ag_ctx *c) {
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->cells_ev) (void)hipEventDestroy(c->cells_ev);
+    if (c->near_ev) (void)hipEventDestroy(c->near_ev);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -2191,9 +2193,11 @@ static std::vector<std::pair<int, int>> plan_chunks(const ag_plan &p, int want, 
 // With events, HIP events bracket the cell list, the near-field and the
 // far-field phases (single chunk).  With host outputs, each chunk's maps
 // are copied to the host on the context's copy stream as soon as the chunk
-// is done, while the next chunk computes.
+// is done, while the next chunk computes.  mark_near records the context's
+// near_ev once the near field of every conformation is enqueued (single
+// chunk; in a stream capture, as an external event node of the graph).
 static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far,
-                    const std::vector<std::pair<int, int>> &chunks, const HostOut *host) {
+                    const std::vector<std::pair<int, int>> &chunks, const HostOut *host, bool mark_near = false) {
     ag_ctx *c = p->ctx;
     hipStream_t st = c->stream;
     const int N = p->N;
@@ -2336,6 +2340,14 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
         for (int k = 0; k < nchunks; k++) {
             near_pass(chunks[k].first, chunks[k].second, st_main);
             if (events && k == nchunks - 1) HIP_CHECK(hipEventRecord(c->ev[2], st_main));
+            if (mark_near && k == nchunks - 1) {
+                hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+                HIP_CHECK(hipStreamIsCapturing(st_main, &cs));
+                if (cs == hipStreamCaptureStatusActive)
+                    HIP_CHECK(hipEventRecordWithFlags(c->near_ev, st_main, hipEventRecordExternal));
+                else
+                    HIP_CHECK(hipEventRecord(c->near_ev, st_main));
+            }
             far_pass(k, chunks[k].first, chunks[k].second, st_main);
             if (host) copy_out(k, chunks[k].first, chunks[k].second, st_main);
         }
@@ -2350,8 +2362,9 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
     if (events) HIP_CHECK(hipEventRecord(c->ev[3], st));
 }
 
-static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far) {
-    enqueue(p, d_maps, d_fmap, d_cmap, events, skip_far, {{0, p->nconf}}, nullptr);
+static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far,
+                    bool mark_near = false) {
+    enqueue(p, d_maps, d_fmap, d_cmap, events, skip_far, {{0, p->nconf}}, nullptr, mark_near);
 }
 
 // Whether a brick of the last map set overflowed the near kernel's LDS lists
@@ -2489,55 +2502,121 @@ void plan_destroy(ag_plan *p) {
 
 // One map set (all conformations of the plan) into d_maps
 // ([sum n_maps * P] float, device) and, when asked for, d_fmap / d_cmap
-// ([sum P]).  A single plan runs its kernels directly with phase timing; a
-// batch replays one captured hipGraph (re-captured when the outputs move).
-void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
-    if ((p->want_fmap && !d_fmap) || (p->want_cmap && !d_cmap))
-        ag::fail(AG_ERR_BAD_INPUT, "the map set asks for a floating/constriction grid but no output was given");
+// ([sum P]), enqueued on the context's stream.  A single plan runs its
+// kernels directly with phase timing; a batch replays one captured hipGraph
+// (re-captured when the outputs move).  Either way the context's near_ev is
+// recorded once the near field is done.
+static void launch_mapset(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
     ag_ctx *c = p->ctx;
-    HIP_CHECK(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     if (!p->persistent) {
-        run_checked(p, [&] { enqueue(p, d_maps, d_fmap, d_cmap, true, false); });
+        enqueue(p, d_maps, d_fmap, d_cmap, true, false, true);
+        return;
+    }
+    if (!p->graph || p->g_maps != d_maps || p->g_fmap != d_fmap || p->g_cmap != d_cmap) {
+        if (p->graph) {
+            HIP_CHECK(hipGraphExecDestroy(p->graph));
+            p->graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue(p, d_maps, d_fmap, d_cmap, false, false, true);
+        } catch (...) {
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(st, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(st, &g));
+        hipError_t e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            p->graph = nullptr;
+            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
+        }
+        p->g_maps = d_maps;
+        p->g_fmap = d_fmap;
+        p->g_cmap = d_cmap;
+    }
+    HIP_CHECK(hipEventRecord(c->ev[0], st));
+    HIP_CHECK(hipGraphLaunch(p->graph, st));
+    HIP_CHECK(hipEventRecord(c->ev[3], st));
+}
+
+static void read_timing(ag_plan *p) {
+    ag_ctx *c = p->ctx;
+    if (!p->persistent) {
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[0], c->ev[0], c->ev[1]));
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[1], c->ev[1], c->ev[2]));
         HIP_CHECK(hipEventElapsedTime(&c->last_ms[2], c->ev[2], c->ev[3]));
-        HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
-        return;
+    } else {
+        c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = -1.f;
     }
-    run_checked(p, [&] {
-        if (!p->graph || p->g_maps != d_maps || p->g_fmap != d_fmap || p->g_cmap != d_cmap) {
-            if (p->graph) {
-                HIP_CHECK(hipGraphExecDestroy(p->graph));
-                p->graph = nullptr;
-            }
-            hipGraph_t g = nullptr;
-            HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            try {
-                enqueue(p, d_maps, d_fmap, d_cmap, false, false);
-            } catch (...) {
-                hipGraph_t dead = nullptr;
-                (void)hipStreamEndCapture(st, &dead);
-                if (dead) (void)hipGraphDestroy(dead);
-                throw;
-            }
-            HIP_CHECK(hipStreamEndCapture(st, &g));
-            hipError_t e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) {
-                p->graph = nullptr;
-                ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
-            }
-            p->g_maps = d_maps;
-            p->g_fmap = d_fmap;
-            p->g_cmap = d_cmap;
-        }
-        HIP_CHECK(hipEventRecord(c->ev[0], st));
-        HIP_CHECK(hipGraphLaunch(p->graph, st));
-        HIP_CHECK(hipEventRecord(c->ev[3], st));
-    });
     HIP_CHECK(hipEventElapsedTime(&c->last_ms[3], c->ev[0], c->ev[3]));
-    c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = -1.f;
+}
+
+void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
+    if ((p->want_fmap && !d_fmap) || (p->want_cmap && !d_cmap))
+        ag::fail(AG_ERR_BAD_INPUT, "the map set asks for a floating/constriction grid but no output was given");
+    HIP_CHECK(hipSetDevice(p->ctx->device));
+    run_checked(p, [&] { launch_mapset(p, d_maps, d_fmap, d_cmap); });
+    read_timing(p);
+}
+
+// plan_execute, then every conformation's outputs into host memory
+// (maps_out[i]: [n_maps][P_i] floats; fmap_out[i] / cmap_out[i]: [P_i]).
+// Only the e map is written after the near field (by the far kernel and the
+// tree build's collision rule, which also sets fmap), so the copy stream
+// takes the other maps, cmap and -- without the collision rule -- fmap as
+// soon as near_ev is reached, while the far kernel runs; the e maps (and
+// fmap) follow the end of the map set.  Host memory pinned with
+// hipHostMalloc lets the copies overlap the kernels.
+void plan_execute_host(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, float *const *maps_out,
+                       float *const *fmap_out, int32_t *const *cmap_out) {
+    if ((p->want_fmap && !d_fmap) || (p->want_cmap && !d_cmap))
+        ag::fail(AG_ERR_BAD_INPUT, "the map set asks for a floating/constriction grid but no output was given");
+    if (!maps_out) ag::fail(AG_ERR_BAD_INPUT, "no map outputs");
+    for (int i = 0; i < p->nconf; i++)
+        if (!maps_out[i] || (p->want_fmap && (!fmap_out || !fmap_out[i])) ||
+            (p->want_cmap && (!cmap_out || !cmap_out[i])))
+            ag::fail(AG_ERR_BAD_INPUT, "missing output for a conformation");
+    ag_ctx *c = p->ctx;
+    HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t st = c->stream, cs = c->copy_stream;
+    const int e = p->e_index;
+    const bool fmap_late = p->want_collide;
+    auto copy = [&](void *dst, const void *src, size_t bytes) {
+        if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, cs));
+    };
+    run_checked(p, [&] {
+        launch_mapset(p, d_maps, d_fmap, d_cmap);
+        HIP_CHECK(hipStreamWaitEvent(cs, c->near_ev, 0));
+        for (int i = 0; i < p->nconf; i++) {
+            const ConfDev &cf = p->confs[i];
+            const size_t pb = sizeof(float) * (size_t)cf.P;
+            const float *src = d_maps + cf.map_off;
+            if (e < 0) {
+                copy(maps_out[i], src, pb * p->n_maps);
+            } else {
+                copy(maps_out[i], src, pb * e);
+                copy(maps_out[i] + (size_t)(e + 1) * cf.P, src + (size_t)(e + 1) * cf.P, pb * (p->n_maps - e - 1));
+            }
+            if (d_cmap) copy(cmap_out[i], d_cmap + cf.pt_off, sizeof(int32_t) * (size_t)cf.P);
+            if (d_fmap && !fmap_late) copy(fmap_out[i], d_fmap + cf.pt_off, pb);
+        }
+        HIP_CHECK(hipEventRecord(c->chunk_ev[0], st));
+        HIP_CHECK(hipStreamWaitEvent(cs, c->chunk_ev[0], 0));
+        for (int i = 0; i < p->nconf; i++) {
+            const ConfDev &cf = p->confs[i];
+            const size_t pb = sizeof(float) * (size_t)cf.P;
+            if (e >= 0) copy(maps_out[i] + (size_t)e * cf.P, d_maps + cf.map_off + (size_t)e * cf.P, pb);
+            if (d_fmap && fmap_late) copy(fmap_out[i], d_fmap + cf.pt_off, pb);
+        }
+        HIP_CHECK(hipEventRecord(c->chunk_ev[kMaxChunks], cs));
+        HIP_CHECK(hipStreamWaitEvent(st, c->chunk_ev[kMaxChunks], 0));
+    });
+    read_timing(p);
 }
 
 // One map set, handed to the host map by map: each map is copied into host

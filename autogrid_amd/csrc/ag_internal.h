@@ -209,6 +209,8 @@ void maps_batch(ag_ctx *c, int n, const ag_receptor *recs, const ag_grid *grids,
                 const ag_tables *tab, float *const *maps_out, float *const *fmap_out, int32_t *const *cmap_out,
                 int potential, const ag_tune *tune);
 void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap);
+void plan_execute_host(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, float *const *maps_out,
+                       float *const *fmap_out, int32_t *const *cmap_out);
 void plan_audit(ag_plan *p, long long out[4]);
 void maps_plain_streamed(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag_mapset *ms,
                          const ag_tables *tab, float *maps_out, float *fmap, int32_t *cmap,

@@ -548,6 +548,17 @@ ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32
     return guarded([&] { agdev::plan_execute(plan, maps_dev, fmap_dev, cmap_dev); });
 }
 
+ag_status ag_plan_execute_host(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev,
+                               float *const *maps_out, float *const *fmap_out, int32_t *const *cmap_out) {
+    if (!plan || !maps_dev || !maps_out) {
+        ag::set_last_error("null argument to ag_plan_execute_host");
+        return AG_ERR_BAD_INPUT;
+    }
+    return guarded([&] {
+        agdev::plan_execute_host(plan, maps_dev, fmap_dev, cmap_dev, maps_out, fmap_out, cmap_out);
+    });
+}
+
 ag_status ag_plan_audit(ag_plan *plan, ag_audit *out) {
     if (!plan || !out) {
         ag::set_last_error("null argument to ag_plan_audit");

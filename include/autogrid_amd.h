@@ -271,6 +271,19 @@ typedef struct ag_plan ag_plan;
 ag_status ag_plan_create(ag_ctx *ctx, const ag_receptor *rec, const ag_grid *grid,
                          const ag_mapset *mapset, const ag_tables *tables, ag_plan **out);
 ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev);
+/* ag_plan_execute, then conformation i's outputs copied into host memory:
+ * maps_out[i] ([n_maps][P_i] floats) and, when the map set asks for them,
+ * fmap_out[i] / cmap_out[i] ([P_i]).  The device buffers are the work space
+ * and hold the same values afterwards.  Every map but e is final once the
+ * near field is done, so those copies run on a copy stream while the
+ * far-field kernel computes (behind an event the batch's graph records);
+ * the e maps follow the end of the map set.  With pinned host outputs
+ * (hipHostMalloc) the copies overlap the kernels.  Returns after the
+ * outputs are written.  It replaces the reference's hand-over of the
+ * finished gridmap[j].energy arrays to its map writer (main.cpp:3474-3497),
+ * for a plan or for every conformation of a batch. */
+ag_status ag_plan_execute_host(ag_plan *plan, float *maps_dev, float *fmap_dev, int32_t *cmap_dev,
+                               float *const *maps_out, float *const *fmap_out, int32_t *const *cmap_out);
 ag_status ag_plan_destroy(ag_plan *plan);
 
 /* Table-bin audit of a plan (a diagnostic, outside the map computation):
@@ -410,8 +423,9 @@ ag_status ag_plan_tune(const ag_plan *plan, ag_tune *out);
 /* Batch with host outputs: create, execute once, copy conformation i's maps
  * to maps_out[i] ([n_maps][P_i] floats; fmap_out[i] / cmap_out[i] when the
  * map set asks for them, else those arrays may be NULL), destroy.  When all
- * outputs are pinned host memory (hipHostMalloc), the copies are part of
- * the graph and overlap the computation chunk by chunk. */
+ * outputs are pinned host memory (hipHostMalloc), the conformations run in
+ * chunks launched directly on two streams, and each chunk's copies overlap
+ * the later chunks' kernels. */
 ag_status ag_maps_batch(ag_ctx *ctx, int n, const ag_receptor *recs, const ag_grid *grids,
                         const ag_mapset *mapset, const ag_tables *tables,
                         float *const *maps_out, float *const *fmap_out,

@@ -241,3 +241,75 @@ def test_batch_pinned_outputs_with_distance_grids(ensemble12):
                 assert np.array_equal(cms[i], rc[i]), f"cmap {i}"
     finally:
         hip.hipHostFree(buf)
+
+
+@pytest.mark.parametrize("bhtree", [0, 1])
+def test_plan_execute_host_equals_device_outputs(ensemble12, bhtree):
+    """ag_plan_execute_host: the batch's graph, with every map but e (and
+    cmap, and fmap unless the tree build's collision rule rewrites it after
+    the far kernel) copied to the host behind the event the graph records
+    after the near field, the rest after the graph.  Maps, fmap and cmap
+    must equal ag_plan_execute's device outputs bit for bit: into pinned
+    destinations on two launches (the second a replay) and into pageable
+    ones.  A single-conformation plan must equal ag_maps_plain."""
+    import ctypes as C
+    states = [dataclasses.replace(s, floating_grid=1, constriction_grid=1, constriction_cutoff=6.5, bhtree=bhtree,
+                                  _keep=[]) for s in ensemble12]
+    print(f"[gpu] plan execute to host, bhtree={bhtree}, {len(states)} conformations", flush=True)
+    hip = _hip()
+    sizes = [st.n_maps * st.points for st in states]
+    pts = [st.points for st in states]
+    total = 4 * (sum(sizes) + 2 * sum(pts))
+    buf = C.c_void_p()
+    assert hip.hipHostMalloc(C.byref(buf), C.c_size_t(total), 0) == 0
+    dev = [C.c_void_p() for _ in range(3)]
+    for d, n in zip(dev, (sum(sizes), sum(pts), sum(pts))):
+        assert hip.hipMalloc(C.byref(d), C.c_size_t(4 * n)) == 0
+    try:
+        raw = np.ctypeslib.as_array(C.cast(buf, C.POINTER(C.c_uint8)), shape=(total,))
+        outs, fms, cms, o = [], [], [], 0
+        for st, n in zip(states, sizes):
+            outs.append(raw[o:o + 4 * n].view(np.float32).reshape(st.n_maps, st.n1[2], st.n1[1], st.n1[0]))
+            o += 4 * n
+        for st in states:
+            fms.append(raw[o:o + 4 * st.points].view(np.float32).reshape(st.n1[2], st.n1[1], st.n1[0]))
+            o += 4 * st.points
+        for st in states:
+            cms.append(raw[o:o + 4 * st.points].view(np.int32).reshape(st.n1[2], st.n1[1], st.n1[0]))
+            o += 4 * st.points
+        with ag.Context(0) as ctx:
+            with ctx.batch(states, tune={"cell_size": 4.0, "cube": True}) as b:
+                b.execute(dev[0].value, dev[1].value, dev[2].value)
+                ref = np.empty(sum(sizes), np.float32)
+                rf = np.empty(sum(pts), np.float32)
+                rc = np.empty(sum(pts), np.int32)
+                for a, d in zip((ref, rf, rc), dev):
+                    assert hip.hipMemcpy(a.ctypes.data_as(C.c_void_p), d, C.c_size_t(a.nbytes), 2) == 0
+                mo, po = b.map_offsets, b.point_offsets
+
+                def check(got_maps, got_f, got_c, what):
+                    for i, st in enumerate(states):
+                        r = ref[mo[i]:mo[i] + sizes[i]]
+                        assert np.array_equal(got_maps[i].reshape(-1).view(np.uint32), r.view(np.uint32)), \
+                            f"{what}: maps {i}"
+                        assert np.array_equal(got_f[i].reshape(-1).view(np.uint32),
+                                              rf[po[i]:po[i] + pts[i]].view(np.uint32)), f"{what}: fmap {i}"
+                        assert np.array_equal(got_c[i].reshape(-1), rc[po[i]:po[i] + pts[i]]), f"{what}: cmap {i}"
+                for rep in range(2):
+                    raw[:] = 0xFF
+                    b.execute_host(dev[0].value, outs, fms, cms, dev[1].value, dev[2].value)
+                    check(outs, fms, cms, f"pinned, launch {rep}")
+                got = b.execute_host(dev[0].value, fmap_dev_ptr=dev[1].value, cmap_dev_ptr=dev[2].value)
+                check(got, b.last_fmaps, b.last_cmaps, "pageable")
+            st = states[-1]
+            ref1 = ctx.maps_plain(st)
+            f1, c1 = ctx.last_fmap, ctx.last_cmap
+            with ctx.plan(st) as p:
+                got = p.execute_host(dev[0].value, fmap_dev_ptr=dev[1].value, cmap_dev_ptr=dev[2].value)
+                assert np.array_equal(got[0].view(np.uint32), ref1.view(np.uint32))
+                assert np.array_equal(p.last_fmaps[0].view(np.uint32), f1.view(np.uint32))
+                assert np.array_equal(p.last_cmaps[0], c1)
+    finally:
+        for d in dev:
+            hip.hipFree(d)
+        hip.hipHostFree(buf)

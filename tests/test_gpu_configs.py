@@ -2,8 +2,9 @@
 
 Batched ensemble: 64 receptor conformations (seed 1005) of 1,594-11,710
 atoms with cubic boxes of 41-155 points, maps A C HD N NA OA + e + d.
-  * the one-graph path (ag_batch_create, then ag_plan_execute) and the
-    chunked path with pinned host outputs (ag_maps_batch) each give every
+  * the one-graph path (ag_batch_create, then ag_plan_execute, and
+    ag_plan_execute_host into pinned memory) and the chunked path with
+    pinned host outputs (ag_maps_batch) each give every
     conformation's maps bit for bit equal to ag_maps_plain's, at the plain
     path's 4 A cells (the same per-point summation order);
   * the autotuned batch is repeatable bit for bit and matches the FP64
@@ -93,13 +94,25 @@ def test_ensemble64_graph_path_equals_plain(ensemble64, plain64):
         nbytes = 4 * b.map_floats
         d = C.c_void_p()
         assert hip.hipMalloc(C.byref(d), C.c_size_t(nbytes)) == 0
+        pinned = Pinned(nbytes)
         try:
             b.execute(d.value)
             b.execute(d.value)             # a replay of the captured graph
             flat = np.empty(b.map_floats, np.float32)
             assert hip.hipMemcpy(flat.ctypes.data_as(C.c_void_p), d, C.c_size_t(nbytes), 2) == 0
+            # the same graph with its host copies overlapping the far kernel
+            host = pinned.raw.view(np.float32)
+            host[:] = np.nan
+            outs, o = [], 0
+            for st, ref in zip(ensemble64, plain64):
+                outs.append(host[o:o + ref.size].reshape(ref.shape))
+                o += ref.size
+            b.execute_host(d.value, outs)
+            for i, (g, ref) in enumerate(zip(outs, plain64)):
+                assert np.array_equal(g.view(np.uint32), ref.view(np.uint32)), f"host copies, conformation {i}"
         finally:
             hip.hipFree(d)
+            pinned.free()
         offs = b.map_offsets
     for i, (st, ref) in enumerate(zip(ensemble64, plain64)):
         got = flat[offs[i]:offs[i] + ref.size]

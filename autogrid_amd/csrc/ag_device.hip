@@ -827,6 +827,29 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                     if (want_cmap) ccount += a0 <= ccf && a1 <= ccf && a2 <= ccf && D <= ccf2;
                     rB = sqrtf(D);   // correctly rounded, as the tree's sqrt of a float
                 }
+                // The pair's vdW/H-bond table row for the pass's slots, loaded
+                // here so that the loads are in flight during the constriction,
+                // electrostatic and directional work below.  The type's rows
+                // start at a scalar base, the bin's row is a 32-bit byte offset
+                // from it; the bin is the one Loop 2 uses (the tree build's FP32
+                // distance in that mode).  Every lane here has r < 8.03 A, so
+                // the row index stays inside the table.
+                const int indx_n = min(bh ? (int)((double)rB * 128.0) : bin, AG_NEINT - 1);
+                float E[kMapSlots];
+                {
+                    const char *trow =
+                        reinterpret_cast<const char *>(tab + (size_t)(ameta & 0xff) * AG_NEINT * kMapSlots);
+                    const unsigned roff = (unsigned)indx_n * (unsigned)(kMapSlots * sizeof(float));
+    #pragma unroll
+                    for (int q4 = 0; q4 < kMapSlots / 4; q4++) {
+                        if (q4 < NPLAIN / 4 || q4 == kMapSlots / 4 - 1) {
+                            const float4 v = *reinterpret_cast<const float4 *>(trow + roff + 16u * q4);
+                            E[4 * q4] = v.x; E[4 * q4 + 1] = v.y; E[4 * q4 + 2] = v.z; E[4 * q4 + 3] = v.w;
+                        } else {
+                            E[4 * q4] = E[4 * q4 + 1] = E[4 * q4 + 2] = E[4 * q4 + 3] = 0.f;
+                        }
+                    }
+                }
                 // constriction grid: atoms within the cutoff (main.cpp:2969-2973)
                 if (want_cmap && !bh) {
                     if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
@@ -951,7 +974,6 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                         }
                     }
                 }  // directional classes
-                const int indx_n = min(bin, AG_NEINT - 1);
                 const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
                 const float S = sol_s[indx_r];
                 // desolvation (main.cpp:3375-3390): slot m gains
@@ -960,26 +982,17 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 // instead of a term per slot, combined after the loop
                 dv_acc = __builtin_fmaf(unif(a.vol), S, dv_acc);
                 dq_acc = __builtin_fmaf(unif(a.solq), S, dq_acc);
-                // vdW/H-bond table values of this pair for the pass's slots
-                const float *row = tab + ((size_t)type * AG_NEINT + indx_n) * kMapSlots;
-                float E[kMapSlots];
-    #pragma unroll
-                for (int q4 = 0; q4 < kMapSlots / 4; q4++) {
-                    if (q4 < NPLAIN / 4 || q4 == kMapSlots / 4 - 1) {
-                        const float4 v = reinterpret_cast<const float4 *>(row)[q4];
-                        E[4 * q4] = v.x; E[4 * q4 + 1] = v.y; E[4 * q4 + 2] = v.z; E[4 * q4 + 3] = v.w;
-                    } else {
-                        E[4 * q4] = E[4 * q4 + 1] = E[4 * q4 + 2] = E[4 * q4 + 3] = 0.f;
-                    }
-                }
                 // plain slots: the table value
     #pragma unroll
                 for (int m = 0; m < NPLAIN; m++) acc[m] += E[m];
                 // H-bonding slots (main.cpp:3286-3374): the (probe, atom) case is a
-                // set of 0/1 coefficients chosen per atom class.  The class is the
-                // same for every lane, so the case is a scalar branch: a plain
-                // table add, nothing, or the directional mix.  A disordered
-                // partner uses the hydrogen-type row at r - 1.10 A
+                // set of 0/1 coefficients chosen per atom class, uniform across
+                // the wave.  The plain table add is fma(c, E, acc) with c = 1 or
+                // 0 (bit for bit acc + E, or acc); the directional mix runs in a
+                // scalar branch for the classes that have one, with every slot's
+                // update written without a branch of its own, so the
+                // accumulators and min/max keep their registers through it.  A
+                // disordered partner uses the hydrogen-type row at r - 1.10 A
                 // (main.cpp:3313-3349), FP64 like the reference.
                 const int dsh = dis ? 4 : 0;
                 bool any_mix = false;
@@ -987,14 +1000,10 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 for (int k = 0; k < kHbSlots; k++) {
                     const int code = (hw >> (8 * k + dsh)) & 15;
                     any_mix |= (code & ~HC_PLAIN) != 0;
+                    const float kP = __int_as_float((code & HC_PLAIN) ? 0x3f800000 : 0);
+                    acc[kPlainSlots + k] = __builtin_fmaf(kP, E[kPlainSlots + k], acc[kPlainSlots + k]);
                 }
-                if (!any_mix) {
-    #pragma unroll
-                    for (int k = 0; k < kHbSlots; k++) {
-                        const int code = (hw >> (8 * k + dsh)) & 15;
-                        if (code & HC_PLAIN) acc[kPlainSlots + k] += E[kPlainSlots + k];
-                    }
-                } else {
+                if (any_mix) {
                     float Eh[kHbSlots] = {0.f, 0.f, 0.f, 0.f};
                     if (dis && p.hydrogen_type >= 0) {
                         double rdx, rdy, rdz;
@@ -1010,19 +1019,17 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                         const int m = kPlainSlots + k;
                         const int code = (hw >> (8 * k + dsh)) & 15;
                         const float e = E[m];
-                        if (code & HC_PLAIN) acc[m] += e;
-                        if ((code & ~HC_PLAIN) == 0) continue;
                         // 0/1 coefficients as scalar float bit patterns
                         const float kF = __int_as_float((code & HC_RACC) ? 0x3f800000 : 0);
+                        const float kH = __int_as_float((code & HC_HRAMP) ? 0x3f800000 : 0);
+                        const bool mm = (code & HC_MINMAX) != 0;
                         const float rsph = __builtin_amdgcn_fmed3f(e * 0.01f, 0.f, 1.f);
                         const float f = __builtin_fmaf(kF, dfac, rdon);
                         const float mix = __builtin_fmaf(1.f - f, rsph, f);
-                        if (code & HC_HRAMP) acc[m] = __builtin_fmaf((dis ? Eh[k] : e) * Hramp, mix, acc[m]);
-                        if (code & HC_MINMAX) {
-                            const float v = e * mix;
-                            hmin[k] = fminf(hmin[k], v);
-                            hmax[k] = fmaxf(hmax[k], v);
-                        }
+                        acc[m] = __builtin_fmaf((kH * (dis ? Eh[k] : e)) * Hramp, mix, acc[m]);
+                        const float v = e * mix;
+                        hmin[k] = fminf(hmin[k], mm ? v : __builtin_inff());
+                        hmax[k] = fmaxf(hmax[k], mm ? v : -__builtin_inff());
                     }
                 }
                 if (p.d_slot >= 0) d_acc += unif(a.dvol) * S;
@@ -1556,12 +1563,17 @@ struct ag_plan {
     double coeff_estat = 0.0;
     int far_grid = 0;                  // persistent far-kernel grid
     bool paged = false;                // near kernel in its paged form (a brick overflowed one LDS list)
-    hipGraphExec_t graph = nullptr;    // captured map set (persistent mode)
+    // captured map set (persistent mode): the cell list and near field, then
+    // the far field, as two graphs launched back to back
+    hipGraphExec_t graph = nullptr, graph_far = nullptr;
     float *g_maps = nullptr, *g_fmap = nullptr;
     int32_t *g_cmap = nullptr;
-    ~ag_plan() {
+    void drop_graphs() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (graph_far) (void)hipGraphExecDestroy(graph_far);
+        graph = graph_far = nullptr;
     }
+    ~ag_plan() { drop_graphs(); }
 };
 
 // The far-kernel mode of conformations [c0, c1): one path for all, or mixed.
@@ -2189,15 +2201,19 @@ static std::vector<std::pair<int, int>> plan_chunks(const ag_plan &p, int want, 
     return out;
 }
 
+enum : int { kPartAll = 0, kPartNear = 1, kPartFar = 2 };
+
 // Enqueue one map set on the context's stream, conformation chunk by chunk.
 // With events, HIP events bracket the cell list, the near-field and the
 // far-field phases (single chunk).  With host outputs, each chunk's maps
 // are copied to the host on the context's copy stream as soon as the chunk
-// is done, while the next chunk computes.  mark_near records the context's
-// near_ev once the near field of every conformation is enqueued (single
-// chunk; in a stream capture, as an external event node of the graph).
+// is done, while the next chunk computes.  `part` (single chunk) enqueues
+// only the cell list and near field (kPartNear) or only the far field
+// (kPartFar); with kPartAll, mark_near records the context's near_ev once the
+// near field of every conformation is enqueued.
 static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far,
-                    const std::vector<std::pair<int, int>> &chunks, const HostOut *host, bool mark_near = false) {
+                    const std::vector<std::pair<int, int>> &chunks, const HostOut *host, bool mark_near = false,
+                    int part = kPartAll) {
     ag_ctx *c = p->ctx;
     hipStream_t st = c->stream;
     const int N = p->N;
@@ -2209,8 +2225,10 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
     int *d_queue = (int *)p->queue_d.ptr;
     const int nchunks = (int)chunks.size();
     if (nchunks < 1 || nchunks > kMaxChunks) ag::fail(AG_ERR_BAD_INPUT, "bad chunking of the batch");
+    if (part != kPartAll && (nchunks != 1 || host)) ag::fail(AG_ERR_BAD_INPUT, "bad chunking of the batch");
     const unsigned cgrid = (unsigned)((p->total_cells + tpb - 1) / tpb);
     if (events) HIP_CHECK(hipEventRecord(c->ev[0], st));
+    if (part != kPartFar) {
     k_zero_cells<<<std::max(cgrid, (unsigned)((kMaxChunks + tpb - 1) / tpb)), tpb, 0, st>>>(
         d_count, d_cursor, p->total_cells, d_status, d_queue, kMaxChunks);
     if (N > 0)
@@ -2225,6 +2243,7 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
         k_gather<<<(N + tpb - 1) / tpb, tpb, 0, st>>>(d_start + p->total_cells, d_slot_atom, p->src,
                                                       (NearAtom *)p->near_atoms.ptr);
     HIP_CHECK(hipGetLastError());
+    }
     if (events) HIP_CHECK(hipEventRecord(c->ev[1], st));
     // The near field (near kernel, floating grid, covalent and zero maps) of
     // conformations [c0, c1) on stream s.
@@ -2338,17 +2357,10 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
         }
     } else {
         for (int k = 0; k < nchunks; k++) {
-            near_pass(chunks[k].first, chunks[k].second, st_main);
+            if (part != kPartFar) near_pass(chunks[k].first, chunks[k].second, st_main);
             if (events && k == nchunks - 1) HIP_CHECK(hipEventRecord(c->ev[2], st_main));
-            if (mark_near && k == nchunks - 1) {
-                hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-                HIP_CHECK(hipStreamIsCapturing(st_main, &cs));
-                if (cs == hipStreamCaptureStatusActive)
-                    HIP_CHECK(hipEventRecordWithFlags(c->near_ev, st_main, hipEventRecordExternal));
-                else
-                    HIP_CHECK(hipEventRecord(c->near_ev, st_main));
-            }
-            far_pass(k, chunks[k].first, chunks[k].second, st_main);
+            if (mark_near && k == nchunks - 1) HIP_CHECK(hipEventRecord(c->near_ev, st_main));
+            if (part != kPartNear) far_pass(k, chunks[k].first, chunks[k].second, st_main);
             if (host) copy_out(k, chunks[k].first, chunks[k].second, st_main);
         }
     }
@@ -2363,8 +2375,8 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
 }
 
 static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far,
-                    bool mark_near = false) {
-    enqueue(p, d_maps, d_fmap, d_cmap, events, skip_far, {{0, p->nconf}}, nullptr, mark_near);
+                    bool mark_near = false, int part = kPartAll) {
+    enqueue(p, d_maps, d_fmap, d_cmap, events, skip_far, {{0, p->nconf}}, nullptr, mark_near, part);
 }
 
 // Whether a brick of the last map set overflowed the near kernel's LDS lists
@@ -2389,10 +2401,7 @@ static void run_checked(ag_plan *p, F &&run) {
     if (!overflowed(p)) return;
     if (p->paged) ag::fail(AG_ERR_DEVICE, "the paged near kernel reported an overflow");
     p->paged = true;
-    if (p->graph) {
-        HIP_CHECK(hipGraphExecDestroy(p->graph));
-        p->graph = nullptr;
-    }
+    p->drop_graphs();
     run();
     if (overflowed(p)) ag::fail(AG_ERR_DEVICE, "the paged near kernel reported an overflow");
 }
@@ -2503,9 +2512,31 @@ void plan_destroy(ag_plan *p) {
 // One map set (all conformations of the plan) into d_maps
 // ([sum n_maps * P] float, device) and, when asked for, d_fmap / d_cmap
 // ([sum P]), enqueued on the context's stream.  A single plan runs its
-// kernels directly with phase timing; a batch replays one captured hipGraph
-// (re-captured when the outputs move).  Either way the context's near_ev is
-// recorded once the near field is done.
+// kernels directly with phase timing; a batch replays two captured hipGraphs
+// (re-captured when the outputs move): the cell list and near field, then
+// the persistent far field.  Either way the context's near_ev is recorded
+// once the near field is done (between the two graphs for a batch).
+static hipGraphExec_t capture(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, int part) {
+    hipStream_t st = p->ctx->stream;
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue(p, d_maps, d_fmap, d_cmap, false, false, false, part);
+    } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(st, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(st, &g));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+        ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
+    return exec;
+}
+
 static void launch_mapset(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
     ag_ctx *c = p->ctx;
     hipStream_t st = c->stream;
@@ -2514,33 +2545,17 @@ static void launch_mapset(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_c
         return;
     }
     if (!p->graph || p->g_maps != d_maps || p->g_fmap != d_fmap || p->g_cmap != d_cmap) {
-        if (p->graph) {
-            HIP_CHECK(hipGraphExecDestroy(p->graph));
-            p->graph = nullptr;
-        }
-        hipGraph_t g = nullptr;
-        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        try {
-            enqueue(p, d_maps, d_fmap, d_cmap, false, false, true);
-        } catch (...) {
-            hipGraph_t dead = nullptr;
-            (void)hipStreamEndCapture(st, &dead);
-            if (dead) (void)hipGraphDestroy(dead);
-            throw;
-        }
-        HIP_CHECK(hipStreamEndCapture(st, &g));
-        hipError_t e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            p->graph = nullptr;
-            ag::fail(AG_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate");
-        }
+        p->drop_graphs();
+        p->graph = capture(p, d_maps, d_fmap, d_cmap, kPartNear);
+        p->graph_far = capture(p, d_maps, d_fmap, d_cmap, kPartFar);
         p->g_maps = d_maps;
         p->g_fmap = d_fmap;
         p->g_cmap = d_cmap;
     }
     HIP_CHECK(hipEventRecord(c->ev[0], st));
     HIP_CHECK(hipGraphLaunch(p->graph, st));
+    HIP_CHECK(hipEventRecord(c->near_ev, st));
+    HIP_CHECK(hipGraphLaunch(p->graph_far, st));
     HIP_CHECK(hipEventRecord(c->ev[3], st));
 }
 

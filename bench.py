@@ -25,8 +25,9 @@ Also reported on the same JSON line, each with an `*_is` label:
                 of 41-155 points, maps A C HD N NA OA + e + d) against 64
                 sequential single-receptor runs (tables, bond vectors,
                 ag_maps_plain with its uploads and copies), both without
-                ASCII output: one captured-graph launch (one-shot and
-                resident replays), and the chunked pinned-output pipeline.
+                ASCII output: the captured graphs with their host copies
+                (one-shot and resident replays), and the chunked
+                pinned-output pipeline.
   large         the 20,000-atom, 257^3 map set (seconds per map set,
                 kernel times, pass-E pairs/s).
   analytic      the differentiable mode on the full receptor with a 65^3
@@ -252,9 +253,9 @@ def batched_leg(reps: int = 2) -> dict:
 
             def graph_once(tune):
                 # host precompute, ag_batch_create, one launch of the captured
-                # graph (cell list, near pass, persistent far kernel) with the
-                # maps other than e copied to pinned memory during the far
-                # kernel and the e maps after it
+                # graphs (cell list and near pass, then the persistent far
+                # kernel) with the maps other than e copied to pinned memory
+                # during the far kernel and the e maps after it
                 with ctx.batch(batch_states(), tune=tune) as b:
                     assert b.map_floats == total
                     b.execute_host(dmaps.value, outs)
@@ -312,10 +313,10 @@ def batched_leg(reps: int = 2) -> dict:
                              "D2H into pinned memory), no ASCII",
             "graph_s": tg, "graph_speedup": ts / tg,
             "graph_is": "host tables + bond vectors for all conformations, ag_batch_create, ag_plan_execute_host: "
-                        "one captured-graph launch (cell list, near pass, persistent far kernel), the maps other "
+                        "the captured graphs (cell list and near pass, then the persistent far kernel), the maps other "
                         "than e copied into pinned memory during the far kernel, the e maps after it",
             "graph_replay_s": tr, "graph_replay_speedup": ts / tr, "graph_replay_gpu_ms": replay_gpu_ms,
-            "graph_replay_is": "ag_plan_execute_host replay of the resident batch plan's graph with its copies "
+            "graph_replay_is": "ag_plan_execute_host replay of the resident batch plan's graphs with their copies "
                                "into pinned memory",
             "chunked_s": tb, "chunked_speedup": ts / tb,
             "chunked_is": "ag_maps_batch with pinned outputs: host precompute, 10 chunks on two compute streams, "

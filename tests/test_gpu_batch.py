@@ -1,7 +1,7 @@
 """Batched map sets (ag_batch_create / ag_maps_batch) on the GPU.
 
 A batch runs many receptor conformations, with their own atoms and boxes,
-through one captured graph.  For a fixed near-kernel layout with the plain
+through one replay of its captured graphs.  For a fixed near-kernel layout with the plain
 path's 4 A cells every per-point sum runs in the same order as in a single
 plan, so each conformation's maps must equal ag_maps_plain's bit for bit;
 other cell sizes reorder the FP32 sums and are held to the oracle under the
@@ -245,10 +245,10 @@ def test_batch_pinned_outputs_with_distance_grids(ensemble12):
 
 @pytest.mark.parametrize("bhtree", [0, 1])
 def test_plan_execute_host_equals_device_outputs(ensemble12, bhtree):
-    """ag_plan_execute_host: the batch's graph, with every map but e (and
+    """ag_plan_execute_host: the batch's graphs, with every map but e (and
     cmap, and fmap unless the tree build's collision rule rewrites it after
-    the far kernel) copied to the host behind the event the graph records
-    after the near field, the rest after the graph.  Maps, fmap and cmap
+    the far kernel) copied to the host behind the event recorded after the
+    near-field graph, the rest after the far-field graph.  Maps, fmap and cmap
     must equal ag_plan_execute's device outputs bit for bit: into pinned
     destinations on two launches (the second a replay) and into pageable
     ones.  A single-conformation plan must equal ag_maps_plain."""

@@ -2,7 +2,7 @@
 
 Batched ensemble: 64 receptor conformations (seed 1005) of 1,594-11,710
 atoms with cubic boxes of 41-155 points, maps A C HD N NA OA + e + d.
-  * the one-graph path (ag_batch_create, then ag_plan_execute, and
+  * the graph path (ag_batch_create, then ag_plan_execute, and
     ag_plan_execute_host into pinned memory) and the chunked path with
     pinned host outputs (ag_maps_batch) each give every
     conformation's maps bit for bit equal to ag_maps_plain's, at the plain

@@ -1969,14 +1969,19 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         else if (!vina && (role & ROLE_AFF) && ms->maps[m].is_hbonder) hb_maps.push_back(m);
         else plain_maps.push_back(m);
     }
-    p.groups = std::max<int>(1, std::max((int)(plain_maps.size() + kPlainSlots - 1) / kPlainSlots,
-                                         (int)(hb_maps.size() + kHbSlots - 1) / kHbSlots));
+    // A pass has kPlainSlots plain slots and kHbSlots H-bond slots; plain
+    // maps beyond the plain slots take free H-bond slots, whose code is then
+    // the plain table add for every atom class (13 Vina maps: one pass).
+    const int n_plain = (int)plain_maps.size(), n_hb = (int)hb_maps.size();
+    p.groups = std::max({1, (n_hb + kHbSlots - 1) / kHbSlots, (n_plain + n_hb + kMapSlots - 1) / kMapSlots});
     const size_t gstride = (size_t)(T > 0 ? T : 1) * AG_NEINT * kMapSlots;
     std::vector<float> gtab(gstride * p.groups, 0.f);
     std::vector<std::array<int, kMapSlots>> slot_map(p.groups);
+    std::vector<std::array<bool, kHbSlots>> hb_plain(p.groups);
     int nplain_max = 0;
     for (int g = 0; g < p.groups; g++) {
         slot_map[g].fill(-1);
+        hb_plain[g].fill(false);
         int np_g = 0;
         for (int j = 0; j < kPlainSlots; j++) {
             const size_t i = (size_t)g * kPlainSlots + j;
@@ -1987,6 +1992,19 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
             const size_t i = (size_t)g * kHbSlots + k;
             if (i < hb_maps.size()) slot_map[g][kPlainSlots + k] = hb_maps[i];
         }
+    }
+    for (size_t i = (size_t)p.groups * kPlainSlots; i < plain_maps.size(); i++) {
+        bool placed = false;
+        for (int g = 0; g < p.groups && !placed; g++)
+            for (int k = 0; k < kHbSlots && !placed; k++)
+                if (slot_map[g][kPlainSlots + k] < 0) {
+                    slot_map[g][kPlainSlots + k] = plain_maps[i];
+                    hb_plain[g][k] = true;
+                    placed = true;
+                }
+        if (!placed) ag::fail(AG_ERR_UNSUPPORTED, "near-kernel slot assignment left a map without a slot");
+    }
+    for (int g = 0; g < p.groups; g++) {
         for (int t = 0; t < T; t++)
             for (int j = 0; j < kMapSlots; j++) {
                 const int m = slot_map[g][j];
@@ -2032,6 +2050,10 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
             if (j >= kPlainSlots) {
                 const int k = j - kPlainSlots;
                 for (int ah = 0; ah <= 6; ah++) {
+                    if (hb_plain[g][k]) {   // a plain map in an H-bond slot
+                        q.hcode[k][ah] = HC_PLAIN | (HC_PLAIN << 4);
+                        continue;
+                    }
                     const int cse = hb_case(md.hbond, ah);
                     int ordered = 0, disordered = 0;
                     if (cse == HBC_PLAIN) ordered = disordered = HC_PLAIN;

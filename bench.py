@@ -17,7 +17,8 @@ Also reported on the same JSON line, each with an `*_is` label:
                 D2H into pinned memory), and the autogrid4 drop-in end to
                 end with its ASCII output.
   accuracy      the GPU maps against the reference autogrid4's .map text
-                (every value) and the FP64 oracle (3 z-planes); W_A.
+                (every value) and the FP64 oracle (3 z-planes); W_A; the
+                Vina-potential maps against the oracle (3 z-planes).
   cpu_baseline  the reference autogrid4 (built from its sources with
                 OpenMP) on the whole workload on this host's cores, and a
                 labelled z-slab of the large workload scaled by point count.
@@ -29,7 +30,11 @@ Also reported on the same JSON line, each with an `*_is` label:
                 (one-shot and resident replays), and the chunked
                 pinned-output pipeline.
   large         the 20,000-atom, 257^3 map set (seconds per map set,
-                kernel times, pass-E pairs/s).
+                kernel times, far-field pairs/s).
+  protein       the reference's x1hpv receptor at protein density on a
+                121^3 box with its CASE0 map set.
+  vina          the full receptor and box with the Vina potential
+                (calc_vina_potential's tables): map set and kernel times.
   analytic      the differentiable mode on the full receptor with a 65^3
                 box: forward and forward+adjoint times (cotangent from 256
                 ligand poses) and a 200-step Adam fit of the AD4 weights.
@@ -106,7 +111,7 @@ def read_map_values(path: str):
     return np.array(body.split(), dtype=np.float64)
 
 
-def cpu_leg(st, gpu_maps, map_files, large: bool = True) -> dict:
+def cpu_leg(st, gpu_maps, map_files, large: bool = True, vina=None) -> dict:
     """The reference on the host's cores (the CPU baseline) and the checks
     of the GPU map set against it and the FP64 oracle.
 
@@ -166,6 +171,22 @@ def cpu_leg(st, gpu_maps, map_files, large: bool = True) -> dict:
                              "planes": [0, nz // 2, nz - 1], "tolerance": "max(1e-3, 1e-3 |ref|)"}
     acc["W_A"] = ol.count_pairs(st)
     acc["W_E"] = P * N
+    if vina is not None:
+        # the Vina-potential map set of the same box against the oracle's
+        # Vina tables and sums, at every point of 3 z-planes
+        vst, vmaps = vina
+        worst, sq, cnt, over = 0.0, 0.0, 0, 0
+        for z in (0, nz // 2, nz - 1):
+            ref = ol.maps_vina(vst, z, z + 1)[:, z].reshape(vst.n_maps, -1)
+            g = vmaps[:, z].reshape(vst.n_maps, -1).astype(np.float64)
+            err = np.abs(g - ref)
+            worst = max(worst, float(err.max()))
+            sq += float(np.sum(err ** 2))
+            cnt += err.size
+            over += int(np.count_nonzero(err > np.maximum(1e-3, 1e-3 * np.abs(ref))))
+        acc["vina_vs_fp64_oracle"] = {"max_abs": worst, "rms": (sq / cnt) ** 0.5, "values": cnt,
+                                      "over_tolerance": over, "planes": [0, nz // 2, nz - 1],
+                                      "tolerance": "max(1e-3, 1e-3 |ref|)"}
     out["accuracy"] = acc
     if large:
         from autogrid_amd import synth
@@ -414,6 +435,57 @@ def protein_leg(reps: int = 5) -> dict:
                                                "far": tms[k]["far_ms"]}}
 
 
+def vina_leg(reps: int = 10):
+    """The full workload's receptor and 127^3 box with the Vina potential
+    (calc_vina_potential, main.cpp:1975-2127): Vina tables from the front
+    end (ag_job_prepare_vina), the same 13 affinity maps, from
+    device-resident inputs.  The e and d maps are zero under this potential,
+    so the map set is the cell list and the near kernel.  Returns the leg's
+    figures, the state and the maps (for the accuracy check of the CPU leg)."""
+    import numpy as np
+
+    import autogrid_amd as ag
+    from autogrid_amd import synth
+
+    work = tempfile.mkdtemp(prefix="agvina_")
+    try:
+        info = synth.make_workload("full", work)
+        cwd = os.getcwd()
+        os.chdir(work)
+        try:
+            st = ag.Job(info["gpf"]).prepare_vina().state()
+        finally:
+            os.chdir(cwd)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    hip = hip_runtime()
+    nbytes = 4 * st.n_maps * st.points
+    dptr = C.c_void_p()
+    if hip.hipMalloc(C.byref(dptr), C.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMalloc failed")
+    try:
+        with ag.Context(0) as ctx, ctx.plan(st, potential=ag.AG_POTENTIAL_VINA) as plan:
+            plan.execute(dptr.value)
+            wall, tms = [], []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                plan.execute(dptr.value)
+                wall.append(time.perf_counter() - t0)
+                tms.append(ctx.timing())
+        maps = np.empty((st.n_maps, st.n1[2], st.n1[1], st.n1[0]), np.float32)
+        if hip.hipMemcpy(maps.ctypes.data_as(C.c_void_p), dptr, C.c_size_t(nbytes), 2) != 0:
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(dptr)
+    k = min(range(reps), key=lambda i: wall[i])
+    near = sum(t["near_ms"] for t in tms) / reps
+    leg = {"workload": f"full with Vina tables: {st.n_atoms} atoms, {st.n1[0]}^3 points, {st.n_maps} maps",
+           "map_set_s": wall[k], "kernels_ms": {"cell_list": tms[k]["cell_ms"], "near": near},
+           "pair_evals_per_s": st.points * st.n_atoms / (wall[k]),
+           "pair_evals_per_s_is": "all P*N (point, atom) pairs of the map set over its wall time"}
+    return leg, st, maps
+
+
 def analytic_leg(reps: int = 10, fit_steps: int = 200) -> dict:
     """Analytic mode on the full workload's receptor with a 65^3 box (maps
     A C HD N OA + e + d): forward and forward+adjoint times with the
@@ -508,6 +580,7 @@ def main():
     ap.add_argument("--no-analytic", action="store_true", help="skip the analytic forward/adjoint leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 20,000-atom, 257^3 map set")
     ap.add_argument("--no-protein", action="store_true", help="skip the x1hpv receptor at protein density")
+    ap.add_argument("--no-vina", action="store_true", help="skip the Vina-potential map set of the full box")
     ap.add_argument("--fit-steps", type=int, default=200, help="Adam steps of the analytic leg's fit (0: none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration) and exit")
@@ -620,11 +693,15 @@ def main():
 
     large = None if args.no_large else large_leg()
     protein = None if args.no_protein else protein_leg()
+    vina, vina_state = None, None
+    if not args.no_vina and args.workload == "full":
+        vina, vst, vmaps = vina_leg()
+        vina_state = (vst, vmaps)
     batch = None if args.no_batch else batched_leg()
     analytic = None if args.no_analytic else analytic_leg(fit_steps=args.fit_steps)
     cpu = {"cpu_baseline": None, "accuracy": None}
     if not args.no_cpu_baseline and args.workload == "full":
-        cpu = cpu_leg(st, gpu_maps, info["maps"], large=not args.no_large)
+        cpu = cpu_leg(st, gpu_maps, info["maps"], large=not args.no_large, vina=vina_state)
     value = ms_per_step / 1000.0
     out = {
         "metric": "map-set wall s (5k atoms,126^3,14 maps); batched & fwd+adj vs plain GPU",
@@ -664,6 +741,7 @@ def main():
         "accuracy": cpu["accuracy"],
         "large": large,
         "protein": protein,
+        "vina": vina,
         "batched": batch,
         "analytic": analytic,
         "cpu_baseline": cpu["cpu_baseline"],

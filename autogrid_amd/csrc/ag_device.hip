@@ -1564,14 +1564,18 @@ struct ag_plan {
     int far_grid = 0;                  // persistent far-kernel grid
     bool paged = false;                // near kernel in its paged form (a brick overflowed one LDS list)
     // captured map set (persistent mode): the cell list and near field, then
-    // the far field, as two graphs launched back to back
-    hipGraphExec_t graph = nullptr, graph_far = nullptr;
+    // the far field of conformations [0, far_split) and of [far_split,
+    // nconf), as graphs launched back to back (the second far graph is
+    // empty when far_split == nconf)
+    hipGraphExec_t graph = nullptr, graph_far = nullptr, graph_far2 = nullptr;
+    int far_split = 0;
     float *g_maps = nullptr, *g_fmap = nullptr;
     int32_t *g_cmap = nullptr;
     void drop_graphs() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graph_far) (void)hipGraphExecDestroy(graph_far);
-        graph = graph_far = nullptr;
+        if (graph_far2) (void)hipGraphExecDestroy(graph_far2);
+        graph = graph_far = graph_far2 = nullptr;
     }
     ~ag_plan() { drop_graphs(); }
 };
@@ -1613,6 +1617,7 @@ ag_status context_create(ag_ctx **out, int device) {
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreateWithFlags(&c->cells_ev, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&c->near_ev, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&c->far_ev, hipEventDisableTiming));
     for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
     for (auto &e : c->chunk_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
@@ -1643,6 +1648,7 @@ void context_destroy(ag_ctx *c) {
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->cells_ev) (void)hipEventDestroy(c->cells_ev);
     if (c->near_ev) (void)hipEventDestroy(c->near_ev);
+    if (c->far_ev) (void)hipEventDestroy(c->far_ev);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -2175,6 +2181,29 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
         fp.e_slot = ms->elec_index;
     }
     plan_geometry(p, grids, natoms.data(), p.run_far ? far_flags.data() : nullptr);
+    // A batch's far field is captured as two graphs: conformations [0,
+    // far_split), then the rest, so that the first part's e maps can go to
+    // the host while the second part computes.  The second part is sized so
+    // that its far field (at about 5.5e12 pairs/s) lasts about as long as
+    // that copy (at about 45 GB/s): what is left after the map set is the
+    // second part's e maps.
+    p.far_split = n;
+    if (persistent && n > 1 && p.run_far) {
+        double work = 0.0, ebytes = 0.0;
+        for (const ConfDev &cf : p.confs) {
+            work += (double)cf.P * cf.n_atoms;
+            ebytes += 4.0 * (double)cf.P;
+        }
+        const double t_far = work / 5.5e12, t_copy = ebytes / 45e9;
+        const double want_b = work * t_copy / (t_far + t_copy);
+        double acc = 0.0;
+        int split = n;
+        while (split > 1 && acc < want_b) {
+            --split;
+            acc += (double)p.confs[split].P * p.confs[split].n_atoms;
+        }
+        p.far_split = split;
+    }
     HIP_CHECK(hipStreamSynchronize(st));   // host staging vectors go out of scope
     return pl.release();
 }
@@ -2235,7 +2264,7 @@ enum : int { kPartAll = 0, kPartNear = 1, kPartFar = 2 };
 // near field of every conformation is enqueued.
 static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, bool events, bool skip_far,
                     const std::vector<std::pair<int, int>> &chunks, const HostOut *host, bool mark_near = false,
-                    int part = kPartAll) {
+                    int part = kPartAll, int queue_base = 0) {
     ag_ctx *c = p->ctx;
     hipStream_t st = c->stream;
     const int N = p->N;
@@ -2328,7 +2357,7 @@ static void enqueue(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, b
             fp.maps = d_maps;
             fp.item_base = t0;
             fp.item_end = t1;
-            fp.queue = p->persistent ? d_queue + k : nullptr;
+            fp.queue = p->persistent ? d_queue + queue_base + k : nullptr;
             const int blocks = p->persistent ? std::min(p->far_grid, t1 - t0) : t1 - t0;
             launch_far(blocks, fp, far_mode(*p, c0, c1), s);
         }
@@ -2534,16 +2563,18 @@ void plan_destroy(ag_plan *p) {
 // One map set (all conformations of the plan) into d_maps
 // ([sum n_maps * P] float, device) and, when asked for, d_fmap / d_cmap
 // ([sum P]), enqueued on the context's stream.  A single plan runs its
-// kernels directly with phase timing; a batch replays two captured hipGraphs
+// kernels directly with phase timing; a batch replays captured hipGraphs
 // (re-captured when the outputs move): the cell list and near field, then
-// the persistent far field.  Either way the context's near_ev is recorded
-// once the near field is done (between the two graphs for a batch).
-static hipGraphExec_t capture(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, int part) {
+// the persistent far field in two parts (conformations split at
+// far_split).  Either way the context's near_ev is recorded once the near
+// field is done; a batch records far_ev after its first far-field part.
+static hipGraphExec_t capture(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, int part, int c0, int c1,
+                              int queue_base) {
     hipStream_t st = p->ctx->stream;
     hipGraph_t g = nullptr;
     HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     try {
-        enqueue(p, d_maps, d_fmap, d_cmap, false, false, false, part);
+        enqueue(p, d_maps, d_fmap, d_cmap, false, false, {{c0, c1}}, nullptr, false, part, queue_base);
     } catch (...) {
         hipGraph_t dead = nullptr;
         (void)hipStreamEndCapture(st, &dead);
@@ -2568,8 +2599,10 @@ static void launch_mapset(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_c
     }
     if (!p->graph || p->g_maps != d_maps || p->g_fmap != d_fmap || p->g_cmap != d_cmap) {
         p->drop_graphs();
-        p->graph = capture(p, d_maps, d_fmap, d_cmap, kPartNear);
-        p->graph_far = capture(p, d_maps, d_fmap, d_cmap, kPartFar);
+        p->graph = capture(p, d_maps, d_fmap, d_cmap, kPartNear, 0, p->nconf, 0);
+        p->graph_far = capture(p, d_maps, d_fmap, d_cmap, kPartFar, 0, p->far_split, 0);
+        if (p->far_split < p->nconf)
+            p->graph_far2 = capture(p, d_maps, d_fmap, d_cmap, kPartFar, p->far_split, p->nconf, 1);
         p->g_maps = d_maps;
         p->g_fmap = d_fmap;
         p->g_cmap = d_cmap;
@@ -2578,6 +2611,8 @@ static void launch_mapset(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_c
     HIP_CHECK(hipGraphLaunch(p->graph, st));
     HIP_CHECK(hipEventRecord(c->near_ev, st));
     HIP_CHECK(hipGraphLaunch(p->graph_far, st));
+    HIP_CHECK(hipEventRecord(c->far_ev, st));
+    if (p->graph_far2) HIP_CHECK(hipGraphLaunch(p->graph_far2, st));
     HIP_CHECK(hipEventRecord(c->ev[3], st));
 }
 
@@ -2607,8 +2642,9 @@ void plan_execute(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap) {
 // tree build's collision rule, which also sets fmap), so the copy stream
 // takes the other maps, cmap and -- without the collision rule -- fmap as
 // soon as near_ev is reached, while the far kernel runs; the e maps (and
-// fmap) follow the end of the map set.  Host memory pinned with
-// hipHostMalloc lets the copies overlap the kernels.
+// fmap) of a batch's first far-field graph follow far_ev, during its second
+// far-field graph, and the rest the end of the map set.  Host memory pinned
+// with hipHostMalloc lets the copies overlap the kernels.
 void plan_execute_host(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, float *const *maps_out,
                        float *const *fmap_out, int32_t *const *cmap_out) {
     if ((p->want_fmap && !d_fmap) || (p->want_cmap && !d_cmap))
@@ -2642,14 +2678,24 @@ void plan_execute_host(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap
             if (d_cmap) copy(cmap_out[i], d_cmap + cf.pt_off, sizeof(int32_t) * (size_t)cf.P);
             if (d_fmap && !fmap_late) copy(fmap_out[i], d_fmap + cf.pt_off, pb);
         }
+        // the e maps (and late fmaps) of a batch's first far-field graph,
+        // then of the rest
+        const auto late = [&](int c0, int c1) {
+            for (int i = c0; i < c1; i++) {
+                const ConfDev &cf = p->confs[i];
+                const size_t pb = sizeof(float) * (size_t)cf.P;
+                if (e >= 0) copy(maps_out[i] + (size_t)e * cf.P, d_maps + cf.map_off + (size_t)e * cf.P, pb);
+                if (d_fmap && fmap_late) copy(fmap_out[i], d_fmap + cf.pt_off, pb);
+            }
+        };
+        const int split = p->persistent ? p->far_split : p->nconf;
+        if (split < p->nconf) {
+            HIP_CHECK(hipStreamWaitEvent(cs, c->far_ev, 0));
+            late(0, split);
+        }
         HIP_CHECK(hipEventRecord(c->chunk_ev[0], st));
         HIP_CHECK(hipStreamWaitEvent(cs, c->chunk_ev[0], 0));
-        for (int i = 0; i < p->nconf; i++) {
-            const ConfDev &cf = p->confs[i];
-            const size_t pb = sizeof(float) * (size_t)cf.P;
-            if (e >= 0) copy(maps_out[i] + (size_t)e * cf.P, d_maps + cf.map_off + (size_t)e * cf.P, pb);
-            if (d_fmap && fmap_late) copy(fmap_out[i], d_fmap + cf.pt_off, pb);
-        }
+        late(split < p->nconf ? split : 0, p->nconf);
         HIP_CHECK(hipEventRecord(c->chunk_ev[kMaxChunks], cs));
         HIP_CHECK(hipStreamWaitEvent(st, c->chunk_ev[kMaxChunks], 0));
     });

@@ -276,9 +276,9 @@ ag_status ag_plan_execute(ag_plan *plan, float *maps_dev, float *fmap_dev, int32
  * fmap_out[i] / cmap_out[i] ([P_i]).  The device buffers are the work space
  * and hold the same values afterwards.  Every map but e is final once the
  * near field is done, so those copies run on a copy stream while the
- * far-field kernel computes (behind an event recorded between a batch's
- * near-field and far-field graphs); the e maps follow the end of the map
- * set.  With pinned host outputs
+ * far-field kernel computes (behind an event recorded after a batch's
+ * near-field graph); a batch's far field runs in two graphs, and the e
+ * maps of the first follow it, the rest the end of the map set.  With pinned host outputs
  * (hipHostMalloc) the copies overlap the kernels.  Returns after the
  * outputs are written.  It replaces the reference's hand-over of the
  * finished gridmap[j].energy arrays to its map writer (main.cpp:3474-3497),
@@ -397,10 +397,10 @@ ag_status ag_bond_vectors_vjp(const ag_setup *setup, int n_atoms, const double *
 /* ------------------------------------------------------------------------ */
 /* The reference computes one receptor per process run; a batch computes n
  * receptors (conformations: their own atoms, coordinates and box, sharing
- * the map set and tables) in one replay of two captured hipGraphs: the cell
+ * the map set and tables) in one replay of captured hipGraphs: the cell
  * list and one near-field launch over the bricks of every conformation,
- * then a persistent far-field kernel whose workgroups pull the tiles of all
- * conformations from a work counter.  Outputs are
+ * then a persistent far-field kernel, in two parts, whose workgroups pull
+ * the tiles of their conformations from a work counter.  Outputs are
  * concatenated: conformation i's maps start at map_offsets[i] floats
  * ([n_maps][P_i]), its fmap / cmap values at point_offsets[i]. */
 typedef struct ag_tune {

@@ -521,16 +521,14 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     __shared__ double hpos[kHCap][3];
     __shared__ int hidx[kHCap];
     __shared__ float hrvs[kHCap][3];
-    __shared__ float reps_s[kNearRepsLen];
-    __shared__ float sol_s[AG_NEINT + 1];
+    __shared__ float2 rs_s[kNearRepsLen];   // (electrostatic factor, sol_fn) per bin
     __shared__ int wave_tot[kNearThreads / 64];
     __shared__ int cell_off[kNearThreads + 1];
     __shared__ int cstart_s[kNearThreads];
     __shared__ int ncand_s, nh_s;
 
     const int tid = threadIdx.x;
-    for (int k = tid; k < kNearRepsLen; k += kNearThreads) reps_s[k] = p.reps[k];
-    for (int k = tid; k <= AG_NEINT; k += kNearThreads) sol_s[k] = p.sol[k];
+    for (int k = tid; k < kNearRepsLen; k += kNearThreads) rs_s[k] = make_float2(p.reps[k], p.sol[min(k, AG_NEINT)]);
     int lx, ly, lz;
     brick_lane<CUBE>(tid, lx, ly, lz);
 
@@ -796,9 +794,11 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 const float dxf = (float)dx64, dyf = (float)dy64, dzf = (float)dz64;
                 const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
                 if (r2 > 64.33f) continue;  // beyond 8.02 A: neither cutoff pass nor near e term
-                // hardware rsq: 128 r is good to ~3e-4 bins here, inside the guard band
+                // hardware rsq: 128 r is good to ~3e-4 bins here, inside the guard band;
+                // the same rsq is 1/r for the near e term and the unit vector
                 const float r2c = fmaxf(r2, 1.0e-12f);
-                float r = r2c * __builtin_amdgcn_rsqf(r2c);
+                float inv_r = __builtin_amdgcn_rsqf(r2c);
+                float r = r2c * inv_r;
                 const float g = r * 128.f;
                 int bin;
                 bool cut;
@@ -808,6 +808,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                     bin = (int)(r64 * 128.0);
                     cut = r64 > 8.0;
                     r = (float)r64;
+                    inv_r = __builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f));
                 } else {
                     bin = (int)g;
                     cut = r > 8.0f;
@@ -817,6 +818,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 // squared distance against the rounded squared radius
                 bool in8 = false;
                 float rB = 0.f;
+                int bin_bh = 0;
                 if (bh) {
                     // the tree build's query point: the grid point as float (main.cpp:2860)
                     const float e0 = (float)cx - (float)ax, e1 = (float)cy - (float)ay, e2 = (float)cz - (float)az;
@@ -826,6 +828,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                     if (a0 <= 1.5f && a1 <= 1.5f && a2 <= 1.5f && D <= 2.25f) collide = true;
                     if (want_cmap) ccount += a0 <= ccf && a1 <= ccf && a2 <= ccf && D <= ccf2;
                     rB = sqrtf(D);   // correctly rounded, as the tree's sqrt of a float
+                    bin_bh = (int)((double)rB * 128.0);
                 }
                 // The pair's vdW/H-bond table row for the pass's slots, loaded
                 // here so that the loads are in flight during the constriction,
@@ -834,7 +837,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 // from it; the bin is the one Loop 2 uses (the tree build's FP32
                 // distance in that mode).  Every lane here has r < 8.03 A, so
                 // the row index stays inside the table.
-                const int indx_n = min(bh ? (int)((double)rB * 128.0) : bin, AG_NEINT - 1);
+                const int indx_n = min(bh ? bin_bh : bin, AG_NEINT - 1);
                 float E[kMapSlots];
                 {
                     const char *trow =
@@ -850,6 +853,8 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                         }
                     }
                 }
+                // electrostatic factor and sol_fn of the bin, one LDS read
+                const float2 rsv = rs_s[min(bin, kNearRepsLen - 1)];
                 // constriction grid: atoms within the cutoff (main.cpp:2969-2973)
                 if (want_cmap && !bh) {
                     if (fabsf(r - (float)p.ccut) < 1.0e-5f) {
@@ -875,8 +880,8 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                     }
                     if (near_e) {
                         // 1/max(r, 0.5) with r clamped at APPROX_ZERO
-                        const float inv_rmax = fminf(__builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f)), 2.f);
-                        e_near += unif(a.qe) * inv_rmax * reps_s[min(bin, kNearRepsLen - 1)];
+                        const float inv_rmax = fminf(inv_r, 2.f);
+                        e_near += unif(a.qe) * inv_rmax * rsv.x;
                     }
                 }
                 // Loop 2 of the tree build takes the atoms its 8 A query returns,
@@ -884,7 +889,8 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 if (bh) {
                     cut = !in8;
                     r = rB;
-                    bin = (int)((double)rB * 128.0);
+                    bin = bin_bh;
+                    inv_r = __builtin_amdgcn_rcpf(fmaxf(rB, 1.0e-6f));
                 }
                 if (cut || !p.has_affinity) continue;
                 const int meta = ameta;
@@ -894,7 +900,6 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 float racc = 1.f, rdon = 1.f, Hramp = 1.f;
                 // directional classes only: the unit vector to the point
                 if (hb == AG_HB_D1 || hb == AG_HB_A2 || hb == AG_HB_A1) {
-                    const float inv_r = __builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f));
                     const float dn0 = dxf * inv_r, dn1 = dyf * inv_r, dn2 = dzf * inv_r;
                     const float rv0 = unif(a.rv[0]), rv1 = unif(a.rv[1]), rv2 = unif(a.rv[2]);
                     const int aidx = aindex;
@@ -974,8 +979,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                         }
                     }
                 }  // directional classes
-                const int indx_r = min(bin, AG_NEINT);  // r <= 8 here
-                const float S = sol_s[indx_r];
+                const float S = bh ? rs_s[min(bin, AG_NEINT)].y : rsv.y;   // sol_fn[min(bin, 1024)]
                 // desolvation (main.cpp:3375-3390): slot m gains
                 // (sp[m] vol + volp[m] solq) S per pair, i.e. sp[m] sum(vol S) +
                 // volp[m] sum(solq S) over the point's pairs: two sums per point

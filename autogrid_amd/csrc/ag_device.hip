@@ -508,14 +508,21 @@ __device__ __forceinline__ void brick_lane(int tid, int &lx, int &ly, int &lz) {
     }
 }
 
-// CUBE selects the lane layout inside the 8x8x4 brick (brick_lane).  PAGED
-// false: a brick whose candidates (or donor H) do not fit one LDS list sets
-// the status word and is left to the PAGED form, which the host runs for the
-// whole map set when that word is set (the lists' pages then live in
-// registers beside the accumulators, with spills the common case should not
-// pay for).
-template <int NPLAIN, bool CUBE, bool PAGED>
+// CUBE selects the lane layout inside the 8x8x4 brick (brick_lane).  FORM:
+//   kNearPlain   neither the tree build's queries nor a constriction count
+//                (their branches compiled out);
+//   kNearGeneral every map-set option;
+//   kNearPaged   every option, and candidate / donor-H lists longer than
+//                one LDS page.
+// The unpaged forms leave a brick whose candidates (or donor H) do not fit
+// one LDS list: they set the status word, and the host runs the whole map
+// set again with the paged form (whose page loops keep state beside the
+// accumulators, with spills the common case should not pay for).
+enum : int { kNearPlain = 0, kNearGeneral = 1, kNearPaged = 2 };
+template <int NPLAIN, bool CUBE, int FORM>
 __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_eu(5))) k_near(NearParams p) {
+    constexpr bool PAGED = FORM == kNearPaged;
+    constexpr bool GENERAL = FORM != kNearPlain;
     __shared__ int cand[kCandCap];
     __shared__ int hcand[kHCap];
     __shared__ double hpos[kHCap][3];
@@ -568,7 +575,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     // (the far kernel's coordinates of the point and the tree build's FP32
     // query point are formed where they are used, in rare branches of the
     // candidate loop, so they hold no registers across it)
-    const int bh = p.bhtree;
+    const int bh = GENERAL ? p.bhtree : 0;
     const float ccf = (float)p.ccut, ccf2 = ccf * ccf;
     bool collide = false;
 
@@ -738,7 +745,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
     };
     float e_near = 0.f, d_acc = 0.f, dv_acc = 0.f, dq_acc = 0.f;
     int ccount = 0;
-    const bool want_cmap = p.cmap != nullptr;
+    const bool want_cmap = GENERAL && p.cmap != nullptr;
     const float *tab = p.tab;
     const bool want_e = p.e_slot >= 0;
     // cos and sin of 1.24791 rad, the disordered-hydroxyl offset angle
@@ -1734,8 +1741,10 @@ static int hb_case(int mh, int ah) {
 static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStream_t st) {
 #define AG_NEAR(NP, CB)                                                                 \
     do {                                                                                \
-        if (p.paged) k_near<NP, CB, true><<<grid, kNearThreads, 0, st>>>(np);           \
-        else k_near<NP, CB, false><<<grid, kNearThreads, 0, st>>>(np);                  \
+        if (p.paged) k_near<NP, CB, kNearPaged><<<grid, kNearThreads, 0, st>>>(np);     \
+        else if (np.bhtree || np.cmap)                                                  \
+            k_near<NP, CB, kNearGeneral><<<grid, kNearThreads, 0, st>>>(np);            \
+        else k_near<NP, CB, kNearPlain><<<grid, kNearThreads, 0, st>>>(np);             \
     } while (0)
     if (p.tune.cube) {
         if (p.nplain == 4) AG_NEAR(4, true);

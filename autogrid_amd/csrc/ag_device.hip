@@ -509,8 +509,8 @@ __device__ __forceinline__ void brick_lane(int tid, int &lx, int &ly, int &lz) {
 }
 
 // CUBE selects the lane layout inside the 8x8x4 brick (brick_lane).  FORM:
-//   kNearPlain   neither the tree build's queries nor a constriction count
-//                (their branches compiled out);
+//   kNearPlain   neither the tree build's queries, a constriction count
+//                nor disordered hydroxyls (their branches compiled out);
 //   kNearGeneral every map-set option;
 //   kNearPaged   every option, and candidate / donor-H lists longer than
 //                one LDS page.
@@ -902,7 +902,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 if (cut || !p.has_affinity) continue;
                 const int meta = ameta;
                 const int type = meta & 0xff, hb = (meta >> 8) & 0xff, rexp = (meta >> 16) & 0xff;
-                const bool dis = (meta >> 24) & 1;
+                const bool dis = GENERAL && ((meta >> 24) & 1);
                 if (p.disorder_h && dis && type == p.hydrogen_type) continue;
                 float racc = 1.f, rdon = 1.f, Hramp = 1.f;
                 // directional classes only: the unit vector to the point
@@ -1574,6 +1574,7 @@ struct ag_plan {
     double coeff_estat = 0.0;
     int far_grid = 0;                  // persistent far-kernel grid
     bool paged = false;                // near kernel in its paged form (a brick overflowed one LDS list)
+    bool any_disorder = false;         // some atom flagged as a disordered hydroxyl
     // captured map set (persistent mode): the cell list and near field, then
     // the far field of conformations [0, far_split) and of [far_split,
     // nconf), as graphs launched back to back (the second far graph is
@@ -1742,7 +1743,7 @@ static void launch_near(const ag_plan &p, int grid, const NearParams &np, hipStr
 #define AG_NEAR(NP, CB)                                                                 \
     do {                                                                                \
         if (p.paged) k_near<NP, CB, kNearPaged><<<grid, kNearThreads, 0, st>>>(np);     \
-        else if (np.bhtree || np.cmap)                                                  \
+        else if (np.bhtree || np.cmap || p.any_disorder)                                \
             k_near<NP, CB, kNearGeneral><<<grid, kNearThreads, 0, st>>>(np);            \
         else k_near<NP, CB, kNearPlain><<<grid, kNearThreads, 0, st>>>(np);             \
     } while (0)
@@ -1959,6 +1960,7 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
     int *d_hbond = upload(st, p.hbond, hbond, nn);
     int *d_rexp = upload(st, p.rexp, rexp, nn);
     unsigned char *d_dis = upload(st, p.disorder, dis, nn);
+    p.any_disorder = std::any_of(dis, dis + N, [](unsigned char v) { return v != 0; });
     if (n > 1) upload(st, p.atom_conf_d, atom_conf, nn);
     p.src = AtomSrc{d_coord, d_charge, d_vol, d_solpar, d_rvec, d_rvec2, d_type, d_hbond, d_rexp, d_dis,
                     ms->coeff_estat, ms->solpar_q};

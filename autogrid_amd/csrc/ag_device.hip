@@ -807,8 +807,8 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 float inv_r = __builtin_amdgcn_rsqf(r2c);
                 float r = r2c * inv_r;
                 const float g = r * 128.f;
-                int bin;
-                bool cut;
+                int bin = (int)g;
+                bool cut = r > 8.0f;
                 if (fabsf(g - rintf(g)) < kGuardBins) {
                     double rdx, rdy, rdz;
                     const double r64 = ref_dist(ax, ay, az, cx, cy, cz, rdx, rdy, rdz);
@@ -816,9 +816,6 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                     cut = r64 > 8.0;
                     r = (float)r64;
                     inv_r = __builtin_amdgcn_rcpf(fmaxf(r, 1.0e-6f));
-                } else {
-                    bin = (int)g;
-                    cut = r > 8.0f;
                 }
                 // the tree build's radius queries (bhtree.cpp:341-362): FP32 point
                 // and atom, each component within the radius, then the rounded
@@ -873,10 +870,8 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 }
                 // electrostatics of pairs the far kernel leaves out
                 if (want_e) {
-                    bool near_e;
-                    if (g < 1022.f) near_e = true;
-                    else if (g > 1027.f) near_e = false;
-                    else {
+                    bool near_e = g < 1022.f;
+                    if (g >= 1022.f && g <= 1027.f) {
                         // the far kernel's point coordinates: run centre, offset in the run, y, z
                         int qx, qy, qz;
                         point_index(qx, qy, qz);
@@ -885,11 +880,11 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                                           (float)(qy - ney) * s128, (float)(qz - nez) * s128, far_bias(s128)) <
                                  4u * kFarBinMin;
                     }
-                    if (near_e) {
-                        // 1/max(r, 0.5) with r clamped at APPROX_ZERO
-                        const float inv_rmax = fminf(inv_r, 2.f);
-                        e_near += unif(a.qe) * inv_rmax * rsv.x;
-                    }
+                    // 1/max(r, 0.5) with r clamped at APPROX_ZERO; adding 0 leaves
+                    // the sum as it is (it is never -0)
+                    const float inv_rmax = fminf(inv_r, 2.f);
+                    const float te = unif(a.qe) * inv_rmax * rsv.x;
+                    e_near += near_e ? te : 0.f;
                 }
                 // Loop 2 of the tree build takes the atoms its 8 A query returns,
                 // at their FP32 distance (main.cpp:3019-3028)
@@ -905,8 +900,9 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 const bool dis = GENERAL && ((meta >> 24) & 1);
                 if (p.disorder_h && dis && type == p.hydrogen_type) continue;
                 float racc = 1.f, rdon = 1.f, Hramp = 1.f;
-                // directional classes only: the unit vector to the point
-                if (hb == AG_HB_D1 || hb == AG_HB_A2 || hb == AG_HB_A1) {
+                // directional classes only (D1, A1, A2): the unit vector to the point
+                constexpr unsigned kDirClasses = (1u << AG_HB_D1) | (1u << AG_HB_A1) | (1u << AG_HB_A2);
+                if ((kDirClasses >> hb) & 1u) {
                     const float dn0 = dxf * inv_r, dn1 = dyf * inv_r, dn2 = dzf * inv_r;
                     const float rv0 = unif(a.rv[0]), rv1 = unif(a.rv[1]), rv2 = unif(a.rv[2]);
                     const int aidx = aindex;

@@ -2816,13 +2816,16 @@ void maps_plain(ag_ctx *c, const ag_receptor *rec, const ag_grid *grid, const ag
         if (ms->floating_grid) d_fmap = (float *)base;
         if (ms->constriction_grid) d_cmap = (int32_t *)(base + pbytes);
     }
-    plan_execute(p.get(), d_maps, d_fmap, d_cmap);
-    if (!outputs_on_device) {
-        HIP_CHECK(hipMemcpyAsync(maps_out, d_maps, bytes, hipMemcpyDeviceToHost, c->stream));
-        if (d_fmap) HIP_CHECK(hipMemcpyAsync(fmap, d_fmap, pbytes, hipMemcpyDeviceToHost, c->stream));
-        if (d_cmap) HIP_CHECK(hipMemcpyAsync(cmap, d_cmap, pbytes, hipMemcpyDeviceToHost, c->stream));
-        HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (outputs_on_device) {
+        plan_execute(p.get(), d_maps, d_fmap, d_cmap);
+        return;
     }
+    // host outputs: the maps other than e go to the host while the far
+    // kernel runs, the e map after it
+    float *const mo[1] = {maps_out};
+    float *const fo[1] = {fmap};
+    int32_t *const co[1] = {cmap};
+    plan_execute_host(p.get(), d_maps, d_fmap, d_cmap, mo, d_fmap ? fo : nullptr, d_cmap ? co : nullptr);
 }
 
 static bool is_pinned(const void *ptr) {

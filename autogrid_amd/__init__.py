@@ -711,10 +711,11 @@ def bond_vectors_batch(states: list, n_threads: int = 0) -> list:
     counts = np.array([st.n_atoms for st in states], np.int64)
     offs = np.concatenate([[0], np.cumsum(counts)])
     total = int(offs[-1])
-    rexp_all = np.zeros(total, np.int32)
-    rv_all = np.zeros((total, 3))
-    rv2_all = np.zeros((total, 3))
-    dis_all = np.zeros(total, np.uint8)
+    # every element is written by the library (it clears each output first)
+    rexp_all = np.empty(total, np.int32)
+    rv_all = np.empty((total, 3))
+    rv2_all = np.empty((total, 3))
+    dis_all = np.empty(total, np.uint8)
     b_rexp, b_rv, b_rv2, b_dis = (a.ctypes.data for a in (rexp_all, rv_all, rv2_all, dis_all))
     recs = (_abi.ReceptorAddr * n)()
     outs = (_abi.BondOutAddr * n)()

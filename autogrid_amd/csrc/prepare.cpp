@@ -126,10 +126,31 @@ static bool distance_le(const double *a, const double *b, double limit, double d
     return d <= limit;
 }
 
+// Atoms of the index window [from, to] (other than `self`) whose squared
+// distance from atom `c` could pass a distance_le test of at most `lim`: a
+// branch-free pass over the window, in index order.  The squared distance of
+// any atom left out exceeds lim^2 (1 + 1e-9), so its hypot length exceeds
+// lim and distance_le would reject it; the callers run distance_le on the
+// rest in the same order, so they find what a scan of the whole window finds.
+static int window_candidates(const double *coord, int c, int self, int from, int to, double lim, int *out) {
+    const double cx = coord[3 * (size_t)c], cy = coord[3 * (size_t)c + 1], cz = coord[3 * (size_t)c + 2];
+    const double l2 = lim * lim * (1.0 + 1e-9);
+    int k = 0;
+    for (int ib = from; ib <= to; ib++) {
+        const double dx = coord[3 * (size_t)ib] - cx, dy = coord[3 * (size_t)ib + 1] - cy,
+                     dz = coord[3 * (size_t)ib + 2] - cz;
+        out[k] = ib;
+        k += (dx * dx + dy * dy + dz * dz <= l2) & (ib != self);
+    }
+    return k;
+}
+
 struct Bonds {
     std::vector<int> n;
     std::vector<int> b;  // [atom][kMaxNbonds]
+    // the list is only kept for disorder_h; without it no bond is added
     explicit Bonds(int natoms) : n(natoms, 0), b((size_t)natoms * kMaxNbonds, -1) {}
+    Bonds() = default;
     void add(int i, int j) {  // bondmanager.cpp:44-93
         if (n[i] >= kMaxNbonds || n[j] >= kMaxNbonds) fail(AG_ERR_BAD_INPUT, "could not add bond");
         b[(size_t)i * kMaxNbonds + n[i]++] = j;
@@ -151,17 +172,19 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int
     memset(rvector, 0, sizeof(double) * 3 * (size_t)n);
     memset(rvector2, 0, sizeof(double) * 3 * (size_t)n);
     memset(disorder, 0, (size_t)n);
-    Bonds bonds(n);
+    Bonds bonds = disorder_h ? Bonds(n) : Bonds();
     auto C = [&](int a) { return coord + 3 * (size_t)a; };
     for (int ia = 0; ia < n; ia++) {
         const int from = ia - 20 > 0 ? ia - 20 : 0;
         const int to = ia + 20 < n - 1 ? ia + 20 : n - 1;
         double *rv = rvector + 3 * (size_t)ia, *rv2 = rvector2 + 3 * (size_t)ia;
+        int cand[41];
         if (hbond[ia] == AG_HB_D1) {
             // Donor hydrogen: every atom within 1.378 A in the window sets the
             // bond vector (the last one wins) and rexp.
-            for (int ib = from; ib <= to; ib++) {
-                if (ib == ia) continue;
+            const int nc = window_candidates(coord, ia, ia, from, to, 1.378, cand);
+            for (int q = 0; q < nc; q++) {
+                const int ib = cand[q];
                 double d[3], rd;
                 if (!distance_le(C(ia), C(ib), 1.378, d, rd)) continue;
                 for (int i = 0; i < 3; i++) rv[i] = d[i];
@@ -174,8 +197,9 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int
             }
         } else if (hbond[ia] == AG_HB_A2) {
             int nbond = 0, i1 = -1, i2 = -1;
-            for (int ib = from; ib <= to; ib++) {
-                if (ib == ia) continue;
+            const int nc = window_candidates(coord, ia, ia, from, to, 1.9, cand);
+            for (int q = 0; q < nc; q++) {
+                const int ib = cand[q];
                 double dc[3], rd;
                 const double lim = (type[ib] == hydrogen || type[ib] == nonHB_hydrogen) ? 1.3 : 1.9;
                 if (!distance_le(C(ia), C(ib), lim, dc, rd)) continue;
@@ -197,8 +221,11 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int
                 // Carbonyl O=C-X: rvector = O - C, rvector2 = (O-C) x (C-X).
                 sub_normalize(rv, C(ia), C(i1));
                 if (recipes) recipes[ia] = BondRecipe{BondRecipe::kCarbonyl, i1, -1, -1, 0};
-                for (int j = from; j <= to; j++) {
-                    if (j == i1 || j == ia) continue;
+                int cj[41];
+                const int ncj = window_candidates(coord, i1, i1, from, to, 1.7, cj);
+                for (int q = 0; q < ncj; q++) {
+                    const int j = cj[q];
+                    if (j == ia) continue;
                     double d[3], rdcx;
                     if (!distance_le(C(i1), C(j), type[j] == hydrogen ? 1.3 : 1.7, d, rdcx)) continue;
                     if (recipes) recipes[ia].b = j;
@@ -234,8 +261,9 @@ void build_bond_vectors(const ag_setup &s, int n, const double *coord, const int
             int nbond = 0;
             double bondv[3][3];
             int bonded[3] = {-1, -1, -1};
-            for (int ib = from; ib <= to; ib++) {
-                if (ib == ia) continue;
+            const int nc = window_candidates(coord, ia, ia, from, to, 1.7, cand);
+            for (int q = 0; q < nc; q++) {
+                const int ib = cand[q];
                 double rd;
                 const double lim = (type[ib] == hydrogen || type[ib] == nonHB_hydrogen) ? 1.3 : 1.7;
                 if (!distance_le(C(ia), C(ib), lim, bondv[nbond], rd)) continue;

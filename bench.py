@@ -19,6 +19,9 @@ Also reported on the same JSON line, each with an `*_is` label:
   accuracy      the GPU maps against the reference autogrid4's .map text
                 (every value) and the FP64 oracle (3 z-planes); W_A; the
                 Vina-potential maps against the oracle (3 z-planes).
+  map_set_roofline
+                the whole map set's FP32 work, 16 W_E + F_A W_A FLOPs
+                (W_A from the CPU leg), over its kernels' time.
   cpu_baseline  the reference autogrid4 (built from its sources with
                 OpenMP) on the whole workload on this host's cores, and a
                 labelled z-slab of the large workload scaled by point count.
@@ -703,6 +706,23 @@ def main():
     if not args.no_cpu_baseline and args.workload == "full":
         cpu = cpu_leg(st, gpu_maps, info["maps"], large=not args.no_large, vina=vina_state)
     value = ms_per_step / 1000.0
+    # the whole map set's FP32 work over its kernels' time, when the CPU leg
+    # has the exact count of pairs within 8 A: 16 FLOPs per point-atom pair
+    # of the e pass, 16 + 4 per affinity map (+2 with a d map) per pair of
+    # the affinity pass
+    map_set_roofline = None
+    acc = cpu.get("accuracy") or {}
+    if acc.get("W_A") is not None:
+        n_aff = sum(1 for k in range(st.n_maps) if k not in (st.elec_index, st.dsolv_index))
+        f_a = 16 + 4 * n_aff + (2 if st.dsolv_index >= 0 else 0)
+        ms_flops = 16.0 * P * N + f_a * acc["W_A"]
+        t_k = sum(cell_ms) / len(cell_ms) + sum(near_ms) / len(near_ms) + far_avg
+        ach = ms_flops / (t_k * 1e-3) / 1e12
+        map_set_roofline = {"achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": ach / FP32_PEAK_TFLOPS, "frac_nonpacked_peak": ach / FP32_PEAK_NONPACKED_TFLOPS,
+                            "flops": ms_flops, "kernels_ms": t_k,
+                            "flops_rule": f"16 W_E + F_A W_A, W_E = P*N, W_A = pairs within 8 A (oracle count), "
+                                          f"F_A = 16 + 4*{n_aff} maps" + (" + 2 (d map)" if st.dsolv_index >= 0 else "")}
     out = {
         "metric": "map-set wall s (5k atoms,126^3,14 maps); batched & fwd+adj vs plain GPU",
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -723,6 +743,7 @@ def main():
                      "avg_launch_ms": far_avg,
                      "traffic": profiled_traffic() if args.workload == "full" else None,
                      "traffic_source": TRAFFIC_PROFILE if args.workload == "full" else None},
+        "map_set_roofline": map_set_roofline,
         "kernels_ms": {"cell_list": sum(cell_ms) / len(cell_ms), "near": sum(near_ms) / len(near_ms),
                        "far": far_avg},
         "pair_evals_per_s": far_pairs / (far_avg * 1e-3),

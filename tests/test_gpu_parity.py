@@ -94,6 +94,22 @@ def test_repeat_is_bitwise_identical(ctx, tmp_path):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+@pytest.mark.parametrize("name", ["small", "x1hpv_CASE0"])
+def test_near_kernel_forms_agree(ctx, name, tmp_path):
+    """A map set without a constriction grid, tree-build queries or
+    disordered hydroxyls runs the near kernel's plain form (those branches
+    compiled out); asking for a constriction grid runs the general form.
+    The maps they share must be the same bits: the forms differ in branches
+    only, not in arithmetic."""
+    import dataclasses
+    st, _ = parity.load_case(name, str(tmp_path))
+    assert not st.constriction_grid and not st.bhtree and not np.any(st.disorder)
+    plain = ctx.maps_plain(st)
+    general = ctx.maps_plain(dataclasses.replace(st, constriction_grid=1, constriction_cutoff=6.5, _keep=[]))
+    assert ctx.last_cmap is not None and np.any(ctx.last_cmap > 0)
+    assert np.array_equal(plain.view(np.uint32), general.view(np.uint32))
+
+
 def test_context_reuse_across_sizes(ctx, tmp_path):
     """Scratch grows and is reused: big, small, big again give the same maps."""
     big, _ = parity.load_case("small", str(tmp_path / "a"))

@@ -651,7 +651,7 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
                 const int pos = block_compact(ok, wave_tot, tot);
                 const int base = ncand_s;
                 const int cj = base + pos - cpage;
-                if (cpage >= 0 && ok && cj >= 0 && cj < kCandCap) cand[cj] = slot;
+                if (cpage >= 0 && ok && cj >= 0 && cj < kCandCap) cand[cj] = slot * (int)sizeof(NearAtom);
                 int htot;
                 const int hpos_i = block_compact(is_h, wave_tot, htot);
                 const int hbase = nh_s;
@@ -774,28 +774,40 @@ __global__ void __launch_bounds__(kNearThreads) __attribute__((amdgpu_waves_per_
         // wave holds 64 candidate indices in one register and loads the
         // next record's coordinates, class and index one candidate ahead,
         // which also brings both of its cache lines into the scalar cache.
-        const AG_CONST NearAtom *catoms = as_const(p.atoms);
+        // The list holds byte offsets of the records (below 4 GiB, checked
+        // when the plan is built), so a record's address is the array base
+        // plus an unsigned 32-bit scalar, which the scalar loads take as
+        // their offset operand: no 64-bit multiply-add per record.
+        const AG_CONST char *cbase = reinterpret_cast<const AG_CONST char *>(as_const(p.atoms));
+        const auto rec = [&](unsigned off) {
+            return reinterpret_cast<const AG_CONST NearAtom *>(cbase + off);
+        };
         for (int j0 = 0; j0 < cn; j0 += 64) {
             const int vcand = cand[min(j0 + wlane, cn - 1)];
             const int jn = min(64, cn - j0);
-            int idx = __builtin_amdgcn_readlane(vcand, 0);
-            double nx = catoms[idx].x, ny = catoms[idx].y, nz = catoms[idx].z;
-            int nmeta = catoms[idx].meta, nindex = catoms[idx].index;
+            unsigned off = (unsigned)__builtin_amdgcn_readlane(vcand, 0);
+            double nx = rec(off)->x, ny = rec(off)->y, nz = rec(off)->z;
+            int nmeta = rec(off)->meta, nindex = rec(off)->index;
             for (int jj = 0; jj < jn; jj++) {
-                const AG_CONST NearAtom &a = catoms[idx];
+                // this record's address, formed once: its fields load from it
+                // with immediate offsets in whichever branch needs them
+                const AG_CONST NearAtom *ap = rec(off);
+                asm volatile("" : "+s"(ap));
+                const AG_CONST NearAtom &a = *ap;
                 const double ax = nx, ay = ny, az = nz;
                 const int ameta = nmeta, aindex = nindex;
                 const unsigned hw = (unsigned)__builtin_amdgcn_readlane(hcw, (ameta >> 8) & 63);
-                idx = __builtin_amdgcn_readlane(vcand, min(jj + 1, jn - 1));
+                unsigned noff = (unsigned)__builtin_amdgcn_readlane(vcand, min(jj + 1, jn - 1));
                 // the next loads may issue only once this record has arrived:
                 // scalar loads complete out of order, so the wait for this
                 // record would otherwise also wait for the next one
-                asm volatile("" : "+s"(idx) : "s"(ameta), "s"(aindex), "s"(ax), "s"(ay), "s"(az));
-                nx = catoms[idx].x;
-                ny = catoms[idx].y;
-                nz = catoms[idx].z;
-                nmeta = catoms[idx].meta;
-                nindex = catoms[idx].index;
+                asm volatile("" : "+s"(noff) : "s"(ameta), "s"(aindex), "s"(ax), "s"(ay), "s"(az));
+                nx = rec(noff)->x;
+                ny = rec(noff)->y;
+                nz = rec(noff)->z;
+                nmeta = rec(noff)->meta;
+                nindex = rec(noff)->index;
+                off = noff;   // a still names this record
                 // geometry: FP64 difference as the reference forms it, FP32 after
                 const double dx64 = ax - cx, dy64 = ay - cy, dz64 = az - cz;
                 const float dxf = (float)dx64, dyf = (float)dy64, dzf = (float)dz64;
@@ -1962,6 +1974,9 @@ static ag_plan *plan_build(ag_ctx *c, int n, const ag_receptor *recs, const ag_g
                     ms->coeff_estat, ms->solpar_q};
     p.atom_cell.get(sizeof(int) * (size_t)N);
     p.slot_atom.get(sizeof(int) * (size_t)N);
+    // the near kernel's candidate lists hold 32-bit byte offsets of records
+    if ((unsigned long long)N * sizeof(NearAtom) > UINT32_MAX)
+        ag::fail(AG_ERR_UNSUPPORTED, "too many atoms in one plan for the near kernel (at most 44,739,242)");
     p.near_atoms.get(sizeof(NearAtom) * (size_t)N);
     p.status.get(sizeof(int));
 

@@ -162,6 +162,7 @@ struct ag_ctx {
     hipEvent_t cells_ev = nullptr;       // cell list done (orders stream2 after it)
     hipEvent_t near_ev = nullptr;        // near field of a map set done
     hipEvent_t far_ev = nullptr;         // first far-field graph of a batch done
+    hipEvent_t far2_ev = nullptr;        // second far-field graph of a batch done (on stream2)
     hipEvent_t chunk_ev[kMaxChunks + 1] = {};   // ordering between the two streams
     std::shared_ptr<DevPool> pool = std::make_shared<DevPool>();   // blocks of destroyed plans
     DevBuf maps;                       // output staging of ag_maps_plain

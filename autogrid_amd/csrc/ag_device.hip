@@ -1638,6 +1638,7 @@ ag_status context_create(ag_ctx **out, int device) {
     HIP_CHECK(hipEventCreateWithFlags(&c->cells_ev, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&c->near_ev, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&c->far_ev, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&c->far2_ev, hipEventDisableTiming));
     for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
     for (auto &e : c->chunk_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
@@ -1669,6 +1670,7 @@ void context_destroy(ag_ctx *c) {
     if (c->cells_ev) (void)hipEventDestroy(c->cells_ev);
     if (c->near_ev) (void)hipEventDestroy(c->near_ev);
     if (c->far_ev) (void)hipEventDestroy(c->far_ev);
+    if (c->far2_ev) (void)hipEventDestroy(c->far2_ev);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -2594,6 +2596,12 @@ void plan_destroy(ag_plan *p) {
 // the persistent far field in two parts (conformations split at
 // far_split).  Either way the context's near_ev is recorded once the near
 // field is done; a batch records far_ev after its first far-field part.
+// The second part goes on stream2 behind near_ev, submitted after the
+// first: the first part's persistent grid fills the GPU, and the second
+// part's workgroups take the CUs as the first part's drain, instead of
+// starting after its last one.  The parts write different conformations
+// and take separate work counters; the main stream joins the second part
+// before ev[3].
 static hipGraphExec_t capture(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_cmap, int part, int c0, int c1,
                               int queue_base) {
     hipStream_t st = p->ctx->stream;
@@ -2638,7 +2646,12 @@ static void launch_mapset(ag_plan *p, float *d_maps, float *d_fmap, int32_t *d_c
     HIP_CHECK(hipEventRecord(c->near_ev, st));
     HIP_CHECK(hipGraphLaunch(p->graph_far, st));
     HIP_CHECK(hipEventRecord(c->far_ev, st));
-    if (p->graph_far2) HIP_CHECK(hipGraphLaunch(p->graph_far2, st));
+    if (p->graph_far2) {
+        HIP_CHECK(hipStreamWaitEvent(c->stream2, c->near_ev, 0));
+        HIP_CHECK(hipGraphLaunch(p->graph_far2, c->stream2));
+        HIP_CHECK(hipEventRecord(c->far2_ev, c->stream2));
+        HIP_CHECK(hipStreamWaitEvent(st, c->far2_ev, 0));
+    }
     HIP_CHECK(hipEventRecord(c->ev[3], st));
 }
 

@@ -161,7 +161,8 @@ typedef struct ag_setup {
 } ag_setup;
 
 /* ------------------------------------------------------------------------ */
-/* Context: one GPU, one stream, device scratch reused across calls.        */
+/* Context: one GPU, its compute, second compute and copy streams, and     */
+/* device scratch reused across calls.                                      */
 /* ------------------------------------------------------------------------ */
 typedef struct ag_ctx ag_ctx;
 
